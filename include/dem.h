@@ -1,0 +1,223 @@
+/*
+ * dem.h - C ABI of the MI355X (gfx950) DEM time step.
+ *
+ * This library stands in for the compute half of the reference's wgpu host
+ * (WGPUComputeProg, src/wgpu_prog.rs:677-696): it owns the device buffers of one
+ * scene, runs the two-pass time step (2D_LOM.wgsl + 2D_Simulation.wgsl) as HIP
+ * kernels, and converts between the reference's host layouts (src/state.rs:69-93)
+ * and its own device layout on upload and download.
+ *
+ * Conventions
+ *   - Every entry point returns DEM_OK (0) or a negative DEM_E* code; none throws
+ *     or aborts across the ABI.  dem_last_error() returns a thread-local message
+ *     describing the last failure on the calling thread.
+ *   - One dem_ctx is bound to one HIP stream.  Calls on one ctx must be
+ *     serialised by the caller; distinct ctxs may be used from different threads.
+ *   - Arrays handed in through dem_state_view are caller-owned and are copied;
+ *     the library keeps no pointer to them after the call returns.
+ */
+#ifndef PSIM_DEM_H
+#define PSIM_DEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DEM_OK          0
+#define DEM_EINVAL     -1   /* bad argument */
+#define DEM_ENOMEM     -2   /* device or host allocation failed */
+#define DEM_EDEVICE    -3   /* HIP runtime error */
+#define DEM_ECAPACITY  -4   /* a fixed capacity (contacts, bonds, slots) was exceeded */
+#define DEM_ESTATE     -5   /* call out of order (e.g. step before upload) */
+#define DEM_EBUSY      -6   /* serving: no room for the scene right now */
+
+/* Physics uniform: 29 words in the reference's order.
+ * Packed by Settings::collision_settings (src/settings.rs:2086-2119) and read as
+ * `struct Settings` by 2D_LOM.wgsl:26-56 and 2D_Simulation.wgsl:34-64. */
+typedef struct dem_uniform {
+    int32_t walls;                      /*  0 (the step applies walls regardless, 2D_LOM.wgsl:102) */
+    float   hor_bound;                  /*  1 full box width */
+    float   vert_bound;                 /*  2 full box height */
+    int32_t round_bounds;               /*  3 circular wall instead of the box */
+    float   bound_radius;               /*  4 */
+    float   wall_friction;              /*  5 not read by the step */
+    int32_t gravity;                    /*  6 */
+    int32_t planet_mode;                /*  7 gravity toward a point */
+    int32_t bonds;                      /*  8 bond law: 0 none, 1 normal, 2 linear contact, 3 parallel */
+    int32_t collisions;                 /*  9 narrow phase on/off */
+    float   friction_coefficient;       /* 10 */
+    float   gravity_acc;                /* 11 multiplier on 9.81 */
+    float   bond_normal_stiffness;      /* 12 */
+    int32_t bonds_tear;                 /* 13 */
+    float   bond_normal_strength;       /* 14 */
+    float   contact_damping;            /* 15 force factor of bond law 2 */
+    float   bond_damping;               /* 16 not read by the step */
+    float   drag;                       /* 17 not read by the step */
+    float   bond_shear_strength;        /* 18 */
+    float   dT;                         /* 19 time step */
+    float   bond_shear_stiffness;       /* 20 */
+    float   gravity_x;                  /* 21 planet-mode centre (mouse) */
+    float   gravity_y;                  /* 22 */
+    int32_t mouse_gravity;              /* 23 */
+    float   moment_contribution_factor; /* 24 */
+    int32_t collision_interval;         /* 25 narrow phase every k-th step */
+    int32_t local_damping;              /* 26 not read by the step */
+    float   local_damping_alpha;        /* 27 not read by the step */
+    int32_t particles;                  /* 28 active particle count N */
+} dem_uniform;
+
+/* Material record, 6 floats (2D_Simulation.wgsl:66-73, settings.rs:266). */
+typedef struct dem_material {
+    float red, green, blue, density, normal_stiffness, shear_stiffness;
+} dem_material;
+
+/* Broad-phase grid in the reference's GridInfo convention (2D_LOM.wgsl:19-24,
+ * wgpu_prog.rs:698-717): w x h cells of side cell_size centred on the origin,
+ * column 0 at x = -cell_size*w/2, row 0 at y = +cell_size*h/2 (rows top-down).
+ * Positions outside the grid clamp to the edge cells.  cell_size must be at
+ * least twice the largest radius.  cell_cap is kept for layout compatibility;
+ * the counting sort has no per-cell capacity. */
+typedef struct dem_grid {
+    float   cell_size;
+    int32_t cell_cap;
+    int32_t w;
+    int32_t h;
+} dem_grid;
+
+typedef struct dem_params {
+    dem_uniform         u;
+    const dem_material* materials;      /* n_materials records, copied */
+    int32_t             n_materials;
+    dem_grid            grid;
+    int32_t use_f64;           /* reference "$ F64" variant: distances in f64 (2D_Simulation.wgsl:264-284) */
+    int32_t record_data;       /* maintain data[7] per particle (Σfn, Σft, ΣM, 0, v*x, v*y, ω*) */
+    int32_t reorder_interval;  /* re-sort particle storage by cell every k broad phases; 0 = never */
+    int32_t block_size;        /* threads per block for particle kernels; 0 = 256 */
+} dem_params;
+
+/* Caller-owned arrays in the reference host layouts (state.rs:69-93).
+ *   pos, vel, acc      2N floats (x, y interleaved)
+ *   rot, rot_vel, rot_acc, radii   N floats
+ *   forces             6N floats {x, y, rot, delX, delY, delRot}; only x, y, rot are read
+ *   fixity             6N int32 {x_vel, y_vel, rot_vel, x_vel_2, y_vel_2, rot_vel_2}
+ *   material_pointers  N int32
+ *   bonds              3*n_bonds int32 {index (1 intact, -9/-10/-11 broken), angle bits, length bits}
+ *   bond_strengths     optional 2*n_bonds floats {normal, shear} per bond; NULL = the uniform's
+ *   contacts           contact_slots*6*N words {a, b, tangent_force, bond_tangent_force, theta_b, bonded}
+ *                      per particle; b == -1 marks an empty slot
+ *   data               optional 7N floats
+ * Download writes each particle's occupied slots sorted by partner id b, then
+ * fills the rest with -1 words. */
+typedef struct dem_state_view {
+    int32_t  n;
+    float*   pos;
+    float*   vel;
+    float*   acc;
+    float*   rot;
+    float*   rot_vel;
+    float*   rot_acc;
+    float*   forces;
+    float*   radii;
+    int32_t* fixity;
+    int32_t* material_pointers;
+    int32_t* bonds;
+    int32_t  n_bonds;
+    float*   bond_strengths;
+    float*   contacts;
+    int32_t  contact_slots;
+    float*   data;
+} dem_state_view;
+
+typedef struct dem_ctx dem_ctx;
+
+/* Thread-local description of the last error on this thread ("" if none). */
+const char* dem_last_error(void);
+
+/* Library version string. */
+const char* dem_version(void);
+
+/* Number of visible HIP devices (0 if none).  Never fails. */
+int dem_device_count(void);
+
+/* Stands in for WGPUComputeProg::new (wgpu_prog.rs:720-1125) minus scene setup:
+ * allocates device buffers for up to `capacity` particles and `bond_capacity`
+ * bonds on the current HIP device. */
+int dem_create(const dem_params* params, int32_t capacity, int32_t bond_capacity, dem_ctx** out);
+
+/* Frees every device buffer of the ctx. */
+int dem_destroy(dem_ctx* ctx);
+
+/* Uniform / material / grid update: stands in for the collision_settings and
+ * material_buffer updateUniform calls (client.rs:1129-1135, 1189-1195). */
+int dem_set_params(dem_ctx* ctx, const dem_params* params);
+
+/* Host -> device copy of a whole scene: stands in for WGPUComputeProg::restore
+ * (wgpu_prog.rs:1235-1262) and resize_buffers (wgpu_prog.rs:1214-1233). */
+int dem_upload_state(dem_ctx* ctx, const dem_state_view* state);
+
+/* n time steps: stands in for WGPUComputeProg::compute with gen_per_frame = n
+ * (wgpu_prog.rs:1420-1473).  Asynchronous on the ctx's stream. */
+int dem_step(dem_ctx* ctx, int32_t n);
+
+/* Device -> host copy: stands in for WGPUComputeProg::update_state
+ * (wgpu_prog.rs:1166, state.rs:264-289).  Synchronous.  Any pointer in the view
+ * may be NULL to skip that array. */
+int dem_download_state(dem_ctx* ctx, dem_state_view* state);
+
+/* Waits for all work queued on the ctx's stream. */
+int dem_sync(dem_ctx* ctx);
+
+/* Observables computed on the device (replace full readback + State::get_datum,
+ * state.rs:708-737).  kinetic energy = Σ ½ m |v|² + ½ I ω² over active particles,
+ * accumulated in f64 in particle-id order. */
+int dem_kinetic_energy(dem_ctx* ctx, double* out);
+/* Number of bonds whose index word is negative (broken). */
+int dem_broken_bond_count(dem_ctx* ctx, int64_t* out);
+/* Counters since upload: [0] steps, [1] broad phases, [2] contact-list overflows,
+ * [3] bond-list overflows, [4] active contacts after the last step,
+ * [5] active bond endpoints after the last step, [6] storage re-sorts. */
+int dem_counters(dem_ctx* ctx, int64_t* out, int32_t n_out);
+
+/* Sum over the particles flagged in `select` (length N, nonzero = selected) of
+ * the reference's get_datum quantities (state.rs:708-737): out[10] =
+ * {mean x, mean y, mean v*x, mean v*y, mean rot, mean ω*, Σ|Σfn|, mean Σft,
+ *  mean ΣM, mean data3}; returns DEM_ESTATE if nothing is selected.
+ * Requires record_data. */
+int dem_get_datum(dem_ctx* ctx, const int32_t* select, double* out);
+
+/* Stage-level views for parity checks.  Each copies device arrays to the host
+ * after running the broad phase (stages 1-3) on the current positions.
+ *   cell_keys[N]     centre-cell key cy*w+cx of particle i (by particle id),
+ *                    w*h for inactive particles (r == 0)
+ *   sorted_ids[N]    particle ids in (cell, id) order
+ *   cell_start[w*h+1] first sorted slot of each cell
+ * Any output may be NULL. */
+int dem_debug_broad_phase(dem_ctx* ctx, int32_t* cell_keys, int32_t* sorted_ids, int32_t* cell_start);
+
+/* Contact lists of the last narrow phase, by particle id: counts[N] and
+ * partners[N*max_per] (sorted ascending, -1 padded).  Bond partners are not
+ * included. */
+int dem_debug_contacts(dem_ctx* ctx, int32_t max_per, int32_t* counts, int32_t* partners);
+
+/* Bond builder on the cell list: the rule of State::regen_bonds
+ * (state.rs:291-367) - pair (i, j), j > i, bonded iff
+ * sqrt((xj-xi)^2 + (yj-yi)^2) < (ri + rj) * 1.05, at most max_bonds per lower
+ * index i taken in increasing j - evaluated on the device over the current
+ * positions.  Writes up to bond_cap pairs (i, j) in (i, j) order and their
+ * count; returns DEM_ECAPACITY if more exist. */
+int dem_build_bond_pairs(dem_ctx* ctx, int32_t max_bonds, float factor,
+                         int32_t* pairs, int32_t bond_cap, int32_t* n_out);
+
+/* Timing of the last dem_step call measured with HIP events on the ctx's
+ * stream: total milliseconds and per-kernel milliseconds
+ * (out[0] total, out[1] force+drift kernel, out[2] cell scan, out[3] scatter,
+ *  out[4] re-sort), and the number of launches of each. */
+int dem_last_step_timing(dem_ctx* ctx, int32_t enable, double* ms_out, int64_t* launches_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSIM_DEM_H */

@@ -1,0 +1,110 @@
+// dem_device.h - device-side data layout shared by the kernels and the host context.
+//
+// One launch can advance several independent scenes ("batch").  Every scene owns a
+// contiguous range of particle slots, cells and bonds in the global pools; a block
+// never straddles two scenes (scene slot ranges are padded to the block size), so a
+// block reads its scene descriptor once.
+#pragma once
+#include <stdint.h>
+#include "../../include/dem.h"
+
+namespace psim {
+
+constexpr int KMAX = 16;         // contacts (unbonded) per particle
+constexpr int DATA_WORDS = 7;
+
+// per-particle flag bits (word M.x bits 16..31; bits 0..15 hold the material id)
+enum : uint32_t {
+    PF_KICK_X = 1u << 16,   // fixity.x_vel == 0: velocity integrates (2D_LOM.wgsl:88)
+    PF_KICK_Y = 1u << 17,
+    PF_KICK_R = 1u << 18,
+    PF_WALL_X = 1u << 19,   // fixity.x_vel != 1: walls act on x (2D_LOM.wgsl:119)
+    PF_WALL_Y = 1u << 20,
+    PF_WALL_R = 1u << 21,   // fixity.rot_vel != 1 (only a dead write in the reference)
+    PF_EXT = 1u << 22,      // nonzero external force / moment
+    PF_INACTIVE = 1u << 23, // r == 0 or beyond the active count
+    PF_MAT_MASK = 0xffffu,
+};
+
+// per-step flags (one word per scene and step)
+enum : uint32_t {
+    SF_SIM = 1u,        // run the Sim half (forces, first half-kick)
+    SF_LOM = 2u,        // run the LOM half of the next step (second half-kick, drift, walls)
+    SF_NARROW = 4u,     // the Sim half runs the narrow phase (collisions && coll_cont[0])
+    SF_BIN = 8u,        // the LOM half bins particles for the next narrow phase
+    SF_DELOVR = 16u,    // the LOM half keeps del_pos/del_rot (coll_cont[3] != 0)
+    SF_ACTIVE = 32u,    // scene takes part in this launch
+};
+
+struct SceneDev {
+    dem_uniform u;
+    int32_t p0, np;        // particle slots [p0, p0 + np)
+    int32_t c0;            // first cell; cells [c0, c0 + w*h], the last one collects inactive particles
+    int32_t w, h;
+    float cs, base_x, base_y;
+    int32_t b0, nb;        // bonds [b0, b0 + nb)
+    int32_t m0, nm;        // materials [m0, m0 + nm)
+    int32_t use_f64;
+    int32_t record;        // maintain data[7]
+    int32_t has_strength;  // per-bond strengths present
+    int32_t n_real;        // particles of the scene; slots [p0+n_real, p0+np) hold inactive padding
+};
+
+struct DevArrays {
+    // double-buffered neighbour-visible state (cur = read, nxt = written)
+    float4* P[2];      // {x, y, r, del_rot}
+    float2* D[2];      // {del_x, del_y}
+    float4* V;         // {vx, vy, w, rot}  (after a LOM half: the mid-step velocity)
+    float4* A;         // {ax, ay, alpha, 0}
+    int2* M;           // {material | flags, particle id within the scene}
+    float4* EXT;       // external {fx, fy, moment, 0}
+    float4* DATA0;     // {Σfn, Σft, ΣM, data3}
+    float4* DATA1;     // {v*x, v*y, w*, 0}
+    // contacts: slot-major [KMAX][cap]
+    int32_t* CC;       // count
+    int32_t* CP;       // partner id (scene-local)
+    float* CF;         // tangential force
+    int64_t cap;       // particle pool size (slot stride of CP / CF)
+    // bond endpoint records: CSR indexed by (scene p0 + particle id), each list sorted
+    // by partner id; records never move
+    int32_t* BOFF;     // [cap + 1]
+    int4* BR;          // {partner id, bond id, bonded, alive}
+    float4* BF;        // {ft, fbt, theta_b, 0}
+    int2* BS;          // per bond {index word, step of break}
+    float* BL;         // per bond rest length
+    float2* BSTR;      // per bond {normal, shear} strength
+    const dem_material* MAT;
+    // broad phase
+    int32_t* KEY;      // cell of each slot (global cell index)
+    int32_t* RANK;     // rank inside the cell from the counting atomics
+    int32_t* CNT;      // [ncells_total] counts, zeroed by the scan
+    int32_t* START;    // [ncells_total + 1]
+    int32_t* SIDX;     // slots in cell order
+    int32_t* INV;      // [cap] slot of (scene p0 + particle id)
+    // counters
+    unsigned long long* STAT;  // [0] contact overflow, [1] bond overflow, [2] scan timeouts
+    // scan scratch, cleared by the step kernel that precedes each scan
+    unsigned long long* TILES;
+    unsigned int* TICKET;
+    int32_t ntiles;
+};
+
+struct StepArgs {
+    const SceneDev* scenes;
+    const int32_t* blk_scene;  // scene of each block (-1: none)
+    const int2* lf;            // per scene {step flags, global step index of the Sim half}
+    DevArrays a;
+    int cur;                   // which P/D buffer holds the current positions
+};
+
+struct ReorderArgs {
+    const SceneDev* scenes;
+    const int32_t* blk_scene;
+    const int2* lf;
+    DevArrays src;  // current arrays
+    DevArrays dst;  // destination arrays
+    int cur;
+    int with_ext, with_data;
+};
+
+}  // namespace psim

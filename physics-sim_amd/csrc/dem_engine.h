@@ -1,0 +1,154 @@
+// dem_engine.h - host side of the DEM library: device pools, scenes and the step driver.
+//
+// An Engine owns one set of device pools on one HIP stream and advances any number
+// of independent scenes in the same launches.  The C ABI's dem_ctx is an Engine
+// holding a single scene; the serving loop uses one Engine for many scenes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dem_device.h"
+
+namespace psim {
+
+void set_error(const std::string& msg);
+
+struct DevSet {
+    float4* P[2] = {nullptr, nullptr};
+    float2* D[2] = {nullptr, nullptr};
+    float4* V = nullptr;
+    float4* A = nullptr;
+    int2* M = nullptr;
+    float4* EXT = nullptr;
+    float4* DATA0 = nullptr;
+    float4* DATA1 = nullptr;
+    int32_t* CC = nullptr;
+    int32_t* CP = nullptr;
+    float* CF = nullptr;
+};
+
+struct SceneHost {
+    dem_params params{};
+    std::vector<dem_material> mats;
+    bool live = false;
+    // pool regions
+    int64_t p0 = 0;
+    int32_t np = 0;          // slots in use (multiple of the block size)
+    int32_t slot_cap = 0;    // slots reserved
+    int32_t n = 0;           // particles
+    int64_t c0 = 0;
+    int32_t cell_cap = 0;    // cells reserved (w*h + 1 must fit)
+    int64_t b0 = 0;
+    int32_t nb = 0, bond_cap = 0;
+    int64_t r0 = 0;
+    int32_t nrec = 0, rec_cap = 0;
+    int32_t m0 = 0, nm = 0, mat_cap = 0;
+    // host copies of arrays the step never changes
+    std::vector<int32_t> fixity;
+    std::vector<float> forces;
+    std::vector<int32_t> matptr;
+    std::vector<int32_t> bond_angle, bond_len;
+    bool has_ext = false;
+    bool has_strength = false;
+    // coll_cont words (2D_LOM.wgsl:154-163, 2D_Simulation.wgsl:258-261)
+    int32_t coll0 = 0, coll1 = 0;
+    bool del_override = false;
+    bool r0_zero = false;    // particle 0 has radius 0: invocation 0 never updates coll_cont
+    int64_t step = 0;        // Sim halves done since creation (bond-break epochs)
+    // counters
+    int64_t steps = 0, broad_phases = 0, reorders = 0;
+    int32_t since_reorder = 1 << 30;
+};
+
+class Engine {
+public:
+    Engine() = default;
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    // Pools: particle slots, cells, bonds, bond records, materials, scenes.
+    int init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t rec_cap, int32_t mat_cap,
+             int32_t scene_cap, int32_t block, bool with_data);
+
+    // Scene lifetime (regions carved from the pools, first fit).
+    int add_scene(const dem_params& p, int32_t capacity, int32_t bond_cap, int32_t rec_cap, int32_t* sid_out);
+    int remove_scene(int32_t sid);
+    int set_params(int32_t sid, const dem_params& p);
+    int upload(int32_t sid, const dem_state_view& v);
+    int download(int32_t sid, dem_state_view& v);
+
+    // Advances the scenes in `sids` by `nsteps[i]` steps each in shared launches;
+    // scenes that finish early sit out the remaining launches.
+    int step(const std::vector<int32_t>& sids, const std::vector<int32_t>& nsteps);
+
+    int kinetic_energy(int32_t sid, double* out);
+    int broken_bonds(int32_t sid, int64_t* out);
+    int counters(int32_t sid, int64_t* out, int32_t n_out);
+    int datum(int32_t sid, const int32_t* select, double* out);
+    int debug_broad_phase(int32_t sid, int32_t* keys, int32_t* sorted_ids, int32_t* cell_start);
+    int debug_contacts(int32_t sid, int32_t max_per, int32_t* counts, int32_t* partners);
+    int bond_pairs(int32_t sid, int32_t max_bonds, float factor, int32_t* pairs, int32_t cap, int32_t* n_out);
+    int sync();
+
+    // timing of the last step() call
+    void enable_timing(bool on) { timing_ = on; }
+    double t_total = 0, t_step = 0, t_scan = 0, t_scatter = 0, t_reorder = 0;
+    int64_t n_step = 0, n_scan = 0, n_scatter = 0, n_reorder = 0;
+
+    hipStream_t stream() const { return stream_; }
+    int block() const { return block_; }
+    const SceneHost& scene(int32_t sid) const { return scenes_[sid]; }
+    int32_t scene_count() const { return (int32_t)scenes_.size(); }
+    int64_t slot_cap() const { return slot_cap_; }
+    int64_t free_slots() const;
+
+private:
+    int check(hipError_t e, const char* what);
+    DevArrays arrays(int set) const;
+    StepArgs make_args(const int2* lf) const;
+    int write_scene_table();
+    int reorder(const int2* lf_dev, const std::vector<int32_t>& sids);
+    int launch_grid_blocks() const;
+    int alloc_region(std::vector<std::pair<int64_t, int64_t>>& used, int64_t cap, int64_t need, int64_t align,
+                     int64_t* out);
+    SceneDev scene_dev(const SceneHost& sh) const;
+
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    int block_ = 256;
+    bool with_data_ = true;
+    int64_t slot_cap_ = 0, cell_cap_ = 0, bond_cap_ = 0, rec_cap_ = 0;
+    int32_t mat_cap_ = 0, scene_cap_ = 0;
+    int set_ = 0;   // which DevSet is current
+    int cur_ = 0;   // which P/D buffer is current
+    DevSet sets_[2];
+    int32_t* BOFF_ = nullptr;
+    int4* BR_ = nullptr;
+    float4* BF_ = nullptr;
+    int2* BS_ = nullptr;
+    float* BL_ = nullptr;
+    float2* BSTR_ = nullptr;
+    dem_material* MAT_ = nullptr;
+    int32_t *KEY_ = nullptr, *RANK_ = nullptr, *CNT_ = nullptr, *START_ = nullptr, *SIDX_ = nullptr, *INV_ = nullptr;
+    unsigned long long* STAT_ = nullptr;
+    unsigned long long* TILES_ = nullptr;
+    unsigned int* TICKET_ = nullptr;
+    int32_t ntiles_ = 0;
+    SceneDev* d_scenes_ = nullptr;
+    int32_t* d_blk_scene_ = nullptr;
+    int2* d_lf_ = nullptr;        // launch flags ring
+    int64_t lf_cap_ = 0;          // entries (launches * scene_cap)
+    double* d_ke_ = nullptr;      // per-id kinetic energy scratch
+    unsigned long long* d_cnt_ = nullptr;
+    std::vector<SceneHost> scenes_;
+    std::vector<std::pair<int64_t, int64_t>> used_slots_, used_cells_, used_bonds_, used_recs_, used_mats_;
+    bool timing_ = false;
+    std::vector<hipEvent_t> ev_pool_;
+    bool table_dirty_ = true;
+};
+
+}  // namespace psim

@@ -1,0 +1,1009 @@
+// dem_engine.hip - pools, scene regions, upload/download and the step driver.
+#include "dem_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace psim {
+
+hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, hipStream_t st);
+hipError_t launch_bin(const StepArgs& args, int nblocks, int block, hipStream_t st);
+int scan_tiles(int n);
+hipError_t launch_scan(const DevArrays& a, int ncells, hipStream_t st);
+hipError_t launch_scatter(const StepArgs& args, int nblocks, int block, hipStream_t st);
+hipError_t launch_cell_sort(const StepArgs& args, int cell_lo, int cell_hi, int p0, hipStream_t st);
+hipError_t launch_reorder_move(const ReorderArgs& r, int nblocks, int block, hipStream_t st);
+hipError_t launch_identity(int32_t* sidx, int64_t n, hipStream_t st);
+hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, int np, double* out, hipStream_t st);
+hipError_t launch_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out, hipStream_t st);
+hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
+                             int32_t* out_cnt, int32_t* out_j, hipStream_t st);
+
+namespace {
+inline int32_t fbits(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
+inline float bitsf(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
+inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+template <typename T>
+hipError_t dalloc(T** p, int64_t n) {
+    *p = nullptr;
+    if (n <= 0) n = 1;
+    return hipMalloc((void**)p, sizeof(T) * (size_t)n);
+}
+
+struct Slot {  // 2D_Simulation.wgsl:19-26
+    int32_t a, b;
+    float ft, fbt, tb;
+    int32_t bonded;
+};
+}  // namespace
+
+Engine::~Engine() {
+    if (stream_) hipStreamSynchronize(stream_);
+    for (auto& s : sets_) {
+        hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.D[0]); hipFree(s.D[1]);
+        hipFree(s.V); hipFree(s.A); hipFree(s.M); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
+        hipFree(s.CC); hipFree(s.CP); hipFree(s.CF);
+    }
+    hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BL_); hipFree(BSTR_); hipFree(MAT_);
+    hipFree(KEY_); hipFree(RANK_); hipFree(CNT_); hipFree(START_); hipFree(SIDX_); hipFree(INV_);
+    hipFree(STAT_); hipFree(TILES_); hipFree(TICKET_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
+    hipFree(d_ke_); hipFree(d_cnt_);
+    for (auto e : ev_pool_) hipEventDestroy(e);
+    if (stream_) hipStreamDestroy(stream_);
+}
+
+int Engine::check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return DEM_OK;
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? DEM_ENOMEM : DEM_EDEVICE;
+}
+
+int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t rec_cap, int32_t mat_cap,
+                 int32_t scene_cap, int32_t block, bool with_data) {
+    if (block <= 0) block = 256;
+    if (block % 64 || block > 256) {
+        set_error("block_size must be a multiple of 64 in [64, 256]");
+        return DEM_EINVAL;
+    }
+    block_ = block;
+    with_data_ = with_data;
+    slot_cap_ = round_up(std::max<int64_t>(slot_cap, block), block);
+    cell_cap_ = std::max<int64_t>(cell_cap, 2);
+    bond_cap_ = std::max<int64_t>(bond_cap, 1);
+    rec_cap_ = std::max<int64_t>(rec_cap, 1);
+    mat_cap_ = std::max(mat_cap, 1);
+    scene_cap_ = std::max(scene_cap, 1);
+    if (slot_cap_ > (int64_t)INT32_MAX / 2 || cell_cap_ > (int64_t)INT32_MAX / 2) {
+        set_error("pool too large for 32-bit slot indices");
+        return DEM_EINVAL;
+    }
+    int rc;
+    if ((rc = check(hipGetDevice(&device_), "hipGetDevice"))) return rc;
+    if ((rc = check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate"))) return rc;
+    const int64_t S = slot_cap_;
+    for (auto& s : sets_) {
+        hipError_t e = hipSuccess;
+        e = e ? e : dalloc(&s.P[0], S); e = e ? e : dalloc(&s.P[1], S);
+        e = e ? e : dalloc(&s.D[0], S); e = e ? e : dalloc(&s.D[1], S);
+        e = e ? e : dalloc(&s.V, S); e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S);
+        e = e ? e : dalloc(&s.EXT, S);
+        if (with_data) { e = e ? e : dalloc(&s.DATA0, S); e = e ? e : dalloc(&s.DATA1, S); }
+        e = e ? e : dalloc(&s.CC, S); e = e ? e : dalloc(&s.CP, S * KMAX); e = e ? e : dalloc(&s.CF, S * KMAX);
+        if ((rc = check(e, "hipMalloc particle pools"))) return rc;
+    }
+    hipError_t e = hipSuccess;
+    e = e ? e : dalloc(&BOFF_, S + 1); e = e ? e : dalloc(&BR_, rec_cap_); e = e ? e : dalloc(&BF_, rec_cap_);
+    e = e ? e : dalloc(&BS_, bond_cap_); e = e ? e : dalloc(&BL_, bond_cap_); e = e ? e : dalloc(&BSTR_, bond_cap_);
+    e = e ? e : dalloc(&MAT_, mat_cap_);
+    e = e ? e : dalloc(&KEY_, S); e = e ? e : dalloc(&RANK_, S); e = e ? e : dalloc(&CNT_, cell_cap_ + 1);
+    e = e ? e : dalloc(&START_, cell_cap_ + 2); e = e ? e : dalloc(&SIDX_, S); e = e ? e : dalloc(&INV_, S);
+    e = e ? e : dalloc(&STAT_, 8);
+    ntiles_ = scan_tiles((int)cell_cap_);
+    e = e ? e : dalloc(&TILES_, ntiles_ + 1); e = e ? e : dalloc(&TICKET_, 1);
+    e = e ? e : dalloc(&d_scenes_, scene_cap_);
+    const int64_t nblk = launch_grid_blocks();
+    e = e ? e : dalloc(&d_blk_scene_, nblk);
+    lf_cap_ = (int64_t)scene_cap_ * 4096;
+    e = e ? e : dalloc(&d_lf_, lf_cap_);
+    e = e ? e : dalloc(&d_ke_, S);
+    e = e ? e : dalloc(&d_cnt_, 4);
+    if ((rc = check(e, "hipMalloc pools"))) return rc;
+    e = hipMemsetAsync(CNT_, 0, sizeof(int32_t) * (cell_cap_ + 1), stream_);
+    e = e ? e : hipMemsetAsync(START_, 0, sizeof(int32_t) * (cell_cap_ + 2), stream_);
+    e = e ? e : hipMemsetAsync(STAT_, 0, sizeof(unsigned long long) * 8, stream_);
+    e = e ? e : hipMemsetAsync(BOFF_, 0, sizeof(int32_t) * (S + 1), stream_);
+    for (auto& s : sets_) {
+        e = e ? e : hipMemsetAsync(s.CC, 0, sizeof(int32_t) * S, stream_);
+        e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int2) * S, stream_);
+    }
+    std::vector<int32_t> blk(nblk, -1);
+    e = e ? e : hipMemcpyAsync(d_blk_scene_, blk.data(), sizeof(int32_t) * nblk, hipMemcpyHostToDevice, stream_);
+    e = e ? e : hipStreamSynchronize(stream_);
+    return check(e, "pool init");
+}
+
+int Engine::launch_grid_blocks() const {
+    const int64_t need = std::max<int64_t>(slot_cap_, (int64_t)ntiles_ + 1);
+    return (int)((need + block_ - 1) / block_);
+}
+
+int64_t Engine::free_slots() const {
+    int64_t used = 0;
+    for (auto& r : used_slots_) used += r.second;
+    return slot_cap_ - used;
+}
+
+int Engine::alloc_region(std::vector<std::pair<int64_t, int64_t>>& used, int64_t cap, int64_t need, int64_t align,
+                         int64_t* out) {
+    std::sort(used.begin(), used.end());
+    int64_t at = 0;
+    for (auto& r : used) {
+        if (round_up(at, align) + need <= r.first) break;
+        at = std::max(at, r.first + r.second);
+    }
+    at = round_up(at, align);
+    if (at + need > cap) return DEM_EBUSY;
+    used.push_back({at, need});
+    *out = at;
+    return DEM_OK;
+}
+
+static void release_region(std::vector<std::pair<int64_t, int64_t>>& used, int64_t at) {
+    for (size_t i = 0; i < used.size(); i++)
+        if (used[i].first == at) { used.erase(used.begin() + i); return; }
+}
+
+static int validate_params(const dem_params& p) {
+    if (!p.materials || p.n_materials <= 0) { set_error("params: no materials"); return DEM_EINVAL; }
+    if (!(p.grid.cell_size > 0.0f) || p.grid.w <= 0 || p.grid.h <= 0) {
+        set_error("params: grid needs cell_size > 0, w > 0, h > 0");
+        return DEM_EINVAL;
+    }
+    if ((int64_t)p.grid.w * p.grid.h > (int64_t)INT32_MAX / 4) { set_error("params: grid too large"); return DEM_EINVAL; }
+    if (p.u.particles < 0) { set_error("params: negative particle count"); return DEM_EINVAL; }
+    return DEM_OK;
+}
+
+int Engine::add_scene(const dem_params& p, int32_t capacity, int32_t bond_cap, int32_t rec_cap, int32_t* sid_out) {
+    int rc = validate_params(p);
+    if (rc) return rc;
+    if (capacity <= 0) { set_error("capacity must be positive"); return DEM_EINVAL; }
+    SceneHost sh;
+    sh.params = p;
+    sh.mats.assign(p.materials, p.materials + p.n_materials);
+    sh.params.materials = nullptr;
+    sh.slot_cap = (int32_t)round_up(capacity, block_);
+    sh.cell_cap = p.grid.w * p.grid.h + 1;
+    sh.bond_cap = std::max(bond_cap, 0);
+    sh.rec_cap = std::max(rec_cap, 0);
+    sh.mat_cap = std::max(p.n_materials, 8);
+    int64_t at;
+    if (alloc_region(used_slots_, slot_cap_, sh.slot_cap, block_, &at)) { set_error("no room for the particles"); return DEM_EBUSY; }
+    sh.p0 = at;
+    if (alloc_region(used_cells_, cell_cap_, sh.cell_cap, 1, &at)) {
+        release_region(used_slots_, sh.p0);
+        set_error("no room for the grid cells");
+        return DEM_EBUSY;
+    }
+    sh.c0 = at;
+    if (sh.bond_cap && alloc_region(used_bonds_, bond_cap_, sh.bond_cap, 1, &at)) {
+        release_region(used_slots_, sh.p0); release_region(used_cells_, sh.c0);
+        set_error("no room for the bonds");
+        return DEM_EBUSY;
+    }
+    sh.b0 = sh.bond_cap ? at : 0;
+    if (sh.rec_cap && alloc_region(used_recs_, rec_cap_, sh.rec_cap, 1, &at)) {
+        release_region(used_slots_, sh.p0); release_region(used_cells_, sh.c0);
+        if (sh.bond_cap) release_region(used_bonds_, sh.b0);
+        set_error("no room for the bond records");
+        return DEM_EBUSY;
+    }
+    sh.r0 = sh.rec_cap ? at : 0;
+    if (alloc_region(used_mats_, mat_cap_, sh.mat_cap, 1, &at)) {
+        release_region(used_slots_, sh.p0); release_region(used_cells_, sh.c0);
+        if (sh.bond_cap) release_region(used_bonds_, sh.b0);
+        if (sh.rec_cap) release_region(used_recs_, sh.r0);
+        set_error("no room for the materials");
+        return DEM_EBUSY;
+    }
+    sh.m0 = (int32_t)at;
+    sh.nm = p.n_materials;
+    sh.live = true;
+    // reuse a dead scene id or append
+    int32_t sid = -1;
+    for (size_t i = 0; i < scenes_.size(); i++)
+        if (!scenes_[i].live) { sid = (int32_t)i; break; }
+    if (sid < 0) {
+        if ((int32_t)scenes_.size() >= scene_cap_) {
+            release_region(used_slots_, sh.p0); release_region(used_cells_, sh.c0);
+            if (sh.bond_cap) release_region(used_bonds_, sh.b0);
+            if (sh.rec_cap) release_region(used_recs_, sh.r0);
+            release_region(used_mats_, sh.m0);
+            set_error("scene table full");
+            return DEM_EBUSY;
+        }
+        sid = (int32_t)scenes_.size();
+        scenes_.push_back(sh);
+    } else {
+        scenes_[sid] = sh;
+    }
+    hipError_t e = hipMemcpyAsync(MAT_ + sh.m0, sh.mats.data(), sizeof(dem_material) * sh.nm, hipMemcpyHostToDevice,
+                                  stream_);
+    if ((rc = check(e, "upload materials"))) return rc;
+    table_dirty_ = true;
+    *sid_out = sid;
+    return DEM_OK;
+}
+
+int Engine::remove_scene(int32_t sid) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    SceneHost& sh = scenes_[sid];
+    release_region(used_slots_, sh.p0);
+    release_region(used_cells_, sh.c0);
+    if (sh.bond_cap) release_region(used_bonds_, sh.b0);
+    if (sh.rec_cap) release_region(used_recs_, sh.r0);
+    release_region(used_mats_, sh.m0);
+    sh = SceneHost();
+    table_dirty_ = true;
+    return DEM_OK;
+}
+
+int Engine::set_params(int32_t sid, const dem_params& p) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    int rc = validate_params(p);
+    if (rc) return rc;
+    SceneHost& sh = scenes_[sid];
+    if (p.grid.w * p.grid.h + 1 > sh.cell_cap) { set_error("grid larger than the scene's cell reservation"); return DEM_ECAPACITY; }
+    if (p.n_materials > sh.mat_cap) { set_error("too many materials for the scene's reservation"); return DEM_ECAPACITY; }
+    if (p.use_f64 != sh.params.use_f64 || p.record_data != sh.params.record_data) {
+        // fine: kernels read both per launch
+    }
+    const int32_t n_keep = sh.params.u.particles;
+    sh.params = p;
+    sh.params.materials = nullptr;
+    sh.params.u.particles = n_keep;
+    sh.mats.assign(p.materials, p.materials + p.n_materials);
+    sh.nm = p.n_materials;
+    hipError_t e = hipMemcpyAsync(MAT_ + sh.m0, sh.mats.data(), sizeof(dem_material) * sh.nm, hipMemcpyHostToDevice,
+                                  stream_);
+    if ((rc = check(e, "upload materials"))) return rc;
+    table_dirty_ = true;
+    return DEM_OK;
+}
+
+SceneDev Engine::scene_dev(const SceneHost& sh) const {
+    SceneDev d{};
+    d.u = sh.params.u;
+    d.u.particles = sh.n;
+    d.p0 = (int32_t)sh.p0;
+    d.np = sh.np;
+    d.c0 = (int32_t)sh.c0;
+    d.w = sh.params.grid.w;
+    d.h = sh.params.grid.h;
+    d.cs = sh.params.grid.cell_size;
+    d.base_x = -d.cs * (float)d.w * 0.5f;
+    d.base_y = d.cs * (float)d.h * 0.5f;
+    d.b0 = (int32_t)sh.b0;
+    d.nb = sh.nb;
+    d.m0 = sh.m0;
+    d.nm = sh.nm;
+    d.use_f64 = sh.params.use_f64;
+    d.record = (sh.params.record_data && with_data_) ? 1 : 0;
+    d.has_strength = sh.has_strength ? 1 : 0;
+    d.n_real = sh.n;
+    return d;
+}
+
+int Engine::write_scene_table() {
+    if (!table_dirty_) return DEM_OK;
+    std::vector<SceneDev> tab(scene_cap_);
+    std::vector<int32_t> blk(launch_grid_blocks(), -1);
+    for (size_t i = 0; i < scenes_.size(); i++) {
+        const SceneHost& sh = scenes_[i];
+        if (!sh.live) continue;
+        tab[i] = scene_dev(sh);
+        for (int64_t b = sh.p0 / block_; b < (sh.p0 + sh.np) / block_; b++) blk[b] = (int32_t)i;
+    }
+    hipError_t e = hipMemcpyAsync(d_scenes_, tab.data(), sizeof(SceneDev) * scene_cap_, hipMemcpyHostToDevice, stream_);
+    e = e ? e : hipMemcpyAsync(d_blk_scene_, blk.data(), sizeof(int32_t) * blk.size(), hipMemcpyHostToDevice, stream_);
+    e = e ? e : hipStreamSynchronize(stream_);
+    int rc = check(e, "scene table");
+    if (!rc) table_dirty_ = false;
+    return rc;
+}
+
+DevArrays Engine::arrays(int set) const {
+    const DevSet& s = sets_[set];
+    DevArrays a{};
+    a.P[0] = s.P[0]; a.P[1] = s.P[1]; a.D[0] = s.D[0]; a.D[1] = s.D[1];
+    a.V = s.V; a.A = s.A; a.M = s.M; a.EXT = s.EXT; a.DATA0 = s.DATA0; a.DATA1 = s.DATA1;
+    a.CC = s.CC; a.CP = s.CP; a.CF = s.CF; a.cap = slot_cap_;
+    a.BOFF = BOFF_; a.BR = BR_; a.BF = BF_; a.BS = BS_; a.BL = BL_; a.BSTR = BSTR_; a.MAT = MAT_;
+    a.KEY = KEY_; a.RANK = RANK_; a.CNT = CNT_; a.START = START_; a.SIDX = SIDX_; a.INV = INV_;
+    a.STAT = STAT_; a.TILES = TILES_; a.TICKET = TICKET_; a.ntiles = ntiles_;
+    return a;
+}
+
+StepArgs Engine::make_args(const int2* lf) const {
+    StepArgs s{};
+    s.scenes = d_scenes_;
+    s.blk_scene = d_blk_scene_;
+    s.lf = lf;
+    s.a = arrays(set_);
+    s.cur = cur_;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Upload: reference layouts (state.rs:69-93) -> device layout, storage order = id order.
+int Engine::upload(int32_t sid, const dem_state_view& v) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    SceneHost& sh = scenes_[sid];
+    const int32_t n = v.n;
+    if (n < 0 || n > sh.slot_cap) { set_error("particle count exceeds the scene capacity"); return DEM_ECAPACITY; }
+    if (n > 0 && (!v.pos || !v.vel || !v.acc || !v.rot || !v.rot_vel || !v.rot_acc || !v.forces || !v.radii ||
+                  !v.fixity || !v.material_pointers)) {
+        set_error("state view is missing a particle array");
+        return DEM_EINVAL;
+    }
+    if (v.n_bonds > 0 && !v.bonds) { set_error("n_bonds > 0 but bonds is NULL"); return DEM_EINVAL; }
+    if (v.n_bonds > sh.bond_cap) { set_error("more bonds than the scene's bond capacity"); return DEM_ECAPACITY; }
+    if (v.contacts && v.contact_slots <= 0) { set_error("contacts given with contact_slots <= 0"); return DEM_EINVAL; }
+    const int32_t np = (int32_t)round_up(std::max(n, 1), block_);
+    sh.n = n;
+    sh.np = np;
+    sh.params.u.particles = n;
+    const int32_t nmat = sh.nm;
+    std::vector<float4> P(np), V(np), A(np), EXT(np);
+    std::vector<float2> D(np);
+    std::vector<int2> M(np);
+    std::vector<float4> D0(np), D1(np);
+    std::vector<int32_t> CC(np, 0), CP((size_t)np * KMAX, 0), INV(np), SIDX(np);
+    std::vector<float> CF((size_t)np * KMAX, 0.0f);
+    sh.fixity.assign(v.fixity, v.fixity + 6 * (size_t)n);
+    sh.forces.assign(v.forces, v.forces + 6 * (size_t)n);
+    sh.matptr.assign(v.material_pointers, v.material_pointers + n);
+    sh.has_ext = false;
+    for (int32_t i = 0; i < np; i++) {
+        INV[i] = (int32_t)sh.p0 + i;
+        SIDX[i] = (int32_t)sh.p0 + i;
+        if (i >= n) {
+            P[i] = make_float4(0, 0, 0, 0);
+            D[i] = make_float2(0, 0);
+            V[i] = A[i] = EXT[i] = D0[i] = D1[i] = make_float4(0, 0, 0, 0);
+            M[i] = make_int2((int)PF_INACTIVE, i);
+            continue;
+        }
+        const float r = v.radii[i];
+        // WGPUComputeProg::restore uploads vel into del_pos and rot_vel into del_rot (wgpu_prog.rs:1247-1248)
+        P[i] = make_float4(v.pos[2 * i], v.pos[2 * i + 1], r, v.rot_vel[i]);
+        D[i] = make_float2(v.vel[2 * i], v.vel[2 * i + 1]);
+        V[i] = make_float4(v.vel[2 * i], v.vel[2 * i + 1], v.rot_vel[i], v.rot[i]);
+        A[i] = make_float4(v.acc[2 * i], v.acc[2 * i + 1], v.rot_acc[i], 0.0f);
+        const float* f = v.forces + 6 * (size_t)i;
+        EXT[i] = make_float4(f[0], f[1], f[2], 0.0f);
+        const int32_t* fx = v.fixity + 6 * (size_t)i;
+        uint32_t flags = 0;
+        int32_t mat = v.material_pointers[i];
+        if (mat < 0 || mat >= nmat) mat = 0;
+        flags |= (uint32_t)mat & PF_MAT_MASK;
+        if (fx[0] == 0) flags |= PF_KICK_X;
+        if (fx[1] == 0) flags |= PF_KICK_Y;
+        if (fx[2] == 0) flags |= PF_KICK_R;
+        if (fx[0] != 1) flags |= PF_WALL_X;
+        if (fx[1] != 1) flags |= PF_WALL_Y;
+        if (fx[2] != 1) flags |= PF_WALL_R;
+        if (fbits(f[0]) || fbits(f[1]) || fbits(f[2])) { flags |= PF_EXT; sh.has_ext = true; }
+        if (r == 0.0f) flags |= PF_INACTIVE;
+        M[i] = make_int2((int)flags, i);
+        if (v.data) {
+            const float* d = v.data + 7 * (size_t)i;
+            D0[i] = make_float4(d[0], d[1], d[2], d[3]);
+            D1[i] = make_float4(d[4], d[5], d[6], 0.0f);
+        } else {
+            D0[i] = D1[i] = make_float4(0, 0, 0, 0);
+        }
+    }
+    sh.r0_zero = n > 0 && v.radii[0] == 0.0f;
+    {
+        // the 3x3 stencil needs cells wider than 2 r_max plus the rounding of
+        // (x - base) / cell_size, a few ulp of the grid extent
+        float rmax = 0.0f;
+        for (int32_t i = 0; i < n; i++) rmax = std::max(rmax, v.radii[i]);
+        const float cs = sh.params.grid.cell_size;
+        const float ext = cs * (float)std::max(sh.params.grid.w, sh.params.grid.h);
+        const float need = 2.0f * rmax + 4.0f * (std::nextafter(ext, INFINITY) - ext);
+        if (cs < need) {
+            set_error("grid cell_size " + std::to_string(cs) + " is below 2*max radius + rounding margin " +
+                      std::to_string(need));
+            return DEM_EINVAL;
+        }
+    }
+    // bonds: index word, rest length (angle kept on the host)
+    sh.nb = v.n_bonds;
+    sh.bond_angle.assign(sh.nb, 0);
+    sh.bond_len.assign(sh.nb, 0);
+    std::vector<int2> BS(std::max(sh.nb, 1));
+    std::vector<float> BL(std::max(sh.nb, 1));
+    std::vector<float2> BSTR(std::max(sh.nb, 1));
+    for (int32_t k = 0; k < sh.nb; k++) {
+        const int32_t idx = v.bonds[3 * k];
+        sh.bond_angle[k] = v.bonds[3 * k + 1];
+        sh.bond_len[k] = v.bonds[3 * k + 2];
+        BS[k] = make_int2(idx, INT32_MIN);
+        BL[k] = bitsf(v.bonds[3 * k + 2]);
+        BSTR[k] = v.bond_strengths ? make_float2(v.bond_strengths[2 * k], v.bond_strengths[2 * k + 1])
+                                   : make_float2(sh.params.u.bond_normal_strength, sh.params.u.bond_shear_strength);
+    }
+    sh.has_strength = v.bond_strengths != nullptr;
+    // contact table -> contact lists (plain contacts) and bond records (everything else)
+    std::vector<int32_t> BOFF(np + 1, 0);
+    std::vector<int4> BR;
+    std::vector<float4> BF;
+    if (v.contacts) {
+        const Slot* tab = reinterpret_cast<const Slot*>(v.contacts);
+        const int K = v.contact_slots;
+        std::vector<std::pair<int32_t, int32_t>> rec, con;  // (b, slot index)
+        for (int32_t i = 0; i < n; i++) {
+            rec.clear();
+            con.clear();
+            for (int k = 0; k < K; k++) {
+                const Slot& s = tab[(size_t)i * K + k];
+                if (s.b == -1) continue;
+                if (s.b < 0 || s.b >= n) continue;  // stale word pointing nowhere
+                const bool plain = s.bonded == -1 && fbits(s.fbt) == -1 && fbits(s.tb) == -1;
+                (plain ? con : rec).push_back({s.b, k});
+            }
+            std::sort(con.begin(), con.end());
+            std::sort(rec.begin(), rec.end());
+            con.erase(std::unique(con.begin(), con.end(),
+                                  [](auto& x, auto& y) { return x.first == y.first; }), con.end());
+            if ((int)con.size() > KMAX) {
+                set_error("particle " + std::to_string(i) + " has more than 16 contacts");
+                return DEM_ECAPACITY;
+            }
+            CC[i] = (int32_t)con.size();
+            for (size_t q = 0; q < con.size(); q++) {
+                const Slot& s = tab[(size_t)i * K + con[q].second];
+                CP[(size_t)q * np + i] = s.b;
+                CF[(size_t)q * np + i] = s.ft;
+            }
+            BOFF[i] = (int32_t)BR.size();
+            for (auto& rr : rec) {
+                const Slot& s = tab[(size_t)i * K + rr.second];
+                int32_t bid = s.bonded >= 0 ? s.bonded : -1;
+                BR.push_back(make_int4(s.b, bid, s.bonded, 1));
+                BF.push_back(make_float4(s.ft, s.fbt, s.tb, 0.0f));
+            }
+        }
+        for (int32_t i = n; i <= np; i++) BOFF[i] = (int32_t)BR.size();
+    }
+    if ((int64_t)BR.size() > sh.rec_cap) { set_error("more bond records than the scene's record capacity"); return DEM_ECAPACITY; }
+    sh.nrec = (int32_t)BR.size();
+    for (auto& o : BOFF) o += (int32_t)sh.r0;
+    // device copies: the current set, the current P/D buffer
+    hipError_t e = hipSuccess;
+    const DevSet& S = sets_[set_];
+    const int64_t p0 = sh.p0;
+    auto h2d = [&](void* dst, const void* src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_);
+    };
+    h2d(S.P[cur_] + p0, P.data(), sizeof(float4) * np);
+    h2d(S.D[cur_] + p0, D.data(), sizeof(float2) * np);
+    h2d(S.V + p0, V.data(), sizeof(float4) * np);
+    h2d(S.A + p0, A.data(), sizeof(float4) * np);
+    h2d(S.M + p0, M.data(), sizeof(int2) * np);
+    h2d(S.EXT + p0, EXT.data(), sizeof(float4) * np);
+    if (with_data_) {
+        h2d(S.DATA0 + p0, D0.data(), sizeof(float4) * np);
+        h2d(S.DATA1 + p0, D1.data(), sizeof(float4) * np);
+    }
+    h2d(S.CC + p0, CC.data(), sizeof(int32_t) * np);
+    for (int q = 0; q < KMAX; q++) {
+        h2d(S.CP + (int64_t)q * slot_cap_ + p0, CP.data() + (size_t)q * np, sizeof(int32_t) * np);
+        h2d(S.CF + (int64_t)q * slot_cap_ + p0, CF.data() + (size_t)q * np, sizeof(float) * np);
+    }
+    h2d(INV_ + p0, INV.data(), sizeof(int32_t) * np);
+    h2d(SIDX_ + p0, SIDX.data(), sizeof(int32_t) * np);
+    h2d(BOFF_ + p0, BOFF.data(), sizeof(int32_t) * (np + 1));
+    if (sh.nrec) {
+        h2d(BR_ + sh.r0, BR.data(), sizeof(int4) * sh.nrec);
+        h2d(BF_ + sh.r0, BF.data(), sizeof(float4) * sh.nrec);
+    }
+    if (sh.nb) {
+        h2d(BS_ + sh.b0, BS.data(), sizeof(int2) * sh.nb);
+        h2d(BL_ + sh.b0, BL.data(), sizeof(float) * sh.nb);
+        h2d(BSTR_ + sh.b0, BSTR.data(), sizeof(float2) * sh.nb);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    int rc = check(e, "upload");
+    if (rc) return rc;
+    sh.since_reorder = 1 << 30;  // re-sort storage at the first broad phase
+    table_dirty_ = true;
+    return DEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+int Engine::reorder(const int2* lf_dev, const std::vector<int32_t>& sids) {
+    // all scenes binned in this launch move; the others are copied unchanged
+    hipError_t e = hipSuccess;
+    StepArgs args = make_args(lf_dev);
+    for (int32_t sid : sids) {
+        const SceneHost& sh = scenes_[sid];
+        if (e == hipSuccess)
+            e = launch_cell_sort(args, (int)sh.c0, (int)(sh.c0 + sh.params.grid.w * sh.params.grid.h + 1),
+                                 (int)sh.p0, stream_);
+    }
+    ReorderArgs r{};
+    r.scenes = d_scenes_;
+    r.blk_scene = d_blk_scene_;
+    r.lf = lf_dev;
+    r.src = arrays(set_);
+    r.dst = arrays(set_ ^ 1);
+    r.cur = cur_;
+    r.with_ext = 1;
+    r.with_data = with_data_ ? 1 : 0;
+    if (e == hipSuccess) e = launch_reorder_move(r, (int)(slot_cap_ / block_), block_, stream_);
+    if (e == hipSuccess) e = launch_identity(SIDX_, slot_cap_, stream_);
+    int rc = check(e, "reorder");
+    if (rc) return rc;
+    set_ ^= 1;
+    for (int32_t sid : sids) {
+        scenes_[sid].since_reorder = 0;
+        scenes_[sid].reorders++;
+    }
+    return DEM_OK;
+}
+
+int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& nsteps) {
+    if (sids.size() != nsteps.size()) { set_error("step: size mismatch"); return DEM_EINVAL; }
+    int rc = write_scene_table();
+    if (rc) return rc;
+    int32_t maxn = 0;
+    for (size_t i = 0; i < sids.size(); i++) {
+        const int32_t sid = sids[i];
+        if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("step: no such scene"); return DEM_EINVAL; }
+        if (nsteps[i] < 0) { set_error("step: negative step count"); return DEM_EINVAL; }
+        maxn = std::max(maxn, nsteps[i]);
+    }
+    if (maxn == 0) return DEM_OK;
+    const int64_t nlaunch = (int64_t)maxn + 1;
+    // per-launch, per-scene flags, computed on the host from the coll_cont rules
+    std::vector<int2> lf((size_t)nlaunch * scene_cap_, make_int2(0, 0));
+    std::vector<std::vector<uint8_t>> bin_after(nlaunch);
+    for (size_t i = 0; i < sids.size(); i++) {
+        SceneHost& sh = scenes_[sids[i]];
+        const int32_t n = nsteps[i];
+        if (n == 0) continue;
+        const dem_uniform& u = sh.params.u;
+        bool narrow_prev = false;
+        for (int32_t k = 0; k <= n; k++) {
+            uint32_t f = SF_ACTIVE;
+            int32_t stepno = (int32_t)(sh.step + k);
+            if (k > 0) {
+                f |= SF_SIM;
+                if (narrow_prev) f |= SF_NARROW;
+                // end of Sim_k: invocation 0 resets coll_cont[3] (2D_Simulation.wgsl:258-261)
+                if (!sh.r0_zero) sh.del_override = false;
+            }
+            if (k < n) {
+                f |= SF_LOM;
+                // LOM_{k+1}, invocation 0: interval counter (2D_LOM.wgsl:155-163)
+                if (!sh.r0_zero) {
+                    sh.coll1 += 1;
+                    if (sh.coll1 >= u.collision_interval) { sh.coll0 = 1; sh.coll1 = 0; }
+                    else sh.coll0 = 0;
+                }
+                if (sh.del_override) f |= SF_DELOVR;
+                narrow_prev = u.collisions == 1 && sh.coll0 == 1;
+                if (narrow_prev) f |= SF_BIN;
+            }
+            lf[(size_t)k * scene_cap_ + sids[i]] = make_int2((int)f, stepno);
+        }
+        sh.step += n;
+        sh.steps += n;
+    }
+    // ring of launch flags on the device
+    if (nlaunch * scene_cap_ > lf_cap_) {
+        hipFree(d_lf_);
+        lf_cap_ = nlaunch * scene_cap_;
+        if ((rc = check(dalloc(&d_lf_, lf_cap_), "hipMalloc launch flags"))) return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_lf_, lf.data(), sizeof(int2) * lf.size(), hipMemcpyHostToDevice, stream_);
+    if ((rc = check(e, "upload launch flags"))) return rc;
+
+    bool any_f64 = false, any_bonds = false;
+    for (int32_t sid : sids) {
+        any_f64 |= scenes_[sid].params.use_f64 != 0;
+        any_bonds |= scenes_[sid].nrec > 0;
+    }
+    const int nblk = launch_grid_blocks();
+    // timing
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
+    size_t ev_used = 0;
+    auto ev = [&]() -> hipEvent_t {
+        if (ev_used >= ev_pool_.size()) {
+            hipEvent_t x;
+            hipEventCreate(&x);
+            ev_pool_.push_back(x);
+        }
+        return ev_pool_[ev_used++];
+    };
+    auto timed = [&](int kind, auto&& fn) -> hipError_t {
+        if (!timing_) return fn();
+        hipEvent_t a = ev(), b = ev();
+        hipEventRecord(a, stream_);
+        hipError_t r = fn();
+        hipEventRecord(b, stream_);
+        evs.push_back({kind, {a, b}});
+        return r;
+    };
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (timing_) { t0 = ev(); t1 = ev(); hipEventRecord(t0, stream_); }
+
+    for (int64_t k = 0; k < nlaunch; k++) {
+        const int2* lfk = d_lf_ + k * scene_cap_;
+        StepArgs args = make_args(lfk);
+        e = timed(0, [&]() { return launch_step(args, nblk, block_, any_f64, any_bonds, stream_); });
+        if ((rc = check(e, "step kernel"))) return rc;
+        // the LOM half wrote the next positions
+        bool binned = false, lom = false;
+        std::vector<int32_t> active;
+        for (size_t i = 0; i < sids.size(); i++) {
+            const int2 w = lf[(size_t)k * scene_cap_ + sids[i]];
+            if (w.x & SF_LOM) lom = true;
+            if (w.x & SF_BIN) { binned = true; active.push_back(sids[i]); }
+        }
+        if (lom) cur_ ^= 1;
+        if (binned) {
+            e = timed(1, [&]() { return launch_scan(arrays(set_), (int)cell_cap_, stream_); });
+            if ((rc = check(e, "scan"))) return rc;
+            e = timed(2, [&]() { return launch_scatter(make_args(lfk), nblk, block_, stream_); });
+            if ((rc = check(e, "scatter"))) return rc;
+            // storage re-sort: every binned scene moves when one of them is due
+            bool due = false;
+            for (int32_t sid : active) {
+                SceneHost& sh = scenes_[sid];
+                sh.broad_phases++;
+                const int32_t R = sh.params.reorder_interval;
+                if (sh.since_reorder >= (R > 0 ? R : (1 << 30))) due = true;
+                sh.since_reorder++;
+            }
+            if (due) {
+                e = timed(3, [&]() { return reorder(lfk, active) == DEM_OK ? hipSuccess : hipErrorUnknown; });
+                if (e != hipSuccess) return DEM_EDEVICE;
+            }
+        }
+    }
+    if (timing_) {
+        hipEventRecord(t1, stream_);
+        hipEventSynchronize(t1);
+        float ms = 0;
+        hipEventElapsedTime(&ms, t0, t1);
+        t_total = ms;
+        t_step = t_scan = t_scatter = t_reorder = 0;
+        n_step = n_scan = n_scatter = n_reorder = 0;
+        for (auto& x : evs) {
+            float m = 0;
+            hipEventElapsedTime(&m, x.second.first, x.second.second);
+            switch (x.first) {
+                case 0: t_step += m; n_step++; break;
+                case 1: t_scan += m; n_scan++; break;
+                case 2: t_scatter += m; n_scatter++; break;
+                default: t_reorder += m; n_reorder++; break;
+            }
+        }
+    }
+    return DEM_OK;
+}
+
+int Engine::sync() { return check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+// ---------------------------------------------------------------------------
+int Engine::download(int32_t sid, dem_state_view& v) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    const SceneHost& sh = scenes_[sid];
+    const int32_t n = sh.n, np = sh.np;
+    if (v.n != n) { set_error("view.n does not match the scene's particle count"); return DEM_EINVAL; }
+    const DevSet& S = sets_[set_];
+    const int64_t p0 = sh.p0;
+    std::vector<float4> P(np), V(np), A(np), D0(np), D1(np);
+    std::vector<int2> M(np);
+    std::vector<int32_t> INV(np), CC(np), CP((size_t)np * KMAX);
+    std::vector<float> CF((size_t)np * KMAX);
+    std::vector<int32_t> BOFF(np + 1);
+    std::vector<int4> BR(std::max(sh.nrec, 1));
+    std::vector<float4> BF(std::max(sh.nrec, 1));
+    std::vector<int2> BS(std::max(sh.nb, 1));
+    hipError_t e = hipSuccess;
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_);
+    };
+    d2h(P.data(), S.P[cur_] + p0, sizeof(float4) * np);
+    d2h(V.data(), S.V + p0, sizeof(float4) * np);
+    d2h(A.data(), S.A + p0, sizeof(float4) * np);
+    d2h(M.data(), S.M + p0, sizeof(int2) * np);
+    d2h(INV.data(), INV_ + p0, sizeof(int32_t) * np);
+    const bool rec = with_data_ && sh.params.record_data && v.data;
+    if (rec) {
+        d2h(D0.data(), S.DATA0 + p0, sizeof(float4) * np);
+        d2h(D1.data(), S.DATA1 + p0, sizeof(float4) * np);
+    }
+    if (v.contacts) {
+        d2h(CC.data(), S.CC + p0, sizeof(int32_t) * np);
+        for (int q = 0; q < KMAX; q++) {
+            d2h(CP.data() + (size_t)q * np, S.CP + (int64_t)q * slot_cap_ + p0, sizeof(int32_t) * np);
+            d2h(CF.data() + (size_t)q * np, S.CF + (int64_t)q * slot_cap_ + p0, sizeof(float) * np);
+        }
+        d2h(BOFF.data(), BOFF_ + p0, sizeof(int32_t) * (np + 1));
+        if (sh.nrec) {
+            d2h(BR.data(), BR_ + sh.r0, sizeof(int4) * sh.nrec);
+            d2h(BF.data(), BF_ + sh.r0, sizeof(float4) * sh.nrec);
+        }
+    }
+    if (sh.nb) d2h(BS.data(), BS_ + sh.b0, sizeof(int2) * sh.nb);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    int rc = check(e, "download");
+    if (rc) return rc;
+    for (int32_t i = 0; i < n; i++) {
+        const int64_t s = (int64_t)INV[i] - p0;
+        if (s < 0 || s >= np) { set_error("corrupt slot map"); return DEM_ESTATE; }
+        const float4 p = P[s], vv = V[s], aa = A[s];
+        if (v.pos) { v.pos[2 * i] = p.x; v.pos[2 * i + 1] = p.y; }
+        if (v.vel) { v.vel[2 * i] = vv.x; v.vel[2 * i + 1] = vv.y; }
+        if (v.rot) v.rot[i] = vv.w;
+        if (v.rot_vel) v.rot_vel[i] = vv.z;
+        if (v.acc) { v.acc[2 * i] = aa.x; v.acc[2 * i + 1] = aa.y; }
+        if (v.rot_acc) v.rot_acc[i] = aa.z;
+        if (v.radii) v.radii[i] = p.z;
+        if (v.forces) std::memcpy(v.forces + 6 * (size_t)i, sh.forces.data() + 6 * (size_t)i, 24);
+        if (v.fixity) std::memcpy(v.fixity + 6 * (size_t)i, sh.fixity.data() + 6 * (size_t)i, 24);
+        if (v.material_pointers) v.material_pointers[i] = sh.matptr[i];
+        if (rec) {
+            float* d = v.data + 7 * (size_t)i;
+            d[0] = D0[s].x; d[1] = D0[s].y; d[2] = D0[s].z; d[3] = D0[s].w;
+            d[4] = D1[s].x; d[5] = D1[s].y; d[6] = D1[s].z;
+        }
+    }
+    if (v.bonds) {
+        if (v.n_bonds != sh.nb) { set_error("view.n_bonds does not match the scene"); return DEM_EINVAL; }
+        for (int32_t k = 0; k < sh.nb; k++) {
+            v.bonds[3 * k] = BS[k].x;
+            v.bonds[3 * k + 1] = sh.bond_angle[k];
+            v.bonds[3 * k + 2] = sh.bond_len[k];
+        }
+    }
+    bool overflow = false;
+    if (v.contacts) {
+        const int K = v.contact_slots;
+        Slot* tab = reinterpret_cast<Slot*>(v.contacts);
+        std::vector<Slot> row;
+        for (int32_t i = 0; i < n; i++) {
+            row.clear();
+            const int64_t s = (int64_t)INV[i] - p0;
+            for (int q = 0; q < CC[s] && q < KMAX; q++) {
+                Slot x;
+                x.a = i;
+                x.b = CP[(size_t)q * np + s];
+                x.ft = CF[(size_t)q * np + s];
+                x.fbt = bitsf(-1);
+                x.tb = bitsf(-1);
+                x.bonded = -1;
+                row.push_back(x);
+            }
+            for (int32_t q = BOFF[i] - (int32_t)sh.r0; q < BOFF[i + 1] - (int32_t)sh.r0; q++) {
+                const int4 br = BR[q];
+                if (br.w == 0) continue;
+                const float4 bf = BF[q];
+                Slot x;
+                x.a = i;
+                x.b = br.x;
+                x.ft = bf.x;
+                x.fbt = bf.y;
+                x.tb = bf.z;
+                x.bonded = br.z;
+                row.push_back(x);
+            }
+            std::stable_sort(row.begin(), row.end(), [](const Slot& x, const Slot& y) { return x.b < y.b; });
+            if ((int)row.size() > K) overflow = true;
+            Slot* out = tab + (size_t)i * K;
+            for (int k = 0; k < K; k++) {
+                if (k < (int)row.size()) out[k] = row[k];
+                else std::memset(&out[k], 0xff, sizeof(Slot));
+            }
+        }
+    }
+    if (overflow) { set_error("a particle has more occupied slots than contact_slots"); return DEM_ECAPACITY; }
+    return DEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+int Engine::kinetic_energy(int32_t sid, double* out) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    const SceneHost& sh = scenes_[sid];
+    std::vector<double> ke(sh.np);
+    hipError_t e = launch_kinetic(arrays(set_), cur_, sh.m0, sh.nm, (int)sh.p0, sh.np, d_ke_, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(ke.data(), d_ke_, sizeof(double) * sh.np, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    int rc = check(e, "kinetic energy");
+    if (rc) return rc;
+    double s = 0.0;
+    for (int32_t i = 0; i < sh.n; i++) s += ke[i];
+    *out = s;
+    return DEM_OK;
+}
+
+int Engine::broken_bonds(int32_t sid, int64_t* out) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    const SceneHost& sh = scenes_[sid];
+    std::vector<int2> BS(std::max(sh.nb, 1));
+    hipError_t e = hipSuccess;
+    if (sh.nb) e = hipMemcpyAsync(BS.data(), BS_ + sh.b0, sizeof(int2) * sh.nb, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    int rc = check(e, "broken bonds");
+    if (rc) return rc;
+    int64_t c = 0;
+    for (int32_t k = 0; k < sh.nb; k++) c += BS[k].x < 0;
+    *out = c;
+    return DEM_OK;
+}
+
+int Engine::counters(int32_t sid, int64_t* out, int32_t n_out) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    const SceneHost& sh = scenes_[sid];
+    unsigned long long st[8] = {0}, nc = 0;
+    hipError_t e = hipMemcpyAsync(st, STAT_, sizeof(st), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipMemsetAsync(d_cnt_, 0, sizeof(unsigned long long), stream_);
+    if (e == hipSuccess) e = launch_count_contacts(sets_[set_].CC, (int)sh.p0, sh.np, d_cnt_, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(&nc, d_cnt_, sizeof(nc), hipMemcpyDeviceToHost, stream_);
+    int64_t nrec_alive = 0;
+    std::vector<int4> BR(std::max(sh.nrec, 1));
+    if (e == hipSuccess && sh.nrec) e = hipMemcpyAsync(BR.data(), BR_ + sh.r0, sizeof(int4) * sh.nrec, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    int rc = check(e, "counters");
+    if (rc) return rc;
+    for (int32_t q = 0; q < sh.nrec; q++) nrec_alive += BR[q].w != 0;
+    const int64_t vals[9] = {sh.steps, sh.broad_phases, (int64_t)st[0], (int64_t)st[1], (int64_t)nc, nrec_alive,
+                             sh.reorders, (int64_t)st[2], (int64_t)st[3]};
+    for (int32_t i = 0; i < n_out && i < 9; i++) out[i] = vals[i];
+    return DEM_OK;
+}
+
+int Engine::datum(int32_t sid, const int32_t* select, double* out) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    const SceneHost& sh = scenes_[sid];
+    if (!(with_data_ && sh.params.record_data)) { set_error("get_datum needs record_data"); return DEM_ESTATE; }
+    const int32_t n = sh.n;
+    std::vector<float> pos(2 * (size_t)n), rot(n), data(7 * (size_t)n), vel(2 * (size_t)n), rv(n);
+    dem_state_view v{};
+    v.n = n;
+    v.pos = pos.data();
+    v.rot = rot.data();
+    v.data = data.data();
+    int rc = download(sid, v);
+    if (rc) return rc;
+    double sums[10] = {0};
+    int64_t count = 0;
+    for (int32_t i = 0; i < n; i++) {
+        if (!select[i]) continue;
+        count++;
+        const float* d = data.data() + 7 * (size_t)i;
+        sums[0] += pos[2 * i]; sums[1] += pos[2 * i + 1];
+        sums[2] += d[4]; sums[3] += d[5]; sums[4] += rot[i]; sums[5] += d[6];
+        sums[6] += std::fabs((double)d[0]); sums[7] += d[1]; sums[8] += d[2]; sums[9] += d[3];
+    }
+    if (!count) { set_error("nothing selected"); return DEM_ESTATE; }
+    for (int k = 0; k < 10; k++) out[k] = k == 6 ? sums[k] : sums[k] / (double)count;
+    return DEM_OK;
+}
+
+// Stage views: bin the current positions, scan, scatter and sort each cell by id.
+int Engine::debug_broad_phase(int32_t sid, int32_t* keys, int32_t* sorted_ids, int32_t* cell_start) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    int rc = write_scene_table();
+    if (rc) return rc;
+    const SceneHost& sh = scenes_[sid];
+    std::vector<int2> lf(scene_cap_, make_int2(0, 0));
+    lf[sid] = make_int2((int)(SF_ACTIVE | SF_BIN), 0);
+    hipError_t e = hipMemcpyAsync(d_lf_, lf.data(), sizeof(int2) * scene_cap_, hipMemcpyHostToDevice, stream_);
+    StepArgs args = make_args(d_lf_);
+    const int ncl = sh.params.grid.w * sh.params.grid.h;
+    if (e == hipSuccess) e = launch_bin(args, launch_grid_blocks(), block_, stream_);
+    if (e == hipSuccess) e = launch_scan(arrays(set_), (int)cell_cap_, stream_);
+    if (e == hipSuccess) e = launch_scatter(args, launch_grid_blocks(), block_, stream_);
+    if (e == hipSuccess) e = launch_cell_sort(args, (int)sh.c0, (int)(sh.c0 + ncl + 1), (int)sh.p0, stream_);
+    std::vector<int32_t> KEY(sh.np), SIDX(sh.np), START(ncl + 2);
+    std::vector<int2> M(sh.np);
+    if (e == hipSuccess) e = hipMemcpyAsync(KEY.data(), KEY_ + sh.p0, sizeof(int32_t) * sh.np, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(SIDX.data(), SIDX_ + sh.p0, sizeof(int32_t) * sh.np, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(START.data(), START_ + sh.c0, sizeof(int32_t) * (ncl + 1), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(M.data(), sets_[set_].M + sh.p0, sizeof(int2) * sh.np, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    if ((rc = check(e, "debug broad phase"))) return rc;
+    // the cell lists now describe the current storage; the next step re-bins anyway
+    for (int32_t s = 0; s < sh.np; s++) {
+        const int32_t id = M[s].y;
+        if (id < sh.n && keys) keys[id] = KEY[s] - (int32_t)sh.c0;
+    }
+    if (sorted_ids)
+        for (int32_t e2 = 0; e2 < sh.n; e2++) sorted_ids[e2] = M[SIDX[e2] - sh.p0].y;
+    if (cell_start)
+        for (int32_t c = 0; c <= ncl; c++) cell_start[c] = START[c] - START[0];
+    return DEM_OK;
+}
+
+int Engine::debug_contacts(int32_t sid, int32_t max_per, int32_t* counts, int32_t* partners) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    const SceneHost& sh = scenes_[sid];
+    const int np = sh.np;
+    std::vector<int32_t> CC(np), CP((size_t)np * KMAX), INV(np);
+    hipError_t e = hipMemcpyAsync(CC.data(), sets_[set_].CC + sh.p0, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
+    for (int q = 0; q < KMAX && e == hipSuccess; q++)
+        e = hipMemcpyAsync(CP.data() + (size_t)q * np, sets_[set_].CP + (int64_t)q * slot_cap_ + sh.p0,
+                           sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(INV.data(), INV_ + sh.p0, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    int rc = check(e, "debug contacts");
+    if (rc) return rc;
+    bool over = false;
+    for (int32_t i = 0; i < sh.n; i++) {
+        const int64_t s = (int64_t)INV[i] - sh.p0;
+        const int c = CC[s];
+        counts[i] = c;
+        for (int k = 0; k < max_per; k++)
+            partners[(size_t)i * max_per + k] = (k < c && k < KMAX) ? CP[(size_t)k * np + s] : -1;
+        if (c > max_per) over = true;
+    }
+    if (over) { set_error("max_per smaller than a contact count"); return DEM_ECAPACITY; }
+    return DEM_OK;
+}
+
+int Engine::bond_pairs(int32_t sid, int32_t max_bonds, float factor, int32_t* pairs, int32_t cap, int32_t* n_out) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    if (max_bonds <= 0 || max_bonds > 16) { set_error("max_bonds must be in [1, 16]"); return DEM_EINVAL; }
+    const SceneHost& sh = scenes_[sid];
+    const int n = sh.n;
+    // bin the current positions (also refreshes the cell lists)
+    int32_t dummy = 0;
+    int rc = debug_broad_phase(sid, nullptr, nullptr, nullptr);
+    (void)dummy;
+    if (rc) return rc;
+    std::vector<float4> P(sh.np);
+    hipError_t e = hipMemcpyAsync(P.data(), sets_[set_].P[cur_] + sh.p0, sizeof(float4) * sh.np, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    if ((rc = check(e, "bond pairs"))) return rc;
+    float rmax = 0.0f;
+    for (int i = 0; i < sh.np; i++) rmax = std::max(rmax, P[i].z);
+    const int reach = std::max(1, (int)std::ceil((double)factor * 2.0 * rmax / sh.params.grid.cell_size));
+    int32_t* d_cnt = nullptr;
+    int32_t* d_j = nullptr;
+    e = dalloc(&d_cnt, std::max(n, 1));
+    if (e == hipSuccess) e = dalloc(&d_j, (int64_t)std::max(n, 1) * max_bonds);
+    std::vector<int2> lf(scene_cap_, make_int2(0, 0));
+    lf[sid] = make_int2((int)SF_ACTIVE, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_lf_, lf.data(), sizeof(int2) * scene_cap_, hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) e = launch_bond_pairs(make_args(d_lf_), launch_grid_blocks(), block_, reach, max_bonds, factor, d_cnt, d_j, stream_);
+    std::vector<int32_t> cnt(std::max(n, 1)), jj((size_t)std::max(n, 1) * max_bonds);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(jj.data(), d_j, sizeof(int32_t) * (size_t)n * max_bonds, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    hipFree(d_cnt);
+    hipFree(d_j);
+    if ((rc = check(e, "bond pairs"))) return rc;
+    int64_t k = 0;
+    bool over = false;
+    for (int i = 0; i < n; i++)
+        for (int q = 0; q < cnt[i]; q++) {
+            if (k < cap) { pairs[2 * k] = i; pairs[2 * k + 1] = jj[(size_t)i * max_bonds + q]; }
+            else over = true;
+            k++;
+        }
+    *n_out = (int32_t)std::min<int64_t>(k, INT32_MAX);
+    if (over) { set_error("bond_cap too small"); return DEM_ECAPACITY; }
+    return DEM_OK;
+}
+
+}  // namespace psim

@@ -1,0 +1,913 @@
+// dem_kernels.hip - gfx950 kernels of the DEM time step.
+//
+// The reference runs a time step as two WGSL passes, LOM (2D_LOM.wgsl:80-201:
+// half-kick, drift, walls, AABB binning) and Sim (2D_Simulation.wgsl:108-478:
+// narrow phase, contact/bond laws, first half-kick).  Here one step is:
+//
+//   k_step     fused "Sim half of step k" + "LOM half of step k+1" per particle slot:
+//              contact search over the 3x3 neighbour cells of the current cell list,
+//              force laws in ascending partner-id order, store_forces, then the next
+//              drift and its cell key with a per-cell counting atomic (stages 1, 2a,
+//              4, 5).  The first launch of a dem_step call runs only the LOM half and
+//              the last one only the Sim half, so the state between calls is the
+//              reference's post-Sim state.
+//   k_scan     exclusive prefix sum of the per-cell counts (single pass, decoupled
+//              look-back, wave64 prefix in registers + LDS across waves); zeroes the
+//              counts for the next step (stages 2b, 3: cell start = prefix).
+//   k_scatter  slot -> position START[key] + RANK in the cell-ordered slot list.
+//   k_cell_sort, k_reorder_move   every few broad phases: sort each cell's entries
+//              by particle id and move every per-particle array into that order.
+//
+// Arithmetic follows the WGSL expression by expression (operand order, no fused
+// multiply-add: the library is built with -ffp-contract=off; sqrt and division are
+// the correctly rounded sequences), so the HIP path reproduces the CPU restatement
+// of the reference bit for bit when both sum contacts in the same order (ascending
+// partner id).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dem_device.h"
+
+namespace psim {
+
+constexpr float PI_F = 3.14159265358979323846f;
+constexpr float G_F = 9.81f;
+
+__device__ __forceinline__ float lenf(float x, float y) {
+    float xx = x * x;
+    float yy = y * y;
+    return sqrtf(xx + yy);
+}
+__device__ __forceinline__ double lend(double x, double y) {
+    double xx = x * x;
+    double yy = y * y;
+    return sqrt(xx + yy);
+}
+__device__ __forceinline__ int sign_i(float x) { return x > 0.0f ? 1 : (x < 0.0f ? -1 : 0); }
+__device__ __forceinline__ int f2i(float v) {
+    if (!(v == v)) return 0;
+    if (v >= 2147483520.0f) return INT32_MAX;
+    if (v <= -2147483648.0f) return INT32_MIN;
+    return (int)v;
+}
+
+// Centre cell of a position in the GridInfo convention (2D_LOM.wgsl:167-178 applied
+// to the centre): column from the left edge, row from the top edge, clamped.
+__device__ __forceinline__ int cell_of(const SceneDev& S, float x, float y, int& cx, int& cy) {
+    cx = min(max(f2i((x - S.base_x) / S.cs), 0), S.w - 1);
+    cy = min(max(f2i((S.base_y - y) / S.cs), 0), S.h - 1);
+    return cy * S.w + cx;
+}
+
+struct Self {
+    float x, y, r, drot, dx, dy;
+    float kn, ks;
+};
+
+struct Data3 {
+    float d0, d1, d2;  // data[0..2] of the calling particle
+};
+
+// ---- force laws (2D_Simulation.wgsl:264-432); `a` is the calling particle ----
+
+// distance2 (:273-284): |p_a - p_b| - L
+template <bool F64>
+__device__ __forceinline__ float dist2(const Self& a, float bx, float by, float L) {
+    if (F64) {
+        double dx = (double)a.x - (double)bx, dy = (double)a.y - (double)by;
+        return (float)(lend(dx, dy) - (double)L);
+    }
+    return lenf(a.x - bx, a.y - by) - L;
+}
+
+// distance (:264-271): |p_a - p_b| - (r_a + r_b)
+template <bool F64>
+__device__ __forceinline__ float dist1(const Self& a, const float4& pb) {
+    if (F64) {
+        double dx = (double)a.x - (double)pb.x, dy = (double)a.y - (double)pb.y;
+        return (float)(lend(dx, dy) - ((double)a.r + (double)pb.z));
+    }
+    return lenf(a.x - pb.x, a.y - pb.y) - (a.r + pb.z);
+}
+
+// relative tangential displacement, :352-360
+__device__ __forceinline__ float rel_tangent(const Self& a, const float4& pb, const float2& db, float tx, float ty) {
+    float dra = a.drot * a.r;
+    float drb = pb.w * pb.z;
+    float rtx = db.x - a.dx, rty = db.y - a.dy;
+    float rr = drb + dra;
+    return (rtx * tx + rty * ty) + rr;
+}
+
+// linear_model, :338-373.  L = r_a + r_b (contact) or the bond length.
+template <bool F64>
+__device__ __forceinline__ float3 linear_model(float mu, const Self& a, const float4& pb, const float2& db, float kn_b,
+                                               float ks_b, float L, float& ft, Data3& dd) {
+    float nd = fminf(0.0f, dist2<F64>(a, pb.x, pb.y, L));
+    if (nd == 0.0f) return make_float3(0.0f, 0.0f, 0.0f);
+    float kn = 1.0f / (1.0f / a.kn + 1.0f / kn_b);
+    float ks = 1.0f / (1.0f / a.ks + 1.0f / ks_b);
+    float nf = -nd * kn;
+    float dx = a.x - pb.x, dy = a.y - pb.y;
+    float l = lenf(dx, dy);
+    float nx = dx / l, ny = dy / l;
+    float tx = -ny, ty = nx;
+    float rt = rel_tangent(a, pb, db, tx, ty);
+    float lim = fabsf(nf) * mu;
+    ft = fminf(fmaxf(ft + rt * ks, -lim), lim);
+    float m = -a.r * ft;
+    dd.d0 += nf;
+    dd.d1 += ft;
+    dd.d2 += m;
+    return make_float3(nx * nf + tx * ft, ny * nf + ty * ft, m);
+}
+
+struct BondCtx {
+    float L, sn, ss;
+};
+
+// Intact-bond laws: linear_parallel_bonds (mode 3, :286-336), linear_contact_bonds
+// (mode 2, :375-417), normal_bonds (mode 1, :419-432).  bf = {ft, fbt, theta_b}.
+// Sets `code` to the break code when the bond tears.
+template <bool F64>
+__device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, const float4& pb, const float2& db,
+                                           float kn_b, float ks_b, const BondCtx& bc, float4& bf, Data3& dd,
+                                           int& code) {
+    code = 0;
+    if (u.bonds == 3) {
+        float kn = u.bond_normal_stiffness;
+        float ks = u.bond_shear_stiffness;
+        float R = 0.5f * fminf(a.r, pb.z);
+        float A = 2.0f * R * 1.0f;
+        float I = 2.0f / 3.0f * R * R * R;
+        float nd = dist2<F64>(a, pb.x, pb.y, bc.L);
+        float nf = -kn * nd * A;
+        float dx = a.x - pb.x, dy = a.y - pb.y;
+        float l = lenf(dx, dy);
+        float nx = dx / l, ny = dy / l;
+        float tx = -ny, ty = nx;
+        float rt = rel_tangent(a, pb, db, tx, ty);
+        bf.y = bf.y + rt * ks * A;
+        float dtb = a.drot - pb.w;
+        bf.z = bf.z + dtb;
+        float m = -kn * I * bf.z - bf.y * a.r;
+        float fx = nx * nf + tx * bf.y, fy = ny * nf + ty * bf.y;
+        if (u.bonds_tear == 1 &&
+            (nf / A - u.moment_contribution_factor * (fabsf(-kn * I * bf.z) * R) / I < -bc.sn ||
+             fabsf(bf.y) / A > bc.ss)) {
+            code = -10 + sign_i(nd);
+            nf = 0.0f;
+            bf.y = 0.0f;
+            fx = 0.0f;
+            fy = 0.0f;
+            m = 0.0f;
+        }
+        dd.d0 += -nf;
+        dd.d1 += bf.y;
+        dd.d2 += m;
+        float3 lm = linear_model<F64>(u.friction_coefficient, a, pb, db, kn_b, ks_b, bc.L, bf.x, dd);
+        return make_float3(fx + lm.x, fy + lm.y, m + lm.z);
+    } else if (u.bonds == 2) {
+        float nd = dist1<F64>(a, pb);
+        float kn = 1.0f / (1.0f / u.bond_normal_stiffness + 1.0f / u.bond_normal_stiffness);
+        float ks = 1.0f / (1.0f / u.bond_shear_stiffness + 1.0f / u.bond_shear_stiffness);
+        float nf = -nd * kn;
+        float dx = a.x - pb.x, dy = a.y - pb.y;
+        float l = lenf(dx, dy);
+        float nx = dx / l, ny = dy / l;
+        float tx = -ny, ty = nx;
+        float rt = rel_tangent(a, pb, db, tx, ty);
+        bf.x = bf.x + rt * ks;
+        float cd = u.contact_damping;
+        float fx = cd * (nx * nf + tx * bf.x), fy = cd * (ny * nf + ty * bf.x);
+        float m = -a.r * bf.x;
+        if (u.bonds_tear == 1 && (nf < -bc.sn || fabsf(bf.x) > bc.ss)) {
+            code = -10 + sign_i(nd);
+            nf = 0.0f;
+            bf.y = 0.0f;
+            fx = 0.0f;
+            fy = 0.0f;
+            m = 0.0f;
+        }
+        dd.d0 = -nf;
+        dd.d1 = bf.x;
+        dd.d2 = m;
+        return make_float3(fx, fy, m);
+    } else if (u.bonds == 1) {
+        float disp = dist1<F64>(a, pb);
+        float sf = u.bond_normal_stiffness * disp;
+        float dx = pb.x - a.x, dy = pb.y - a.y;
+        float l = lenf(dx, dy);
+        float fx = 0.0f + sf * (dx / l), fy = 0.0f + sf * (dy / l);
+        if (u.bonds_tear == 1 && sf >= bc.sn) {
+            code = -10 + sign_i(disp);
+            fx = 0.0f;
+            fy = 0.0f;
+        }
+        float3 lm = linear_model<F64>(u.friction_coefficient, a, pb, db, kn_b, ks_b, bc.L, bf.x, dd);
+        return make_float3(lm.x + fx, lm.y + fy, lm.z + 0.0f);
+    }
+    return make_float3(0.0f, 0.0f, 0.0f);
+}
+
+// ---- the fused step kernel ----
+
+template <bool F64, bool BONDS>
+__global__ void __launch_bounds__(256) k_step(StepArgs args) {
+    const DevArrays& a = args.a;
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    // clear the scan scratch for the scan that follows this launch
+    if (s <= a.ntiles) {
+        if (s < a.ntiles) a.TILES[s] = 0ull;
+        else *a.TICKET = 0u;
+    }
+    const int sid = args.blk_scene[blockIdx.x];
+    if (sid < 0) return;
+    const SceneDev& S = args.scenes[sid];
+    const int2 lfw = args.lf[sid];
+    const uint32_t fl = (uint32_t)lfw.x;
+    if (!(fl & SF_ACTIVE)) return;
+    if (s < S.p0 || s >= S.p0 + S.np) return;
+    const int cur = args.cur, nxt = cur ^ 1;
+    const dem_uniform& u = S.u;
+    const float dt = u.dT;
+
+    const int2 mw = a.M[s];
+    const uint32_t pf = (uint32_t)mw.x;
+    float4 p = a.P[cur][s];
+    float2 d = a.D[cur][s];
+    float4 v = a.V[s];
+
+    if (pf & PF_INACTIVE) {
+        // never binned into a real cell, never a partner; state kept in both buffers
+        if ((fl & SF_SIM) && S.record) {
+            const float4 d0 = a.DATA0[s];
+            a.DATA0[s] = make_float4(0.0f, 0.0f, 0.0f, d0.w);
+        }
+        if (fl & SF_LOM) {
+            a.P[nxt][s] = p;
+            a.D[nxt][s] = d;
+            if (fl & SF_BIN) {
+                const int key = S.c0 + S.w * S.h;
+                a.KEY[s] = key;
+                a.RANK[s] = atomicAdd(&a.CNT[key], 1);
+            }
+        }
+        return;
+    }
+
+    const int mat = (int)(pf & PF_MAT_MASK);
+    const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
+    float ax, ay, al;
+
+    if (fl & SF_SIM) {
+        Self me;
+        me.x = p.x; me.y = p.y; me.r = p.z; me.drot = p.w; me.dx = d.x; me.dy = d.y;
+        me.kn = MA.normal_stiffness; me.ks = MA.shear_stiffness;
+        Data3 dd{0.0f, 0.0f, 0.0f};
+        float nfx = 0.0f, nfy = 0.0f, nm = 0.0f;
+        const bool narrow = (fl & SF_NARROW) != 0;
+        const int64_t cap = a.cap;
+        const int step = lfw.y;
+        const int oid = mw.y;
+
+        int b_lo = 0, b_hi = 0;
+        if (BONDS) {
+            b_lo = a.BOFF[S.p0 + oid];
+            b_hi = a.BOFF[S.p0 + oid + 1];
+        }
+
+        // 1. contact set: partners (scene-local ids) ascending, with their slots
+        int lo[KMAX];
+        int ls[KMAX];
+        float lf[KMAX];
+        int n = 0;
+        const int oc = a.CC[s];
+        if (narrow) {
+            int cx, cy;
+            cell_of(S, p.x, p.y, cx, cy);
+            const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
+            const int pos0 = S.p0 - a.START[S.c0];  // cell-list position -> slot-list index
+            for (int ry = max(cy - 1, 0); ry <= min(cy + 1, S.h - 1); ry++) {
+                const int e0 = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
+                const int e1 = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
+                for (int e = e0; e < e1; e++) {
+                    const int j = a.SIDX[e];
+                    if (j == s || j < S.p0 || j >= S.p0 + S.np) continue;
+                    const float4 pj = a.P[cur][j];
+                    // narrow-phase predicate, 2D_Simulation.wgsl:159
+                    if (!(lenf(pj.x - p.x, pj.y - p.y) < (pj.z + p.z))) continue;
+                    const int oj = a.M[j].y;
+                    if (BONDS) {
+                        bool bonded_partner = false;
+                        for (int q = b_lo; q < b_hi; q++) {
+                            const int4 br = a.BR[q];
+                            if (br.w != 0 && br.x == oj) bonded_partner = true;
+                        }
+                        if (bonded_partner) continue;
+                    }
+                    // insert (oj, j) keeping lo[] ascending; the largest drops on overflow
+                    int pos = 0;
+#pragma unroll
+                    for (int q = 0; q < KMAX; q++) pos += (q < n && lo[q] < oj) ? 1 : 0;
+                    if (pos >= KMAX) {
+                        atomicAdd(&a.STAT[0], 1ull);
+                        continue;
+                    }
+                    if (n == KMAX) atomicAdd(&a.STAT[0], 1ull);
+#pragma unroll
+                    for (int q = KMAX - 1; q > 0; q--) {
+                        if (q > pos && q <= n) {
+                            lo[q] = lo[q - 1];
+                            ls[q] = ls[q - 1];
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < KMAX; q++) {
+                        if (q == pos) {
+                            lo[q] = oj;
+                            ls[q] = j;
+                        }
+                    }
+                    n = min(n + 1, KMAX);
+                }
+            }
+            // tangential history from the previous list (both ascending)
+            int pp = 0;
+#pragma unroll
+            for (int q = 0; q < KMAX; q++) {
+                if (q < n) {
+                    float f = 0.0f;
+                    while (pp < oc && a.CP[(int64_t)pp * cap + s] < lo[q]) pp++;
+                    if (pp < oc && a.CP[(int64_t)pp * cap + s] == lo[q]) f = a.CF[(int64_t)pp * cap + s];
+                    lf[q] = f;
+                }
+            }
+        } else {
+            n = min(oc, KMAX);
+#pragma unroll
+            for (int q = 0; q < KMAX; q++) {
+                if (q < n) {
+                    lo[q] = a.CP[(int64_t)q * cap + s];
+                    lf[q] = a.CF[(int64_t)q * cap + s];
+                    ls[q] = a.INV[S.p0 + lo[q]];
+                }
+            }
+        }
+
+        // 2. forces over bond records and contacts merged in ascending partner id
+        int bq = b_lo;
+        auto eval_record = [&](int q) {
+            int4 br = a.BR[q];
+            if (br.w == 0) return;
+            float4 bf = a.BF[q];
+            const int j = a.INV[S.p0 + br.x];
+            const float4 pb = a.P[cur][j];
+            const float2 db = a.D[cur][j];
+            int bonded = br.z;
+            if (narrow) {
+                const bool found = lenf(pb.x - p.x, pb.y - p.y) < (pb.z + p.z);
+                if (!found && (bonded < 0 || u.bonds == 0)) {
+                    br.w = 0;
+                    a.BR[q] = br;
+                    return;
+                } else if (!found && bonded > 0 && u.bonds <= 1) {
+                    bf.x = 0.0f;
+                }
+            }
+            const int bid = br.y;
+            const bool bid_ok = bid >= 0 && bid < S.nb;
+            if (bonded >= 0) {
+                // a break recorded by the partner in an earlier step is seen here
+                const int2 bs = bid_ok ? a.BS[S.b0 + bid] : make_int2(0, 0);
+                if (bs.x < 0 && bs.y < step) bonded = bs.x;
+            }
+            const int mb = a.M[j].x & PF_MAT_MASK;
+            const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
+            float3 f;
+            if (bonded < 0 || u.bonds == 0) {
+                f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness, MB.shear_stiffness,
+                                      me.r + pb.z, bf.x, dd);
+            } else if (u.bonds >= 1 && u.bonds <= 3) {
+                BondCtx bc;
+                bc.L = bid_ok ? a.BL[S.b0 + bid] : 0.0f;
+                if (S.has_strength && bid_ok) {
+                    const float2 st = a.BSTR[S.b0 + bid];
+                    bc.sn = st.x;
+                    bc.ss = st.y;
+                } else {
+                    bc.sn = u.bond_normal_strength;
+                    bc.ss = u.bond_shear_strength;
+                }
+                int code = 0;
+                f = bond_law<F64>(u, me, pb, db, MB.normal_stiffness, MB.shear_stiffness, bc, bf, dd, code);
+                if (code < 0) {
+                    bonded = code;
+                    if (bid_ok) a.BS[S.b0 + bid] = make_int2(code, step);
+                }
+            } else {
+                f = make_float3(0.0f, 0.0f, 0.0f);
+            }
+            nfx += f.x;
+            nfy += f.y;
+            nm += f.z;
+            br.z = bonded;
+            a.BR[q] = br;
+            a.BF[q] = bf;
+        };
+
+#pragma unroll
+        for (int q = 0; q < KMAX; q++) {
+            if (q < n) {
+                if (BONDS) {
+                    while (bq < b_hi && a.BR[bq].x < lo[q]) eval_record(bq++);
+                }
+                const int j = ls[q];
+                const float4 pb = a.P[cur][j];
+                const float2 db = a.D[cur][j];
+                const int mb = a.M[j].x & PF_MAT_MASK;
+                const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
+                const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness,
+                                                   MB.shear_stiffness, me.r + pb.z, lf[q], dd);
+                nfx += f.x;
+                nfy += f.y;
+                nm += f.z;
+            }
+        }
+        if (BONDS) {
+            while (bq < b_hi) eval_record(bq++);
+        }
+        // 3. contact list written back in place (the old list was fully read)
+#pragma unroll
+        for (int q = 0; q < KMAX; q++) {
+            if (q < n) {
+                if (narrow) a.CP[(int64_t)q * cap + s] = lo[q];
+                a.CF[(int64_t)q * cap + s] = lf[q];
+            }
+        }
+        if (narrow && n != oc) a.CC[s] = n;
+
+        // 4. store_forces, 2D_Simulation.wgsl:434-478
+        const float density = MA.density;
+        const float rr = p.z;
+        const float mass = density * PI_F * rr * rr;
+        const float inertia = 0.5f * mass * rr * rr;
+        float fx, fy, mm;
+        if (pf & PF_EXT) {
+            const float4 e = a.EXT[s];
+            fx = nfx + e.x;
+            fy = nfy + e.y;
+            mm = nm + e.z;
+        } else {
+            fx = nfx + 0.0f;
+            fy = nfy + 0.0f;
+            mm = nm + 0.0f;
+        }
+        if (u.gravity == 1 && u.planet_mode == 1) {
+            float gx = 0.0f, gy = 0.0f;
+            if (u.mouse_gravity == 1) {
+                gx = u.gravity_x;
+                gy = u.gravity_y;
+            }
+            const float ddx = gx - p.x, ddy = gy - p.y;
+            if (ddx != 0.0f || ddy != 0.0f) {
+                const float l = lenf(ddx, ddy);
+                fx = fx + ddx / l * G_F * u.gravity_acc * mass;
+                fy = fy + ddy / l * G_F * u.gravity_acc * mass;
+            }
+        } else if (u.gravity == 1) {
+            const float ga = G_F * u.gravity_acc;
+            fx = fx + 0.0f;
+            fy = fy + -ga * mass;
+        }
+        ax = fx / mass;
+        ay = fy / mass;
+        al = mm / inertia;
+        if (pf & PF_KICK_X) v.x = v.x + 0.5f * ax * dt;
+        if (pf & PF_KICK_Y) v.y = v.y + 0.5f * ay * dt;
+        if (pf & PF_KICK_R) v.z = v.z + 0.5f * al * dt;
+        if (S.record) {
+            const float4 d0 = a.DATA0[s];
+            a.DATA0[s] = make_float4(dd.d0, dd.d1, dd.d2, d0.w);
+        }
+        if (!(fl & SF_LOM)) {
+            a.V[s] = v;
+            a.A[s] = make_float4(ax, ay, al, 0.0f);
+            return;
+        }
+    } else {
+        const float4 av = a.A[s];
+        ax = av.x;
+        ay = av.y;
+        al = av.z;
+    }
+
+    // ---- LOM half, 2D_LOM.wgsl:85-152 ----
+    float ivx = v.x, ivy = v.y, iw = v.z;
+    if (pf & PF_KICK_X) ivx += ax * dt * 0.5f;
+    if (pf & PF_KICK_Y) ivy += ay * dt * 0.5f;
+    if (pf & PF_KICK_R) iw += al * dt * 0.5f;
+    if (S.record) a.DATA1[s] = make_float4(ivx, ivy, iw, 0.0f);
+    float dpx = d.x, dpy = d.y, dr = p.w;
+    if (!(fl & SF_DELOVR)) {
+        dpx = ivx * dt;
+        dpy = ivy * dt;
+        dr = iw * dt;
+    }
+    float px = p.x, py = p.y;
+    const float rr = p.z;
+    const float npx = px + dpx, npy = py + dpy;
+    if (u.round_bounds == 0) {
+        const float yH = u.vert_bound / 2.0f;
+        const float xW = u.hor_bound / 2.0f;
+        if (pf & PF_WALL_Y) {
+            if (npy - rr < -yH) {
+                ivy = -ivy * 0.5f;
+                py += -yH - (npy - rr);
+            } else if (npy + rr > yH) {
+                ivy = -ivy * 0.5f;
+                py -= (npy + rr) - yH;
+            }
+        }
+        if (pf & PF_WALL_X) {
+            if (npx - rr < -xW) {
+                ivx = -ivx * 0.5f;
+                px += -xW - (npx - rr);
+            } else if (npx + rr > xW) {
+                ivx = -ivx * 0.5f;
+                px -= (npx + rr) - xW;
+            }
+        }
+    } else if (lenf(npx, npy) + rr > u.bound_radius) {
+        const float l = lenf(npx, npy);
+        const float nx = npx / l, ny = npy / l;
+        const float dd = dpx * nx + dpy * ny;
+        const float dcx = dd * nx, dcy = dd * ny;
+        const float vv = ivx * nx + ivy * ny;
+        const float cvx = vv * nx, cvy = vv * ny;
+        if (pf & PF_WALL_X) {
+            px = nx * (u.bound_radius - rr) - dcx;
+            ivx = -cvx * 0.5f;
+        }
+        if (pf & PF_WALL_Y) {
+            py = ny * (u.bound_radius - rr) - dcy;
+            ivy = -cvy * 0.5f;
+        }
+    }
+    px += dpx;
+    py += dpy;
+    const float rot = v.w + dr;
+    a.P[nxt][s] = make_float4(px, py, rr, dr);
+    a.D[nxt][s] = make_float2(dpx, dpy);
+    a.V[s] = make_float4(ivx, ivy, iw, rot);
+    if (fl & SF_BIN) {
+        int cx, cy;
+        const int key = S.c0 + cell_of(S, px, py, cx, cy);
+        a.KEY[s] = key;
+        a.RANK[s] = atomicAdd(&a.CNT[key], 1);
+    }
+}
+
+// Binning of the current positions only (parity views and the bond builder).
+__global__ void k_bin(StepArgs args) {
+    const DevArrays& a = args.a;
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s <= a.ntiles) {
+        if (s < a.ntiles) a.TILES[s] = 0ull;
+        else *a.TICKET = 0u;
+    }
+    const int sid = args.blk_scene[blockIdx.x];
+    if (sid < 0) return;
+    const SceneDev& S = args.scenes[sid];
+    if (!(args.lf[sid].x & SF_ACTIVE)) return;
+    if (s < S.p0 || s >= S.p0 + S.np) return;
+    const int2 m = a.M[s];
+    int key;
+    if (m.x & PF_INACTIVE) {
+        key = S.c0 + S.w * S.h;
+    } else {
+        const float4 p = a.P[args.cur][s];
+        int cx, cy;
+        key = S.c0 + cell_of(S, p.x, p.y, cx, cy);
+    }
+    a.KEY[s] = key;
+    a.RANK[s] = atomicAdd(&a.CNT[key], 1);
+}
+
+// ---- k_scan: exclusive prefix of cnt[0..n) into start[0..n], zeroing cnt ----
+// Single pass with decoupled look-back.  Each block takes a ticket so every block it
+// waits on has already started; every wait is bounded (SPIN_LIMIT) and counted.
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+constexpr unsigned long long FLAG_AGG = 1ull << 62, FLAG_PRE = 2ull << 62, VAL_MASK = (1ull << 62) - 1;
+constexpr int SPIN_LIMIT = 1 << 22;
+
+__device__ __forceinline__ int wave_incl_scan(int x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan(int32_t* cnt, int32_t* start, int n,
+                                                       unsigned long long* tiles, unsigned int* ticket,
+                                                       unsigned long long* stat) {
+    __shared__ int s_tile;
+    __shared__ int s_wave[SCAN_THREADS / 64];
+    __shared__ int s_prefix;
+    if (threadIdx.x == 0) s_tile = (int)atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int tile = s_tile;
+    const int64_t base = (int64_t)tile * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        const int64_t i = base + k;
+        v[k] = i < n ? cnt[i] : 0;
+        sum += v[k];
+    }
+    const int incl = wave_incl_scan(sum);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) s_wave[wv] = incl;
+    __syncthreads();
+    int wave_off = 0, block_total = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_THREADS / 64; q++) {
+        if (q < wv) wave_off += s_wave[q];
+        block_total += s_wave[q];
+    }
+    const int thread_off = wave_off + incl - sum;
+    if (threadIdx.x == 0) {
+        int prefix = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&tiles[0], FLAG_PRE | (unsigned long long)(unsigned)block_total, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&tiles[tile], FLAG_AGG | (unsigned long long)(unsigned)block_total, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            int t = tile - 1;
+            int spins = 0;
+            while (t >= 0) {
+                const unsigned long long w =
+                    __hip_atomic_load(&tiles[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long f = w & ~VAL_MASK;
+                if (f == FLAG_PRE) {
+                    prefix += (int)(w & VAL_MASK);
+                    break;
+                } else if (f == FLAG_AGG) {
+                    prefix += (int)(w & VAL_MASK);
+                    t--;
+                } else if (++spins > SPIN_LIMIT) {
+                    atomicAdd(&stat[2], 1ull);
+                    break;
+                } else {
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __hip_atomic_store(&tiles[tile], FLAG_PRE | (unsigned long long)(unsigned)(prefix + block_total),
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = prefix;
+    }
+    __syncthreads();
+    int run = s_prefix + thread_off;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        const int64_t i = base + k;
+        if (i < n) {
+            start[i] = run;
+            cnt[i] = 0;
+        }
+        run += v[k];
+    }
+    if (tile == (int)gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) start[n] = s_prefix + block_total;
+}
+
+// ---- k_scatter: cell-ordered slot list (positions relative to the scene) ----
+__global__ void k_scatter(StepArgs args) {
+    const DevArrays& a = args.a;
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int sid = args.blk_scene[blockIdx.x];
+    if (sid < 0) return;
+    const SceneDev& S = args.scenes[sid];
+    const uint32_t fl = (uint32_t)args.lf[sid].x;
+    if (!(fl & SF_ACTIVE) || !(fl & SF_BIN)) return;
+    if (s < S.p0 || s >= S.p0 + S.np) return;
+    const int pos = a.START[a.KEY[s]] - a.START[S.c0] + S.p0 + a.RANK[s];
+    if (pos >= S.p0 && pos < S.p0 + S.np) a.SIDX[pos] = s;
+    else atomicAdd(&a.STAT[3], 1ull);
+}
+
+// ---- storage re-sort ----
+// 1. each cell's entries sorted by particle id: the canonical (cell, id) order
+__global__ void k_cell_sort(StepArgs args, int cell_lo, int cell_hi, int p0) {
+    const DevArrays& a = args.a;
+    const int c = cell_lo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= cell_hi) return;
+    // cells [cell_lo, cell_hi) of one scene whose slots start at p0
+    const int pos0 = p0 - a.START[cell_lo];
+    const int e0 = a.START[c] + pos0, e1 = a.START[c + 1] + pos0;
+    const int plim = p0 + (a.START[cell_hi] - a.START[cell_lo]);  // end of the scene's list
+    if (e0 < p0 || e1 > plim || e1 < e0) return;
+    for (int e = e0 + 1; e < e1; e++) {
+        const int v = a.SIDX[e];
+        const int key = a.M[v].y;
+        int f = e - 1;
+        while (f >= e0 && a.M[a.SIDX[f]].y > key) {
+            a.SIDX[f + 1] = a.SIDX[f];
+            f--;
+        }
+        a.SIDX[f + 1] = v;
+    }
+}
+
+// 2. move every per-particle array to its sorted position: slot j takes the particle
+//    at position j of the scene's sorted list (the scene's cells, sentinel included,
+//    hold exactly its np slots, so the list is a permutation of the scene's slots).
+__global__ void k_reorder_move(ReorderArgs r) {
+    const int sid = r.blk_scene[blockIdx.x];
+    if (sid < 0) return;
+    const SceneDev& S = r.scenes[sid];
+    const uint32_t fl = (uint32_t)r.lf[sid].x;
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j < S.p0 || j >= S.p0 + S.np) return;
+    const bool move = (fl & SF_ACTIVE) && (fl & SF_BIN);
+    int s = move ? r.src.SIDX[j] : j;
+    if (s < S.p0 || s >= S.p0 + S.np) s = j;
+    const int cur = r.cur;
+    r.dst.P[cur][j] = r.src.P[cur][s];
+    r.dst.D[cur][j] = r.src.D[cur][s];
+    r.dst.V[j] = r.src.V[s];
+    r.dst.A[j] = r.src.A[s];
+    const int2 m = r.src.M[s];
+    r.dst.M[j] = m;
+    if (r.with_ext) r.dst.EXT[j] = r.src.EXT[s];
+    if (r.with_data) {
+        r.dst.DATA0[j] = r.src.DATA0[s];
+        r.dst.DATA1[j] = r.src.DATA1[s];
+    }
+    const int c = r.src.CC[s];
+    r.dst.CC[j] = c;
+    for (int q = 0; q < c && q < KMAX; q++) {
+        r.dst.CP[(int64_t)q * r.dst.cap + j] = r.src.CP[(int64_t)q * r.src.cap + s];
+        r.dst.CF[(int64_t)q * r.dst.cap + j] = r.src.CF[(int64_t)q * r.src.cap + s];
+    }
+    r.dst.INV[S.p0 + m.y] = j;
+}
+
+// 3. the sorted list becomes the identity
+__global__ void k_identity(int32_t* sidx, int64_t n) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) sidx[j] = (int32_t)j;
+}
+
+// ---- observables ----
+__global__ void k_kinetic(DevArrays a, int cur, int m0, int nm, int p0, int np, double* out_by_id) {
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= np) return;
+    const int slot = p0 + s;
+    const int2 m = a.M[slot];
+    double e = 0.0;
+    if (!(m.x & PF_INACTIVE)) {
+        const int mat = m.x & PF_MAT_MASK;
+        const double rr = a.P[cur][slot].z;
+        const double mass = (double)a.MAT[m0 + (mat < nm ? mat : 0)].density * 3.141592653589793 * rr * rr;
+        const double inertia = 0.5 * mass * rr * rr;
+        const float4 v = a.V[slot];
+        const double vx = v.x, vy = v.y, w = v.z;
+        e = 0.5 * mass * (vx * vx + vy * vy) + 0.5 * inertia * w * w;
+    }
+    if (m.y >= 0 && m.y < np) out_by_id[m.y] = e;
+}
+
+// Sum of the contact counts of a scene (n̄_c) - integer, order independent.
+__global__ void k_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out) {
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    int c = s < np ? cc[p0 + s] : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
+// ---- bond builder (State::regen_bonds rule on the cell list) ----
+// For particle id i: every j > i with sqrt((xj-xi)^2 + (yj-yi)^2) < (ri + rj) * factor,
+// the `maxb` smallest j kept.  Stencil of (2*reach+1)^2 cells.
+__global__ void k_bond_pairs(StepArgs args, int reach, int maxb, float factor, int32_t* out_cnt, int32_t* out_j) {
+    const DevArrays& a = args.a;
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int sid = args.blk_scene[blockIdx.x];
+    if (sid < 0) return;
+    const SceneDev& S = args.scenes[sid];
+    if (s < S.p0 || s >= S.p0 + S.np) return;
+    const int2 m = a.M[s];
+    const int i = m.y;
+    if (m.x & PF_INACTIVE) {
+        if (i < S.n_real) out_cnt[i] = 0;
+        return;
+    }
+    const float4 p = a.P[args.cur][s];
+    int cx, cy;
+    cell_of(S, p.x, p.y, cx, cy);
+    const int pos0 = S.p0 - a.START[S.c0];
+    int best[16];
+    int n = 0;
+    for (int ry = max(cy - reach, 0); ry <= min(cy + reach, S.h - 1); ry++) {
+        const int e0 = max(a.START[S.c0 + ry * S.w + max(cx - reach, 0)] + pos0, S.p0);
+        const int e1 = min(a.START[S.c0 + ry * S.w + min(cx + reach, S.w - 1) + 1] + pos0, S.p0 + S.np);
+        for (int e = e0; e < e1; e++) {
+            const int js = a.SIDX[e];
+            const int j = a.M[js].y;
+            if (j <= i) continue;
+            const float4 q = a.P[args.cur][js];
+            const float dx = q.x - p.x, dy = q.y - p.y;
+            const float dist = sqrtf(dx * dx + dy * dy);
+            if (!(dist < (p.z + q.z) * factor)) continue;
+            // keep the maxb smallest j, ascending
+            int pos = 0;
+            for (int k = 0; k < n; k++) pos += best[k] < j ? 1 : 0;
+            if (pos >= maxb) continue;
+            for (int k = min(n, maxb - 1); k > pos; k--) best[k] = best[k - 1];
+            best[pos] = j;
+            n = min(n + 1, maxb);
+        }
+    }
+    out_cnt[i] = n;
+    for (int k = 0; k < n; k++) out_j[(int64_t)i * maxb + k] = best[k];
+}
+
+// ---- launchers (host context) ----
+
+hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, hipStream_t st) {
+    dim3 g(nblocks), b(block);
+    if (f64) {
+        if (bonds) hipLaunchKernelGGL((k_step<true, true>), g, b, 0, st, args);
+        else hipLaunchKernelGGL((k_step<true, false>), g, b, 0, st, args);
+    } else {
+        if (bonds) hipLaunchKernelGGL((k_step<false, true>), g, b, 0, st, args);
+        else hipLaunchKernelGGL((k_step<false, false>), g, b, 0, st, args);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bin(const StepArgs& args, int nblocks, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_bin, dim3(nblocks), dim3(block), 0, st, args);
+    return hipGetLastError();
+}
+
+int scan_tiles(int n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+hipError_t launch_scan(const DevArrays& a, int ncells, hipStream_t st) {
+    hipLaunchKernelGGL(k_scan, dim3(scan_tiles(ncells)), dim3(SCAN_THREADS), 0, st, a.CNT, a.START, ncells, a.TILES,
+                       a.TICKET, a.STAT);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const StepArgs& args, int nblocks, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(block), 0, st, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_sort(const StepArgs& args, int cell_lo, int cell_hi, int p0, hipStream_t st) {
+    const int b = 256;
+    const int n = cell_hi - cell_lo;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cell_sort, dim3((n + b - 1) / b), dim3(b), 0, st, args, cell_lo, cell_hi, p0);
+    return hipGetLastError();
+}
+
+hipError_t launch_reorder_move(const ReorderArgs& r, int nblocks, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_reorder_move, dim3(nblocks), dim3(block), 0, st, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_identity(int32_t* sidx, int64_t n, hipStream_t st) {
+    const int b = 256;
+    hipLaunchKernelGGL(k_identity, dim3((unsigned)((n + b - 1) / b)), dim3(b), 0, st, sidx, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, int np, double* out, hipStream_t st) {
+    const int b = 256;
+    hipLaunchKernelGGL(k_kinetic, dim3((np + b - 1) / b), dim3(b), 0, st, a, cur, m0, nm, p0, np, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out, hipStream_t st) {
+    const int b = 256;
+    hipLaunchKernelGGL(k_count_contacts, dim3((np + b - 1) / b), dim3(b), 0, st, cc, p0, np, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
+                             int32_t* out_cnt, int32_t* out_j, hipStream_t st) {
+    hipLaunchKernelGGL(k_bond_pairs, dim3(nblocks), dim3(block), 0, st, args, reach, maxb, factor, out_cnt, out_j);
+    return hipGetLastError();
+}
+
+}  // namespace psim

@@ -1,0 +1,150 @@
+"""Seeded scene generators for the benchmark configurations.
+
+The reference builds scenes with `setup::grid` (src/setup.rs:24-50: a square or hex
+lattice, radii and velocities from the unseeded `rand::thread_rng`).  These
+generators build the same kind of lattice scenes from a seeded PCG64 stream
+(numpy.random.default_rng(seed)), so every run of a configuration starts from the
+same state.
+
+  granular_box     4,096 discs, r ~ U[0.5, 1.0], jittered 64 x 64 lattice of
+                   spacing 2.0 filling a 128 x 128 walled box, gravity -9.81,
+                   effective kn = 1e5 (material kn = ks = 2e5: kn_eff is the series
+                   combination 1/(1/kn_a + 1/kn_b), 2D_Simulation.wgsl:345-346),
+                   mu = 0.5, density 1, dt = 1e-4.
+  column_collapse  polydisperse column collapsing into a wide box; the full size is
+                   4,194,304 discs in a 4096 x 2048 box.  Radii are log-normal
+                   (median 0.35, sigma 0.25, clipped to [0.175, 0.525]): the stated
+                   radii (median 1.0, clip [0.5, 1.5]) scaled by 0.35, because at
+                   full size 4.19M discs of median radius 1 cover 1.75 times the
+                   box.  Hex column of spacing 2 r_max = 1.05, 2048 discs per row.
+  bonded_chains    horizontal chains of r = 0.5 discs at spacing 1.0, neighbours
+                   bonded (parallel bonds), ends pinned, hit by a dense r = 8 disc.
+  bonded_assembly  dense packing of r ~ U[0.8, 1.2] discs bonded where the gap is
+                   below 0.08 (0.1 r_min), top and bottom rows pulled apart.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .settings import Settings, f32
+from .state import State
+
+
+def _base_settings(n, width, height, dt, rmax, rmin, kn=2e5, ks=2e5, density=1.0, mu=0.5) -> Settings:
+    s = Settings()
+    s.setup.particles = n
+    s.setup.max_radius = f32(rmax)
+    s.setup.min_radius = f32(rmin)
+    s.setup.variable_rad = rmin != rmax
+    s.simulation.hor_bound = f32(width)
+    s.simulation.vert_bound = f32(height)
+    s.simulation.timestep = f32(dt)
+    s.simulation.gen_per_frame = 1
+    s.physics.gravity = True
+    s.physics.planet_mode = False
+    s.physics.gravity_acceleration = 1.0
+    s.physics.collisions = True
+    s.physics.collision_interval = 1
+    s.physics.friction_coefficient = f32(mu)
+    s.physics.bonds = 0
+    s.materials = [1.0, 1.0, 1.0, float(density), float(kn), float(ks)]
+    return s
+
+
+def _new_state(n, max_contacts=14) -> State:
+    st = State(max(n, 1), max_contacts)
+    st.p_count = n
+    return st
+
+
+def granular_box(n_side: int = 64, seed: int = 0, spacing: float = 2.0, rmin: float = 0.5, rmax: float = 1.0,
+                 dt: float = 1e-4, mu: float = 0.5):
+    """Plain granular box settle (4,096 discs at n_side = 64)."""
+    rng = np.random.default_rng(seed)
+    n = n_side * n_side
+    box = n_side * spacing
+    s = _base_settings(n, box, box, dt, rmax, rmin, mu=mu)
+    st = _new_state(n)
+    r = rng.uniform(rmin, rmax, n).astype(np.float32)
+    ii, jj = np.meshgrid(np.arange(n_side), np.arange(n_side), indexing="xy")
+    cx = (-box / 2 + spacing / 2 + ii.ravel() * spacing).astype(np.float64)
+    cy = (-box / 2 + spacing / 2 + jj.ravel() * spacing).astype(np.float64)
+    room = (spacing / 2 - r).astype(np.float64)
+    jx = rng.uniform(-1.0, 1.0, n) * room
+    jy = rng.uniform(-1.0, 1.0, n) * room
+    st.pos[0::2][:n] = (cx + jx).astype(np.float32)
+    st.pos[1::2][:n] = (cy + jy).astype(np.float32)
+    st.radii[:n] = r
+    return s, st
+
+
+def column_collapse(n: int = 4194304, per_row: int = 2048, seed: int = 2, width: float = 4096.0,
+                    height: float = 2048.0, dt: float = 1e-4, mu: float = 0.5):
+    """Large polydisperse granular collapse (4,194,304 discs at the default size)."""
+    rng = np.random.default_rng(seed)
+    rmed, sigma, lo, hi = 0.35, 0.25, 0.175, 0.525
+    r = np.clip(rmed * np.exp(sigma * rng.standard_normal(n)), lo, hi).astype(np.float32)
+    spacing = 2 * hi
+    rows = int(math.ceil(n / per_row))
+    k = np.arange(n)
+    row, col = k // per_row, k % per_row
+    x = -width / 2 + hi + col * spacing + (row % 2) * (spacing / 2)
+    y = -height / 2 + hi + row * spacing * math.sqrt(3) / 2
+    if y.max() + hi > height / 2:
+        raise ValueError("column taller than the box: %d rows" % rows)
+    s = _base_settings(n, width, height, dt, hi, lo, mu=mu)
+    st = _new_state(n)
+    st.pos[0::2][:n] = x.astype(np.float32)
+    st.pos[1::2][:n] = y.astype(np.float32)
+    st.radii[:n] = r
+    return s, st
+
+
+def weibull(rng, k, lam, n):
+    return (lam * rng.weibull(k, n)).astype(np.float32)
+
+
+def bonded_chains(chains: int = 64, per_chain: int = 256, seed: int = 1, striker_r: float = 8.0,
+                  striker_v: float = 40.0, striker_density: float = 10.0, dt: float = 5e-5,
+                  kn: float = 1e5, ks: float = 1e5, strength_k: float = 6.0, strength_scale: float = 50.0):
+    """Bonded chains under impact.  Returns (settings, state, strengths); bonds are
+    built on the device with ComputeProg.regen_bonds (factor 1.05 of r_a + r_b)."""
+    rng = np.random.default_rng(seed)
+    r = 0.5
+    gap = 2.0
+    n = chains * per_chain + 1
+    width = per_chain * 1.0 + 40.0
+    height = chains * gap + 4 * striker_r + 40.0
+    s = _base_settings(n, width, height, dt, striker_r, r, kn=kn, ks=ks)
+    s.physics.bonds = 3
+    s.physics.bond_tearing = True
+    s.physics.bond_normal_stiffness = f32(kn)
+    s.physics.bond_shear_stiffness = f32(ks)
+    s.physics.gravity = False
+    s.materials = [1.0, 1.0, 1.0, 1.0, float(2 * kn), float(2 * ks),
+                   1.0, 0.0, 0.0, float(striker_density), float(2 * kn), float(2 * ks)]
+    st = _new_state(n)
+    k = np.arange(chains * per_chain)
+    c, i = k // per_chain, k % per_chain
+    x = -per_chain / 2 + 0.5 + i * 1.0
+    y = -chains * gap / 2 + c * gap
+    st.pos[0:2 * (n - 1):2] = x.astype(np.float32)
+    st.pos[1:2 * (n - 1):2] = y.astype(np.float32)
+    st.radii[: n - 1] = r
+    ends = (i == 0) | (i == per_chain - 1)
+    fx = st.fixity.reshape(-1, 6)
+    fx[: n - 1][ends, 0:3] = 1
+    fx[: n - 1][ends, 3:6] = 1
+    # striker: below the chains' middle, moving up
+    st.pos[2 * (n - 1)] = 0.0
+    st.pos[2 * (n - 1) + 1] = np.float32(-chains * gap / 2 - striker_r - 1.0)
+    st.radii[n - 1] = striker_r
+    st.vel[2 * (n - 1) + 1] = striker_v
+    st.material_pointers[n - 1] = 1
+    nb_est = chains * (per_chain - 1)
+    strengths = np.empty(2 * nb_est, dtype=np.float32)
+    strengths[0::2] = weibull(rng, strength_k, strength_scale, nb_est)
+    strengths[1::2] = weibull(rng, strength_k, strength_scale, nb_est)
+    return s, st, strengths

@@ -1,0 +1,134 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle and the
+reference's recorded traces.  Needs an MI355X."""
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from psim import scenes
+from psim.compute import ComputeProg
+from psim.scripts import ScriptReplay
+from psim.settings import Settings
+from psim.state import State
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+STATE_FIELDS = ("pos", "vel", "acc", "rot", "rot_vel", "rot_acc")
+
+
+def _n(name, n):
+    return 2 * n if name in ("pos", "vel", "acc") else n
+
+
+def assert_bits_equal(got: State, want: State, n, fields=STATE_FIELDS):
+    for name in fields:
+        a = getattr(got, name)[: _n(name, n)]
+        b = getattr(want, name)[: _n(name, n)]
+        if not np.array_equal(a.view(np.int32), b.view(np.int32)):
+            bad = np.nonzero(a.view(np.int32) != b.view(np.int32))[0]
+            raise AssertionError("%s differs at %d entries, first %d: %r vs %r (max abs diff %g)" % (
+                name, len(bad), bad[0], a[bad[0]], b[bad[0]], float(np.max(np.abs(a - b)))))
+
+
+def test_device_visible():
+    from psim import abi
+    assert abi.load_library().dem_device_count() >= 1
+
+
+@pytest.mark.parametrize("steps", [1, 7, 50])
+def test_granular_box_bit_exact(steps):
+    """4,096-disc box settle: positions, velocities, accelerations identical to the
+    oracle (canonical contact order) bit for bit after `steps` steps."""
+    s, st = scenes.granular_box(seed=0)
+    prog = ComputeProg(s, st.copy(), reorder_interval=4)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    prog.step(steps)
+    eng.step(steps)
+    got = prog.update_state()
+    assert_bits_equal(got, eng.st, st.p_count)
+    assert np.array_equal(got.data[: 7 * st.p_count].view(np.int32), eng.st.data[: 7 * st.p_count].view(np.int32))
+    c = prog.counters()
+    assert c["contact_overflow"] == 0 and c["scan_timeouts"] == 0
+    prog.close()
+
+
+def test_granular_contacts_dense_bit_exact():
+    """A compressed box (spacing 1.8 < 2 r_max) so that most discs touch several
+    neighbours; 30 steps bit-exact including the contact table."""
+    s, st = scenes.granular_box(n_side=32, seed=5, spacing=1.7)
+    prog = ComputeProg(s, st.copy(), reorder_interval=3)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL, slots=16)
+    prog.step(30)
+    eng.step(30)
+    got = prog.update_state()
+    n = st.p_count
+    assert_bits_equal(got, eng.st, n)
+    # contact table: canonical layout (occupied slots ascending by partner, -1 words after)
+    want = eng.canonical_contacts().view(np.int32).reshape(-1, 16, 6)[:n, :14]
+    have = got.contacts.view(np.int32).reshape(-1, 14, 6)[:n]
+    assert np.array_equal(have, want)
+    assert prog.counters()["contacts"] > n  # really in contact
+    prog.close()
+
+
+def test_stage_views_match_closed_forms():
+    """Stages 1-4: cell keys, (cell, id) order, cell starts and contact pairs equal
+    the oracle's closed forms / brute force."""
+    import ctypes as C
+    s, st = scenes.granular_box(n_side=32, seed=3, spacing=1.8)
+    prog = ComputeProg(s, st.copy())
+    prog.step(3)
+    got = prog.update_state()
+    n = st.p_count
+    keys, order, start = prog.debug_broad_phase()
+    g = prog.grid
+    L = orc.lib()
+    k_want = np.zeros(n, dtype=np.int32)
+    L.orc_cell_keys(n, n, got.pos.ctypes.data_as(orc.FP), got.radii.ctypes.data_as(orc.FP), C.byref(g),
+                    k_want.ctypes.data_as(orc.IP))
+    assert np.array_equal(keys, k_want)
+    o_want = np.zeros(n, dtype=np.int32)
+    s_want = np.zeros(g.w * g.h + 1, dtype=np.int32)
+    assert L.orc_sort_cells(n, k_want.ctypes.data_as(orc.IP), g.w * g.h, o_want.ctypes.data_as(orc.IP),
+                            s_want.ctypes.data_as(orc.IP)) == 0
+    assert np.array_equal(order, o_want)
+    assert np.array_equal(start, s_want)
+    # contact pairs of the last narrow phase (positions of the last LOM = current)
+    counts, partners = prog.debug_contacts(16)
+    c_want = np.zeros(n, dtype=np.int32)
+    p_want = np.zeros(n * 16, dtype=np.int32)
+    tot = L.orc_contact_pairs(n, got.pos.ctypes.data_as(orc.FP), got.radii.ctypes.data_as(orc.FP), 16,
+                              c_want.ctypes.data_as(orc.IP), p_want.ctypes.data_as(orc.IP))
+    assert tot > 0
+    assert np.array_equal(counts, c_want)
+    assert np.array_equal(partners, p_want.reshape(n, 16))
+    prog.close()
+
+
+def _replay(engine_factory, scene):
+    s = Settings()
+    st = State.load_file(os.path.join(GOLDEN, "saved_states", scene + ".bin"), s)
+    eng = engine_factory(st, s)
+    rep = ScriptReplay(eng, st.scripts)
+    return {p.replace("\\", "/").split("/")[-1][:-4]: rows for p, rows in rep.run()}
+
+
+@pytest.mark.parametrize("scene", ["TEXPS", "PBEXPS", "Chain_TEXP"])
+def test_experiment_replay_matches_oracle_and_traces(scene):
+    """The embedded experiment scripts replayed on the GPU (f64 distances, as the
+    fixtures were recorded): every exported row equals the oracle's bit for bit, and
+    the position/velocity columns follow the reference's CSV."""
+    gpu = _replay(lambda st, s: ComputeProg(s, st, record_data=True), scene)
+    cpu = _replay(lambda st, s: orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL), scene)
+    traces = np.load(os.path.join(GOLDEN, "traces.npz"))
+    assert gpu.keys() == cpu.keys()
+    for stem in gpu:
+        assert np.array_equal(gpu[stem], cpu[stem]), stem
+        if stem in traces.files:
+            ref = traces[stem]
+            n = min(len(ref), len(gpu[stem]))
+            for c in range(1, 7):
+                scale = max(np.abs(ref[:n, c]).max(), 1e-30)
+                assert np.abs(ref[:n, c] - gpu[stem][:n, c]).max() / scale < 1e-3, (stem, c)

@@ -1,0 +1,152 @@
+"""Host-side checks that need no GPU: the C ABI library exports, the saved-state
+codec, the settings packing and the scene generators."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from psim import abi, flatbuf, scenes
+from psim.compute import assemble_bonds, min_cell_size
+from psim.settings import Settings
+from psim.state import State
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "dem.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(dem_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = abi.load_library()
+    names = header_functions()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(abi.EXPORTS)
+
+
+def test_library_fails_loudly_without_device():
+    lib = abi.load_library()
+    if lib.dem_device_count() > 0:
+        pytest.skip("a device is visible")
+    s, st = scenes.granular_box(n_side=4)
+    from psim.compute import ComputeProg
+    with pytest.raises(abi.DemError):
+        ComputeProg(s, st)
+    # the raw entry point reports DEM_EDEVICE with a message, no abort
+    p = abi.DemParams()
+    p.u = s.collision_settings()
+    mats = abi.materials_array(s.materials)
+    p.materials = C.cast(mats, C.POINTER(abi.DemMaterial))
+    p.n_materials = 1
+    p.grid = s.box_grid()
+    ctx = C.c_void_p()
+    assert lib.dem_create(C.byref(p), 16, 0, C.byref(ctx)) == abi.DEM_EDEVICE
+    assert b"device" in lib.dem_last_error()
+
+
+def test_abi_struct_sizes():
+    assert C.sizeof(abi.DemUniform) == 116
+    assert C.sizeof(abi.DemMaterial) == 24
+    assert C.sizeof(abi.DemGrid) == 16
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "saved_states", "*.bin"))))
+def test_flatbuffer_roundtrip_bytes(path):
+    """load -> save reproduces the reference writer's bytes exactly."""
+    b = open(path, "rb").read()
+    st = flatbuf.read_state(b)
+    assert flatbuf.write_state(st.arrays, st.settings_raw, st.scripts) == b
+
+
+def test_flatbuffer_reference_states_if_present():
+    """Every saved state shipped with the reference (when the checkout is mounted)."""
+    files = sorted(glob.glob("/root/reference/saved_states/*.bin"))
+    if not files:
+        pytest.skip("reference checkout not mounted")
+    for f in files:
+        b = open(f, "rb").read()
+        st = flatbuf.read_state(b)
+        assert flatbuf.write_state(st.arrays, st.settings_raw, st.scripts) == b, f
+
+
+def test_state_load_save_roundtrip():
+    s = Settings()
+    path = os.path.join(GOLDEN, "saved_states", "PBEXPS.bin")
+    b = open(path, "rb").read()
+    st = State.load(b, s)
+    assert st.p_count == 2 and s.simulation.use_f64 and s.physics.bonds == 3
+    assert st.save(s) == b
+
+
+def test_collision_settings_word_order():
+    s = Settings()
+    s.simulation.hor_bound, s.simulation.vert_bound = 7.0, 5.0
+    s.simulation.timestep = 0.25
+    s.physics.friction_coefficient = 0.125
+    s.setup.particles = 42
+    u = s.collision_settings()
+    words = np.frombuffer(bytes(u), dtype=np.int32)
+    f = words.view(np.float32)
+    assert len(words) == 29
+    assert f[1] == 7.0 and f[2] == 5.0 and f[10] == 0.125 and f[19] == 0.25 and words[28] == 42
+    assert words[6] == 1 and words[7] == 1 and words[9] == 1   # defaults: gravity, planet mode, collisions
+
+
+def test_grid_info_matches_reference_formula():
+    import orc
+    s = Settings()
+    s.simulation.hor_bound, s.simulation.vert_bound = 3.2679124, 2.0
+    s.setup.max_radius, s.setup.min_radius, s.setup.variable_rad = 0.05, 0.02, True
+    s.setup.particles = 100
+    g = s.grid_info()
+    want = abi.DemGrid()
+    orc.lib().orc_grid_capacity(C.c_float(3.2679124), C.c_float(2.0), C.c_float(0.05), C.c_float(0.02), 1, 100,
+                                C.byref(want))
+    assert (g.cell_size, g.cell_cap, g.w, g.h) == (want.cell_size, want.cell_cap, want.w, want.h)
+
+
+def test_min_cell_size_covers_rounding():
+    cs = min_cell_size(0.525, 4096.0)
+    assert cs > 1.05 and cs < 1.06
+
+
+def test_granular_box_no_overlap_and_inside():
+    s, st = scenes.granular_box()
+    n = st.p_count
+    p = st.pos[: 2 * n].reshape(-1, 2)
+    r = st.radii[:n]
+    assert n == 4096 and r.min() >= 0.5 and r.max() <= 1.0
+    assert np.all(np.abs(p) + r[:, None] <= 64.0)
+    # lattice cells of side 2 keep every disc inside its own cell
+    cell = np.floor((p + 64.0) / 2.0)
+    assert len({tuple(c) for c in cell}) == n
+
+
+def test_column_collapse_small():
+    s, st = scenes.column_collapse(n=4096, per_row=64, width=256.0, height=128.0)
+    n = st.p_count
+    r = st.radii[:n]
+    assert r.min() >= 0.175 and r.max() <= 0.525
+    p = st.pos[: 2 * n].reshape(-1, 2)
+    assert np.all(p[:, 1] - r >= -64.0) and np.all(p[:, 0] - r >= -128.0)
+
+
+def test_assemble_bonds_slots_sorted_by_partner():
+    st = State(4)
+    st.p_count = 4
+    st.pos[:8] = [0, 0, 1, 0, 2, 0, 1, 1]
+    pairs = np.array([[0, 1], [1, 2], [1, 3]], dtype=np.int32)
+    bonds, contacts = assemble_bonds(st, pairs, 14)
+    assert bonds.reshape(-1, 3)[:, 0].tolist() == [1, 1, 1]
+    assert bonds.reshape(-1, 3)[:, 2].view(np.float32).tolist() == [1.0, 1.0, 1.0]
+    t = contacts.view(np.int32).reshape(4, 14, 6)
+    assert t[1, :3, 1].tolist() == [0, 2, 3] and t[1, :3, 5].tolist() == [0, 1, 2]
+    assert t[0, 0, :2].tolist() == [0, 1] and t[3, 0, 1] == 1 and t[3, 1, 1] == -1

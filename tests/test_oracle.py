@@ -1,0 +1,241 @@
+"""Pins the CPU oracle: against the reference's own recorded experiment traces
+(data/*.csv exported by the reference while running its saved scenes), against
+closed forms, and against brute force.  CPU only."""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from psim import abi, scenes
+from psim.scripts import ScriptReplay
+from psim.settings import Settings
+from psim.state import State
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _load(scene):
+    s = Settings()
+    st = State.load_file(os.path.join(GOLDEN, "saved_states", scene + ".bin"), s)
+    return s, st
+
+
+def _replay(scene, mode):
+    s, st = _load(scene)
+    eng = orc.OracleEngine(st, s, mode=mode)
+    rep = ScriptReplay(eng, st.scripts)
+    return {p.replace("\\", "/").split("/")[-1][:-4]: rows for p, rows in rep.run()}
+
+
+@pytest.fixture(scope="module")
+def traces():
+    return np.load(os.path.join(GOLDEN, "traces.npz"))
+
+
+def test_texp1_bit_exact(traces):
+    """TEXP_1 (two discs, parallel bond pulled apart at 1e-3 u/s, f64 distances):
+    every value of every row equals the reference's export."""
+    got = _replay("TEXPS", orc.MODE_REFERENCE)["TEXP_1"]
+    ref = traces["TEXP_1"]
+    assert got.shape[0] == ref.shape[0] == 101
+    assert np.array_equal(got[:, :11], ref[:, :11])
+
+
+@pytest.mark.parametrize("mode", [orc.MODE_REFERENCE, orc.MODE_CANONICAL])
+@pytest.mark.parametrize("scene,cols,tol", [
+    ("TEXPS", range(1, 11), 1e-3),        # all columns incl. force/moment sums
+    ("PBEXPS", range(1, 7), 1e-4),        # kinematics (this export predates the bond data columns)
+    ("Chain_EXP", range(1, 7), 1e-4),
+    ("Chain_TEXP", range(1, 11), 1e-3),
+])
+def test_traces_follow_reference(traces, scene, cols, tol, mode):
+    """All 13 experiments of each scene: the oracle's trace follows the
+    reference's within `tol` of each column's range (the residue is the
+    reference GPU's own float evaluation order)."""
+    got = _replay(scene, mode)
+    assert len(got) >= 13
+    for stem, rows in got.items():
+        ref = traces[stem]
+        n = min(len(ref), len(rows))
+        assert n >= 100
+        for c in cols:
+            scale = max(np.abs(ref[:n, c]).max(), 1e-30)
+            err = np.abs(ref[:n, c] - rows[:n, c]).max() / scale
+            assert err < tol, (stem, c, err)
+
+
+def test_texp_closed_form(traces):
+    """Parallel-bond tension: Δfn = kn A v dt per step, break at the closed-form step
+    floor(σ A / (kn A v dt)) + 1 (2D_Simulation.wgsl:296, :322)."""
+    rows = _replay("TEXPS", orc.MODE_CANONICAL)["TEXP_1"]
+    kn, A, v, dt = 1e11, 2 * 0.5 * 0.05, 1e-3, 4.999000339012127e-06
+    dfn = kn * A * v * dt
+    k_break = math.floor(1e4 * A / dfn) + 1
+    assert k_break == 21
+    fn = rows[1:, 7]
+    np.testing.assert_allclose(fn[: k_break - 1], dfn * np.arange(1, k_break), rtol=1e-5)
+    assert np.all(fn[k_break - 1:] == 0.0)
+
+
+def _granular(n_side=16, spacing=1.8, seed=4):
+    return scenes.granular_box(n_side=n_side, seed=seed, spacing=spacing)
+
+
+def test_canonical_equals_reference_pairs_and_forces():
+    """The canonical slot order (what the HIP path implements) and the WGSL's own
+    slot rules give the same contact set and per-particle forces within 1e-5 of
+    the force scale after one step."""
+    s, st = _granular()
+    a = orc.OracleEngine(st, s, mode=orc.MODE_REFERENCE, grid=s.grid_info())
+    b = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    a.step(1)
+    b.step(1)
+    n = st.p_count
+    ca = a.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n, :, 1]
+    cb = b.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n, :, 1]
+    assert np.array_equal(ca, cb)
+    assert (ca >= 0).sum() > n // 4
+    fa = a.st.acc[: 2 * n].astype(np.float64)
+    fb = b.st.acc[: 2 * n].astype(np.float64)
+    assert np.max(np.abs(fa - fb)) <= 1e-5 * np.max(np.abs(fa))
+    np.testing.assert_allclose(b.st.pos[: 2 * n], a.st.pos[: 2 * n], rtol=1e-5, atol=1e-6)
+
+
+def test_multistep_energy_matches_reference_mode():
+    """Chaotic multi-step run: kinetic energy of canonical vs reference slot order
+    within 1% over 200 steps."""
+    s, st = _granular(spacing=1.7)
+    a = orc.OracleEngine(st, s, mode=orc.MODE_REFERENCE, grid=s.grid_info())
+    b = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    mats = abi.materials_array(s.materials)
+    n = st.p_count
+    for _ in range(4):
+        a.step(50)
+        b.step(50)
+        ke = [orc.lib().orc_kinetic_energy(n, e.st.vel.ctypes.data_as(orc.FP), e.st.rot_vel.ctypes.data_as(orc.FP),
+                                           e.st.radii.ctypes.data_as(orc.FP),
+                                           e.st.material_pointers.ctypes.data_as(orc.IP), mats, len(mats))
+              for e in (a, b)]
+        assert abs(ke[0] - ke[1]) <= 0.01 * abs(ke[0]) + 1e-12
+
+
+def test_free_flight_and_wall_bounce():
+    """One disc, no contacts: v_n = v0 - g n dt exactly as the two half-kicks sum,
+    x advances by v dt; at the floor the velocity flips with factor -0.5
+    (2D_LOM.wgsl:108-112)."""
+    s, st = scenes.granular_box(n_side=1, seed=0, spacing=4.0, rmin=0.5, rmax=0.5)
+    st.pos[:2] = [0.0, 0.0]
+    st.vel[:2] = [0.0, -3.0]
+    s.simulation.timestep = 1e-3
+    e = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    g = np.float32(np.float32(9.81) * np.float32(1.0))
+    vy = np.float32(-3.0)
+    ay = np.float32(0.0)
+    dt = np.float32(1e-3)
+    y = np.float32(0.0)
+    bounced = False
+    for k in range(400):
+        e.step(1)
+        # restatement of the two half-kicks and the drift in f32
+        iv = np.float32(vy + np.float32(np.float32(ay * dt) * np.float32(0.5)))
+        dy = np.float32(iv * dt)
+        ny = np.float32(y + dy)
+        if ny - np.float32(0.5) < np.float32(-2.0):
+            iv = np.float32(-iv * np.float32(0.5))
+            y = np.float32(y + np.float32(np.float32(-2.0) - np.float32(ny - np.float32(0.5))))
+            bounced = True
+        y = np.float32(y + dy)
+        mass = np.float32(np.float32(np.float32(1.0) * np.float32(np.pi)) * np.float32(0.5)) * np.float32(0.5)
+        fy = np.float32(np.float32(0.0) + np.float32(-g * mass))
+        ay = np.float32(fy / mass)
+        vy = np.float32(iv + np.float32(np.float32(np.float32(0.5) * ay) * dt))
+        assert e.st.pos[1] == y and e.st.vel[1] == vy, k
+    assert bounced
+
+
+def test_energy_conserved_frictionless():
+    """μ = 0, no gravity, no walls reached: kinetic + spring energy conserved to O(dt)."""
+    s, st = _granular(n_side=8, spacing=1.75, seed=7)
+    s.physics.gravity = False
+    s.physics.friction_coefficient = 0.0
+    s.simulation.hor_bound = s.simulation.vert_bound = 200.0
+    st.vel[: 2 * st.p_count] = np.random.default_rng(1).normal(0, 2.0, 2 * st.p_count).astype(np.float32)
+    e = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    mats = abi.materials_array(s.materials)
+    n = st.p_count
+
+    def energy():
+        ke = orc.lib().orc_kinetic_energy(n, e.st.vel.ctypes.data_as(orc.FP), e.st.rot_vel.ctypes.data_as(orc.FP),
+                                          e.st.radii.ctypes.data_as(orc.FP),
+                                          e.st.material_pointers.ctypes.data_as(orc.IP), mats, len(mats))
+        p = e.st.pos[: 2 * n].reshape(-1, 2).astype(np.float64)
+        r = e.st.radii[:n].astype(np.float64)
+        d = np.sqrt(((p[:, None, :] - p[None, :, :]) ** 2).sum(-1))
+        ov = np.maximum(r[:, None] + r[None, :] - d, 0.0)
+        np.fill_diagonal(ov, 0.0)
+        kn = 1.0 / (1.0 / 2e5 + 1.0 / 2e5)
+        return ke + 0.5 * kn * (ov ** 2).sum() / 2
+    e0 = energy()
+    e.step(2000)
+    e1 = energy()
+    assert abs(e1 - e0) < 0.02 * e0
+
+
+def test_regen_bonds_bruteforce():
+    """State::regen_bonds restated (oracle) equals an independent numpy pair scan."""
+    rng = np.random.default_rng(3)
+    n = 60
+    pos = rng.uniform(-6, 6, 2 * n).astype(np.float32)
+    r = rng.uniform(0.8, 1.2, n).astype(np.float32)
+    bonds = np.zeros(3 * 400, dtype=np.int32)
+    cont = np.zeros(14 * 6 * n, dtype=np.float32)
+    nb = orc.lib().orc_regen_bonds(n, pos.ctypes.data_as(orc.FP), r.ctypes.data_as(orc.FP), 6, 14,
+                                   bonds.ctypes.data_as(orc.IP), 400, cont.ctypes.data_as(orc.FP))
+    want = []
+    for i in range(n):
+        c = 0
+        for j in range(i + 1, n):
+            dx = np.float32(pos[2 * j] - pos[2 * i])
+            dy = np.float32(pos[2 * j + 1] - pos[2 * i + 1])
+            d = np.sqrt(np.float32(dx * dx + dy * dy))
+            if d < np.float32(np.float32(r[i] + r[j]) * np.float32(1.05)):
+                if c < 6:
+                    want.append((i, j, d))
+                    c += 1
+    assert nb == len(want) > 20
+    tab = cont.view(np.int32).reshape(n, 14, 6)
+    for k, (i, j, d) in enumerate(want):
+        assert bonds[3 * k] == 1 and bonds[3 * k + 2] == np.float32(d).view(np.int32)
+        assert k in tab[i, :, 5] and k in tab[j, :, 5]
+
+
+def test_stage_closed_forms_bruteforce():
+    """Stage 1-3 closed forms against a direct Python evaluation on a tiny grid."""
+    rng = np.random.default_rng(9)
+    n = 200
+    pos = rng.uniform(-9, 9, 2 * n).astype(np.float32)
+    r = rng.uniform(0.2, 0.5, n).astype(np.float32)
+    r[5] = 0.0
+    g = abi.DemGrid(np.float32(1.25), 0, 16, 14)
+    keys = np.zeros(n, dtype=np.int32)
+    orc.lib().orc_cell_keys(n, n, pos.ctypes.data_as(orc.FP), r.ctypes.data_as(orc.FP), C.byref(g),
+                            keys.ctypes.data_as(orc.IP))
+    cs = np.float32(1.25)
+    bx = np.float32(np.float32(-cs * np.float32(16)) * np.float32(0.5))
+    by = np.float32(np.float32(cs * np.float32(14)) * np.float32(0.5))
+    for i in range(n):
+        if r[i] == 0:
+            assert keys[i] == 16 * 14
+            continue
+        cx = min(max(int(np.float32(np.float32(pos[2 * i] - bx) / cs)), 0), 15)
+        cy = min(max(int(np.float32(np.float32(by - pos[2 * i + 1]) / cs)), 0), 13)
+        assert keys[i] == cy * 16 + cx
+    order = np.zeros(n, dtype=np.int32)
+    start = np.zeros(16 * 14 + 1, dtype=np.int32)
+    orc.lib().orc_sort_cells(n, keys.ctypes.data_as(orc.IP), 16 * 14, order.ctypes.data_as(orc.IP),
+                             start.ctypes.data_as(orc.IP))
+    assert np.array_equal(order, np.lexsort((np.arange(n), keys)))
+    assert np.array_equal(start, np.searchsorted(np.sort(keys), np.arange(16 * 14 + 1)))
