@@ -1,0 +1,155 @@
+"""Benchmark of the DEM time step on one MI355X.
+
+Workload (default): the large polydisperse granular collapse - 4,194,304 discs,
+log-normal radii (median 0.35, clip [0.175, 0.525]; the stated radii scaled by
+0.35 so the discs fit the 4096 x 2048 box), hex column of spacing 1.05, gravity,
+contact law kn_eff = 1e5, mu = 0.5, dt = 1e-4.  A "step" is one full time step of
+every disc (LOM + Sim of the reference, here: step kernel, cell scan, scatter).
+
+Prints one JSON line.  `value` is particle-steps/s of the timed region (inputs
+resident in HBM, host clock around K steps bracketed by device syncs).
+`roofline` is for the dominant kernel (the fused step kernel): algorithmic bytes
+per launch = N * (68 + 16 * mean contacts per particle) over its mean duration
+from HIP events on the ctx's stream.  `cpu_baseline` times the CPU oracle's
+restatement of the reference algorithm (AABB bucket grid, 14-slot table) on a
+bounded sample of the same scene.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "physics-sim_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--particles", type=int, default=4194304)
+    ap.add_argument("--per-row", type=int, default=2048)
+    ap.add_argument("--cell", type=float, default=0.0, help="grid cell side (0: just above 2 r_max)")
+    ap.add_argument("--reorder", type=int, default=16)
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-particles", type=int, default=262144)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(args):
+    """Oracle (reference algorithm, f32) on a bounded sample: the first
+    `cpu_particles` discs of the same column generator, timed over enough steps
+    for about cpu_seconds of work on the host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes as C
+    import numpy as np
+    import orc
+    from psim import abi, scenes
+    from psim.compute import ComputeProg
+    n = args.cpu_particles
+    s, st = scenes.column_collapse(n=n, per_row=args.per_row)
+    grid = ComputeProg.default_grid(s, st)
+    # reference grid semantics (AABB buckets with capacity) on the box-sized grid
+    rmin, rmax = float(st.radii[:n].min()), float(st.radii[:n].max())
+    grid.cell_cap = min(int(np.ceil((2 * rmax / rmin + 1.0) ** 2)), n) + 2
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_REFERENCE, grid=grid)
+    cores = min(16, os.cpu_count() or 1)
+    u = s.collision_settings()
+    u.particles = n
+    v = eng._view()
+    L = orc.lib()
+    # calibrate with 2 steps, then time a bounded number of steps
+    t = L.orc_time_reference(C.byref(u), eng.mats, len(eng.mats), C.byref(grid), C.byref(v), 2, cores)
+    steps = max(2, int(args.cpu_seconds / max(t / 2, 1e-6)))
+    t = L.orc_time_reference(C.byref(u), eng.mats, len(eng.mats), C.byref(grid), C.byref(v), steps, cores)
+    return {"value": n * steps / t, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": "CPU restatement of the reference algorithm (oracle, reference slot rules, f32, OpenMP) on "
+                      "the first %d discs of the same column, %d steps in %.1f s" % (n, steps, t)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (multi-GPU sharding is a "
+              "design note only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    from psim import scenes
+    from psim.compute import ComputeProg
+
+    s, st = scenes.column_collapse(n=args.particles, per_row=args.per_row)
+    grid = ComputeProg.default_grid(s, st, cell_size=args.cell if args.cell > 0 else None)
+    prog = ComputeProg(s, st, grid=grid, record_data=False, reorder_interval=args.reorder, block_size=args.block)
+    n = st.p_count
+    print("bench: %d discs, grid %dx%d cell %.4f; warmup %d" % (n, grid.w, grid.h, grid.cell_size, args.warmup),
+          file=sys.stderr, flush=True)
+    if args.warmup:
+        prog.step(args.warmup)
+    prog.sync()
+    c0 = prog.counters()
+    # timed region: K steps, host clock, device synced on both sides
+    prog.timing(False)
+    prog.sync()
+    t0 = time.perf_counter()
+    prog.step(args.steps)
+    prog.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    c1 = prog.counters()
+    # same K steps again with HIP events around every launch (per-kernel means)
+    prog.timing(True)
+    prog.step(args.steps)
+    prog.sync()
+    tm = prog.timing(False)
+    c2 = prog.counters()
+    nc = 0.5 * (c1["contacts"] + c2["contacts"]) / n
+    ms_step = 1e3 * wall / args.steps
+    value = n * args.steps / wall
+    cells = grid.w * grid.h
+    b_alg = 48 + 12 + 16 + 8.0 * cells / n + 16 * nc       # bytes per particle-step, whole step
+    k_bytes = n * (68 + 16 * nc)                            # dominant kernel, per launch
+    k_ms = tm["step_ms"] / max(tm["n_step"], 1)
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": "particle-steps/s at 4M discs (%HBM peak); scenes/s + p99 latency batched vs plain",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "granular column collapse, %d discs (log-normal r median 0.35 clip [0.175,0.525]), "
+                        "4096x2048 box, dt 1e-4" % n,
+            "grid": [grid.w, grid.h], "cell_size": grid.cell_size, "reorder_interval": args.reorder,
+            "block": args.block, "mean_contacts_per_particle": nc,
+            "hbm_frac_whole_step": value * b_alg / (HBM_PEAK_GBS * 1e9),
+            "bytes_per_particle_step_alg": b_alg,
+            "kernel_ms": {"step": k_ms, "scan": tm["scan_ms"] / max(tm["n_scan"], 1),
+                          "scatter": tm["scatter_ms"] / max(tm["n_scatter"], 1),
+                          "reorder": tm["reorder_ms"] / max(tm["n_reorder"], 1)},
+            "counters": {k: c2[k] - c0.get(k, 0) if k in ("steps", "broad_phases", "reorders") else c2[k]
+                         for k in c2},
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
+    prog.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
