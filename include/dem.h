@@ -93,7 +93,8 @@ typedef struct dem_params {
     dem_grid            grid;
     int32_t use_f64;           /* reference "$ F64" variant: distances in f64 (2D_Simulation.wgsl:264-284) */
     int32_t record_data;       /* maintain data[7] per particle (Σfn, Σft, ΣM, 0, v*x, v*y, ω*) */
-    int32_t reorder_interval;  /* re-sort particle storage by cell every k broad phases; 0 = never */
+    int32_t reorder_interval;  /* re-sort particle storage by cell every k broad phases; 0: only at the
+                                  first broad phase after an upload; < 0: never */
     int32_t block_size;        /* threads per block for particle kernels; 0 = 256 */
 } dem_params;
 

@@ -683,18 +683,21 @@ int orc_grid_capacity(float hor_bound, float vert_bound, float max_radius, float
     return 0;
 }
 
-// Stage 1 closed form: centre cell of each particle in the dem_grid convention,
-// the expression shape of 2D_LOM.wgsl:167-178 applied to the centre.  Inactive
-// particles (r == 0 or id >= n_active) get key w*h.
+// Stage 1 closed form: centre cell of each particle in the dem_grid convention
+// (grid centred on the origin, rows from the top, as GridInfo in 2D_LOM.wgsl:167-178):
+// cx = trunc((x - base_x) * (1/cs)) and cy = trunc((base_y - y) * (1/cs)), clamped to
+// the grid, with 1/cs rounded to f32.  Inactive particles (r == 0 or id >= n_active)
+// get key w*h.
 int orc_cell_keys(int32_t n, int32_t n_active, const float* pos, const float* radii, const dem_grid* g,
                   int32_t* keys) {
     float cs = g->cell_size;
     float base_x = -cs * (float)g->w * 0.5f;
     float base_y = cs * (float)g->h * 0.5f;
+    float inv = 1.0f / cs;
     for (int i = 0; i < n; i++) {
         if (i >= n_active || radii[i] == 0.0f) { keys[i] = g->w * g->h; continue; }
-        int cx = std::min(std::max(f2i((pos[2 * i] - base_x) / cs), 0), g->w - 1);
-        int cy = std::min(std::max(f2i((base_y - pos[2 * i + 1]) / cs), 0), g->h - 1);
+        int cx = std::min(std::max(f2i((pos[2 * i] - base_x) * inv), 0), g->w - 1);
+        int cy = std::min(std::max(f2i((base_y - pos[2 * i + 1]) * inv), 0), g->h - 1);
         keys[i] = cy * g->w + cx;
     }
     return 0;

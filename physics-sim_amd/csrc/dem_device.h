@@ -42,6 +42,8 @@ struct SceneDev {
     int32_t c0;            // first cell; cells [c0, c0 + w*h], the last one collects inactive particles
     int32_t w, h;
     float cs, base_x, base_y;
+    float inv_cs;          // 1 / cs (f32), the cell index multiplier
+    float half_w, half_h;  // hor_bound / 2, vert_bound / 2 (exact halvings)
     int32_t b0, nb;        // bonds [b0, b0 + nb)
     int32_t m0, nm;        // materials [m0, m0 + nm)
     int32_t use_f64;
@@ -60,10 +62,14 @@ struct DevArrays {
     float4* EXT;       // external {fx, fy, moment, 0}
     float4* DATA0;     // {Σfn, Σft, ΣM, data3}
     float4* DATA1;     // {v*x, v*y, w*, 0}
-    // contacts: slot-major [KMAX][cap]
+    // contact lists, slot-major [KMAX][cap], double-buffered: every step kernel reads
+    // the current lists (CC/CP/CF) and writes the next ones (CCn/CPn/CFn)
     int32_t* CC;       // count
-    int32_t* CP;       // partner id (scene-local)
+    int32_t* CP;       // partner id (scene-local), ascending
     float* CF;         // tangential force
+    int32_t* CCn;
+    int32_t* CPn;
+    float* CFn;
     int64_t cap;       // particle pool size (slot stride of CP / CF)
     // bond endpoint records: CSR indexed by (scene p0 + particle id), each list sorted
     // by partner id; records never move
@@ -83,9 +89,8 @@ struct DevArrays {
     int32_t* INV;      // [cap] slot of (scene p0 + particle id)
     // counters
     unsigned long long* STAT;  // [0] contact overflow, [1] bond overflow, [2] scan timeouts
-    // scan scratch, cleared by the step kernel that precedes each scan
-    unsigned long long* TILES;
-    unsigned int* TICKET;
+    // per-tile cell counts of the scan (reduce-then-scan, no inter-block waits)
+    int32_t* TSUM;
     int32_t ntiles;
 };
 

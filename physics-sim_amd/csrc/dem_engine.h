@@ -25,6 +25,9 @@ struct DevSet {
     float4* EXT = nullptr;
     float4* DATA0 = nullptr;
     float4* DATA1 = nullptr;
+};
+
+struct ContactLists {
     int32_t* CC = nullptr;
     int32_t* CP = nullptr;
     float* CF = nullptr;
@@ -126,6 +129,8 @@ private:
     int set_ = 0;   // which DevSet is current
     int cur_ = 0;   // which P/D buffer is current
     DevSet sets_[2];
+    ContactLists lists_[2];
+    int lcur_ = 0;  // which contact-list buffer is current
     int32_t* BOFF_ = nullptr;
     int4* BR_ = nullptr;
     float4* BF_ = nullptr;
@@ -135,8 +140,7 @@ private:
     dem_material* MAT_ = nullptr;
     int32_t *KEY_ = nullptr, *RANK_ = nullptr, *CNT_ = nullptr, *START_ = nullptr, *SIDX_ = nullptr, *INV_ = nullptr;
     unsigned long long* STAT_ = nullptr;
-    unsigned long long* TILES_ = nullptr;
-    unsigned int* TICKET_ = nullptr;
+    int32_t* TSUM_ = nullptr;
     int32_t ntiles_ = 0;
     SceneDev* d_scenes_ = nullptr;
     int32_t* d_blk_scene_ = nullptr;

@@ -45,11 +45,11 @@ Engine::~Engine() {
     for (auto& s : sets_) {
         hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.D[0]); hipFree(s.D[1]);
         hipFree(s.V); hipFree(s.A); hipFree(s.M); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
-        hipFree(s.CC); hipFree(s.CP); hipFree(s.CF);
     }
+    for (auto& l : lists_) { hipFree(l.CC); hipFree(l.CP); hipFree(l.CF); }
     hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BL_); hipFree(BSTR_); hipFree(MAT_);
     hipFree(KEY_); hipFree(RANK_); hipFree(CNT_); hipFree(START_); hipFree(SIDX_); hipFree(INV_);
-    hipFree(STAT_); hipFree(TILES_); hipFree(TICKET_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
+    hipFree(STAT_); hipFree(TSUM_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
     hipFree(d_ke_); hipFree(d_cnt_);
     for (auto e : ev_pool_) hipEventDestroy(e);
     if (stream_) hipStreamDestroy(stream_);
@@ -76,7 +76,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     rec_cap_ = std::max<int64_t>(rec_cap, 1);
     mat_cap_ = std::max(mat_cap, 1);
     scene_cap_ = std::max(scene_cap, 1);
-    if (slot_cap_ > (int64_t)INT32_MAX / 2 || cell_cap_ > (int64_t)INT32_MAX / 2) {
+    if (slot_cap_ * KMAX > (int64_t)INT32_MAX || cell_cap_ > (int64_t)INT32_MAX / 2) {
         set_error("pool too large for 32-bit slot indices");
         return DEM_EINVAL;
     }
@@ -91,8 +91,12 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
         e = e ? e : dalloc(&s.V, S); e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S);
         e = e ? e : dalloc(&s.EXT, S);
         if (with_data) { e = e ? e : dalloc(&s.DATA0, S); e = e ? e : dalloc(&s.DATA1, S); }
-        e = e ? e : dalloc(&s.CC, S); e = e ? e : dalloc(&s.CP, S * KMAX); e = e ? e : dalloc(&s.CF, S * KMAX);
         if ((rc = check(e, "hipMalloc particle pools"))) return rc;
+    }
+    for (auto& l : lists_) {
+        hipError_t e = hipSuccess;
+        e = e ? e : dalloc(&l.CC, S); e = e ? e : dalloc(&l.CP, S * KMAX); e = e ? e : dalloc(&l.CF, S * KMAX);
+        if ((rc = check(e, "hipMalloc contact lists"))) return rc;
     }
     hipError_t e = hipSuccess;
     e = e ? e : dalloc(&BOFF_, S + 1); e = e ? e : dalloc(&BR_, rec_cap_); e = e ? e : dalloc(&BF_, rec_cap_);
@@ -102,7 +106,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = e ? e : dalloc(&START_, cell_cap_ + 2); e = e ? e : dalloc(&SIDX_, S); e = e ? e : dalloc(&INV_, S);
     e = e ? e : dalloc(&STAT_, 8);
     ntiles_ = scan_tiles((int)cell_cap_);
-    e = e ? e : dalloc(&TILES_, ntiles_ + 1); e = e ? e : dalloc(&TICKET_, 1);
+    e = e ? e : dalloc(&TSUM_, ntiles_ + 1);
     e = e ? e : dalloc(&d_scenes_, scene_cap_);
     const int64_t nblk = launch_grid_blocks();
     e = e ? e : dalloc(&d_blk_scene_, nblk);
@@ -115,10 +119,8 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = e ? e : hipMemsetAsync(START_, 0, sizeof(int32_t) * (cell_cap_ + 2), stream_);
     e = e ? e : hipMemsetAsync(STAT_, 0, sizeof(unsigned long long) * 8, stream_);
     e = e ? e : hipMemsetAsync(BOFF_, 0, sizeof(int32_t) * (S + 1), stream_);
-    for (auto& s : sets_) {
-        e = e ? e : hipMemsetAsync(s.CC, 0, sizeof(int32_t) * S, stream_);
-        e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int2) * S, stream_);
-    }
+    for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int2) * S, stream_);
+    for (auto& l : lists_) e = e ? e : hipMemsetAsync(l.CC, 0, sizeof(int32_t) * S, stream_);
     std::vector<int32_t> blk(nblk, -1);
     e = e ? e : hipMemcpyAsync(d_blk_scene_, blk.data(), sizeof(int32_t) * nblk, hipMemcpyHostToDevice, stream_);
     e = e ? e : hipStreamSynchronize(stream_);
@@ -126,8 +128,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
 }
 
 int Engine::launch_grid_blocks() const {
-    const int64_t need = std::max<int64_t>(slot_cap_, (int64_t)ntiles_ + 1);
-    return (int)((need + block_ - 1) / block_);
+    return (int)((slot_cap_ + block_ - 1) / block_);
 }
 
 int64_t Engine::free_slots() const {
@@ -286,6 +287,9 @@ SceneDev Engine::scene_dev(const SceneHost& sh) const {
     d.cs = sh.params.grid.cell_size;
     d.base_x = -d.cs * (float)d.w * 0.5f;
     d.base_y = d.cs * (float)d.h * 0.5f;
+    d.inv_cs = 1.0f / d.cs;
+    d.half_w = d.u.hor_bound / 2.0f;
+    d.half_h = d.u.vert_bound / 2.0f;
     d.b0 = (int32_t)sh.b0;
     d.nb = sh.nb;
     d.m0 = sh.m0;
@@ -320,10 +324,14 @@ DevArrays Engine::arrays(int set) const {
     DevArrays a{};
     a.P[0] = s.P[0]; a.P[1] = s.P[1]; a.D[0] = s.D[0]; a.D[1] = s.D[1];
     a.V = s.V; a.A = s.A; a.M = s.M; a.EXT = s.EXT; a.DATA0 = s.DATA0; a.DATA1 = s.DATA1;
-    a.CC = s.CC; a.CP = s.CP; a.CF = s.CF; a.cap = slot_cap_;
+    const ContactLists& lc = lists_[lcur_];
+    const ContactLists& ln = lists_[lcur_ ^ 1];
+    a.CC = lc.CC; a.CP = lc.CP; a.CF = lc.CF;
+    a.CCn = ln.CC; a.CPn = ln.CP; a.CFn = ln.CF;
+    a.cap = slot_cap_;
     a.BOFF = BOFF_; a.BR = BR_; a.BF = BF_; a.BS = BS_; a.BL = BL_; a.BSTR = BSTR_; a.MAT = MAT_;
     a.KEY = KEY_; a.RANK = RANK_; a.CNT = CNT_; a.START = START_; a.SIDX = SIDX_; a.INV = INV_;
-    a.STAT = STAT_; a.TILES = TILES_; a.TICKET = TICKET_; a.ntiles = ntiles_;
+    a.STAT = STAT_; a.TSUM = TSUM_; a.ntiles = ntiles_;
     return a;
 }
 
@@ -501,10 +509,11 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         h2d(S.DATA0 + p0, D0.data(), sizeof(float4) * np);
         h2d(S.DATA1 + p0, D1.data(), sizeof(float4) * np);
     }
-    h2d(S.CC + p0, CC.data(), sizeof(int32_t) * np);
+    const ContactLists& L = lists_[lcur_];
+    h2d(L.CC + p0, CC.data(), sizeof(int32_t) * np);
     for (int q = 0; q < KMAX; q++) {
-        h2d(S.CP + (int64_t)q * slot_cap_ + p0, CP.data() + (size_t)q * np, sizeof(int32_t) * np);
-        h2d(S.CF + (int64_t)q * slot_cap_ + p0, CF.data() + (size_t)q * np, sizeof(float) * np);
+        h2d(L.CP + (int64_t)q * slot_cap_ + p0, CP.data() + (size_t)q * np, sizeof(int32_t) * np);
+        h2d(L.CF + (int64_t)q * slot_cap_ + p0, CF.data() + (size_t)q * np, sizeof(float) * np);
     }
     h2d(INV_ + p0, INV.data(), sizeof(int32_t) * np);
     h2d(SIDX_ + p0, SIDX.data(), sizeof(int32_t) * np);
@@ -543,6 +552,9 @@ int Engine::reorder(const int2* lf_dev, const std::vector<int32_t>& sids) {
     r.lf = lf_dev;
     r.src = arrays(set_);
     r.dst = arrays(set_ ^ 1);
+    r.dst.CC = r.src.CCn;
+    r.dst.CP = r.src.CPn;
+    r.dst.CF = r.src.CFn;
     r.cur = cur_;
     r.with_ext = 1;
     r.with_data = with_data_ ? 1 : 0;
@@ -551,6 +563,7 @@ int Engine::reorder(const int2* lf_dev, const std::vector<int32_t>& sids) {
     int rc = check(e, "reorder");
     if (rc) return rc;
     set_ ^= 1;
+    lcur_ ^= 1;
     for (int32_t sid : sids) {
         scenes_[sid].since_reorder = 0;
         scenes_[sid].reorders++;
@@ -649,6 +662,7 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
         StepArgs args = make_args(lfk);
         e = timed(0, [&]() { return launch_step(args, nblk, block_, any_f64, any_bonds, stream_); });
         if ((rc = check(e, "step kernel"))) return rc;
+        lcur_ ^= 1;  // the step kernel wrote every live slot's next contact list
         // the LOM half wrote the next positions
         bool binned = false, lom = false;
         std::vector<int32_t> active;
@@ -669,7 +683,7 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
                 SceneHost& sh = scenes_[sid];
                 sh.broad_phases++;
                 const int32_t R = sh.params.reorder_interval;
-                if (sh.since_reorder >= (R > 0 ? R : (1 << 30))) due = true;
+                if (R > 0 ? sh.since_reorder >= R : (R == 0 && sh.since_reorder >= (1 << 30))) due = true;
                 sh.since_reorder++;
             }
             if (due) {
@@ -733,10 +747,11 @@ int Engine::download(int32_t sid, dem_state_view& v) {
         d2h(D1.data(), S.DATA1 + p0, sizeof(float4) * np);
     }
     if (v.contacts) {
-        d2h(CC.data(), S.CC + p0, sizeof(int32_t) * np);
+        const ContactLists& L = lists_[lcur_];
+        d2h(CC.data(), L.CC + p0, sizeof(int32_t) * np);
         for (int q = 0; q < KMAX; q++) {
-            d2h(CP.data() + (size_t)q * np, S.CP + (int64_t)q * slot_cap_ + p0, sizeof(int32_t) * np);
-            d2h(CF.data() + (size_t)q * np, S.CF + (int64_t)q * slot_cap_ + p0, sizeof(float) * np);
+            d2h(CP.data() + (size_t)q * np, L.CP + (int64_t)q * slot_cap_ + p0, sizeof(int32_t) * np);
+            d2h(CF.data() + (size_t)q * np, L.CF + (int64_t)q * slot_cap_ + p0, sizeof(float) * np);
         }
         d2h(BOFF.data(), BOFF_ + p0, sizeof(int32_t) * (np + 1));
         if (sh.nrec) {
@@ -857,7 +872,7 @@ int Engine::counters(int32_t sid, int64_t* out, int32_t n_out) {
     unsigned long long st[8] = {0}, nc = 0;
     hipError_t e = hipMemcpyAsync(st, STAT_, sizeof(st), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipMemsetAsync(d_cnt_, 0, sizeof(unsigned long long), stream_);
-    if (e == hipSuccess) e = launch_count_contacts(sets_[set_].CC, (int)sh.p0, sh.np, d_cnt_, stream_);
+    if (e == hipSuccess) e = launch_count_contacts(lists_[lcur_].CC, (int)sh.p0, sh.np, d_cnt_, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(&nc, d_cnt_, sizeof(nc), hipMemcpyDeviceToHost, stream_);
     int64_t nrec_alive = 0;
     std::vector<int4> BR(std::max(sh.nrec, 1));
@@ -940,9 +955,9 @@ int Engine::debug_contacts(int32_t sid, int32_t max_per, int32_t* counts, int32_
     const SceneHost& sh = scenes_[sid];
     const int np = sh.np;
     std::vector<int32_t> CC(np), CP((size_t)np * KMAX), INV(np);
-    hipError_t e = hipMemcpyAsync(CC.data(), sets_[set_].CC + sh.p0, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
+    hipError_t e = hipMemcpyAsync(CC.data(), lists_[lcur_].CC + sh.p0, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
     for (int q = 0; q < KMAX && e == hipSuccess; q++)
-        e = hipMemcpyAsync(CP.data() + (size_t)q * np, sets_[set_].CP + (int64_t)q * slot_cap_ + sh.p0,
+        e = hipMemcpyAsync(CP.data() + (size_t)q * np, lists_[lcur_].CP + (int64_t)q * slot_cap_ + sh.p0,
                            sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(INV.data(), INV_ + sh.p0, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);

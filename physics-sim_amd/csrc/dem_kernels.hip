@@ -52,10 +52,12 @@ __device__ __forceinline__ int f2i(float v) {
 }
 
 // Centre cell of a position in the GridInfo convention (2D_LOM.wgsl:167-178 applied
-// to the centre): column from the left edge, row from the top edge, clamped.
+// to the centre): column from the left edge, row from the top edge, clamped to the
+// grid (NaN -> 0).  Multiplies by the f32 reciprocal of the cell side; clamping in
+// the float domain before truncation equals truncate-then-clamp.
 __device__ __forceinline__ int cell_of(const SceneDev& S, float x, float y, int& cx, int& cy) {
-    cx = min(max(f2i((x - S.base_x) / S.cs), 0), S.w - 1);
-    cy = min(max(f2i((S.base_y - y) / S.cs), 0), S.h - 1);
+    cx = (int)fminf(fmaxf((x - S.base_x) * S.inv_cs, 0.0f), (float)(S.w - 1));
+    cy = (int)fminf(fmaxf((S.base_y - y) * S.inv_cs, 0.0f), (float)(S.h - 1));
     return cy * S.w + cx;
 }
 
@@ -212,34 +214,166 @@ __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, 
 
 // ---- the fused step kernel ----
 
+// Block -> tile of slots, XCD-aware: the hardware hands consecutive blocks to the 8
+// XCDs in turn, each with its own L2; remapping so that XCD k processes one contiguous
+// eighth of the slots keeps a block's neighbour rows in the L2 that already holds them.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_block(int bid, int nb) {
+    const int q = nb / N_XCD, r = nb % N_XCD;
+    const int x = bid % N_XCD, i = bid / N_XCD;
+    return x * q + min(x, r) + i;
+}
+
+
+__device__ __forceinline__ void copy_list(const DevArrays& a, int s) {
+    const int c = a.CC[s];
+    a.CCn[s] = c;
+    const int cap = (int)a.cap;
+    for (int q = 0; q < c && q < KMAX; q++) {
+        a.CPn[q * cap + s] = a.CP[q * cap + s];
+        a.CFn[q * cap + s] = a.CF[q * cap + s];
+    }
+}
+
+__device__ __forceinline__ int wave_min_i(int x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = min(x, __shfl_xor(x, off, 64));
+    return x;
+}
+__device__ __forceinline__ int wave_max_i(int x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = max(x, __shfl_xor(x, off, 64));
+    return x;
+}
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// narrow-phase predicate of 2D_Simulation.wgsl:159, |p_b - p_a| < r_b + r_a, with an
+// exact early reject: d^2 >= (r_b + r_a)^2 * 1.0001 implies fl(sqrt(d^2)) > r_b + r_a
+__device__ __forceinline__ bool overlaps(const float4& pj, const float4& p) {
+    const float dx = pj.x - p.x, dy = pj.y - p.y;
+    const float xx = dx * dx;
+    const float yy = dy * dy;
+    const float d2 = xx + yy;
+    const float R = pj.z + p.z;
+    if (d2 >= R * R * 1.0001f) return false;
+    return sqrtf(d2) < R;
+}
+
+// Adds partner id `oj` to the ascending list kept in the next-list buffer
+// (column s, stride cap); the largest drops when KMAX are held.
+__device__ __forceinline__ void list_insert(const DevArrays& a, int s, int oj, int& n) {
+    int32_t* col = a.CPn + s;
+    const int cap = (int)a.cap;
+    int pos = n;
+    while (pos > 0 && col[(pos - 1) * cap] > oj) {
+        if (pos < KMAX) col[pos * cap] = col[(pos - 1) * cap];
+        pos--;
+    }
+    if (pos < KMAX) {
+        col[pos * cap] = oj;
+        if (n < KMAX) n++;
+        else atomicAdd(&a.STAT[0], 1ull);
+    } else {
+        atomicAdd(&a.STAT[0], 1ull);
+    }
+}
+
+template <bool BONDS>
+__device__ __forceinline__ bool bonded_partner(const DevArrays& a, int b_lo, int b_hi, int oj) {
+    if (!BONDS) return false;
+    bool hit = false;
+    for (int q = b_lo; q < b_hi; q++) {
+        const int4 br = a.BR[q];
+        if (br.w != 0 && br.x == oj) hit = true;
+    }
+    return hit;
+}
+
 template <bool F64, bool BONDS>
 __global__ void __launch_bounds__(256) k_step(StepArgs args) {
     const DevArrays& a = args.a;
-    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    // clear the scan scratch for the scan that follows this launch
-    if (s <= a.ntiles) {
-        if (s < a.ntiles) a.TILES[s] = 0ull;
-        else *a.TICKET = 0u;
-    }
-    const int sid = args.blk_scene[blockIdx.x];
+    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int s = (int)(blk * blockDim.x + threadIdx.x);
+    const int sid = args.blk_scene[blk];
     if (sid < 0) return;
     const SceneDev& S = args.scenes[sid];
+    if (s < S.p0 || s >= S.p0 + S.np) return;   // whole blocks (np is a multiple of the block)
     const int2 lfw = args.lf[sid];
     const uint32_t fl = (uint32_t)lfw.x;
-    if (!(fl & SF_ACTIVE)) return;
-    if (s < S.p0 || s >= S.p0 + S.np) return;
+    if (!(fl & SF_ACTIVE)) {
+        copy_list(a, s);  // the lists flip after every launch
+        return;
+    }
     const int cur = args.cur, nxt = cur ^ 1;
     const dem_uniform& u = S.u;
     const float dt = u.dT;
-
     const int2 mw = a.M[s];
     const uint32_t pf = (uint32_t)mw.x;
+    const bool act = !(pf & PF_INACTIVE);
     float4 p = a.P[cur][s];
     float2 d = a.D[cur][s];
-    float4 v = a.V[s];
+    const int cap = (int)a.cap;  // KMAX * cap < 2^31 (checked at pool creation)
+    const int oid = mw.y;
+    int b_lo = 0, b_hi = 0;
+    if (BONDS && act) {
+        b_lo = a.BOFF[S.p0 + oid];
+        b_hi = a.BOFF[S.p0 + oid + 1];
+    }
 
-    if (pf & PF_INACTIVE) {
+    // ---- narrow phase: every overlapping partner in the 3x3 stencil ----
+    int n = 0;
+    const bool narrow = (fl & SF_SIM) && (fl & SF_NARROW);
+    if (narrow && act) {
+        int cx, cy;
+        cell_of(S, p.x, p.y, cx, cy);
+        const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
+        const int pos0 = S.p0 - a.START[S.c0];  // cell-list position -> slot-list index
+        const int rlo = max(cy - 1, 0), rhi = min(cy + 1, S.h - 1);
+        int e0[3], e1[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const int ry = rlo + r;
+            if (ry <= rhi) {
+                e0[r] = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
+                e1[r] = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
+            } else {
+                e0[r] = 0;
+                e1[r] = 0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            int e = e0[r];
+            for (; e + 1 < e1[r]; e += 2) {
+                const int j0 = a.SIDX[e], j1 = a.SIDX[e + 1];
+                const float4 q0 = a.P[cur][j0];
+                const float4 q1 = a.P[cur][j1];
+                if (j0 != s && overlaps(q0, p)) {
+                    const int oj = a.M[j0].y;
+                    if (!bonded_partner<BONDS>(a, b_lo, b_hi, oj)) list_insert(a, s, oj, n);
+                }
+                if (j1 != s && overlaps(q1, p)) {
+                    const int oj = a.M[j1].y;
+                    if (!bonded_partner<BONDS>(a, b_lo, b_hi, oj)) list_insert(a, s, oj, n);
+                }
+            }
+            if (e < e1[r]) {
+                const int j0 = a.SIDX[e];
+                const float4 q0 = a.P[cur][j0];
+                if (j0 != s && overlaps(q0, p)) {
+                    const int oj = a.M[j0].y;
+                    if (!bonded_partner<BONDS>(a, b_lo, b_hi, oj)) list_insert(a, s, oj, n);
+                }
+            }
+        }
+    }
+
+    if (!act) {
         // never binned into a real cell, never a partner; state kept in both buffers
+        copy_list(a, s);
         if ((fl & SF_SIM) && S.record) {
             const float4 d0 = a.DATA0[s];
             a.DATA0[s] = make_float4(0.0f, 0.0f, 0.0f, d0.w);
@@ -256,6 +390,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs args) {
         return;
     }
 
+    float4 v = a.V[s];
     const int mat = (int)(pf & PF_MAT_MASK);
     const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
     float ax, ay, al;
@@ -266,96 +401,16 @@ __global__ void __launch_bounds__(256) k_step(StepArgs args) {
         me.kn = MA.normal_stiffness; me.ks = MA.shear_stiffness;
         Data3 dd{0.0f, 0.0f, 0.0f};
         float nfx = 0.0f, nfy = 0.0f, nm = 0.0f;
-        const bool narrow = (fl & SF_NARROW) != 0;
-        const int64_t cap = a.cap;
         const int step = lfw.y;
-        const int oid = mw.y;
-
-        int b_lo = 0, b_hi = 0;
-        if (BONDS) {
-            b_lo = a.BOFF[S.p0 + oid];
-            b_hi = a.BOFF[S.p0 + oid + 1];
+        const int oc = min(a.CC[s], KMAX);
+        if (!narrow) {
+            // no narrow phase this step: the existing contacts carry over
+            n = oc;
+            for (int q = 0; q < n; q++) a.CPn[q * cap + s] = a.CP[q * cap + s];
         }
 
-        // 1. contact set: partners (scene-local ids) ascending, with their slots
-        int lo[KMAX];
-        int ls[KMAX];
-        float lf[KMAX];
-        int n = 0;
-        const int oc = a.CC[s];
-        if (narrow) {
-            int cx, cy;
-            cell_of(S, p.x, p.y, cx, cy);
-            const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
-            const int pos0 = S.p0 - a.START[S.c0];  // cell-list position -> slot-list index
-            for (int ry = max(cy - 1, 0); ry <= min(cy + 1, S.h - 1); ry++) {
-                const int e0 = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
-                const int e1 = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
-                for (int e = e0; e < e1; e++) {
-                    const int j = a.SIDX[e];
-                    if (j == s || j < S.p0 || j >= S.p0 + S.np) continue;
-                    const float4 pj = a.P[cur][j];
-                    // narrow-phase predicate, 2D_Simulation.wgsl:159
-                    if (!(lenf(pj.x - p.x, pj.y - p.y) < (pj.z + p.z))) continue;
-                    const int oj = a.M[j].y;
-                    if (BONDS) {
-                        bool bonded_partner = false;
-                        for (int q = b_lo; q < b_hi; q++) {
-                            const int4 br = a.BR[q];
-                            if (br.w != 0 && br.x == oj) bonded_partner = true;
-                        }
-                        if (bonded_partner) continue;
-                    }
-                    // insert (oj, j) keeping lo[] ascending; the largest drops on overflow
-                    int pos = 0;
-#pragma unroll
-                    for (int q = 0; q < KMAX; q++) pos += (q < n && lo[q] < oj) ? 1 : 0;
-                    if (pos >= KMAX) {
-                        atomicAdd(&a.STAT[0], 1ull);
-                        continue;
-                    }
-                    if (n == KMAX) atomicAdd(&a.STAT[0], 1ull);
-#pragma unroll
-                    for (int q = KMAX - 1; q > 0; q--) {
-                        if (q > pos && q <= n) {
-                            lo[q] = lo[q - 1];
-                            ls[q] = ls[q - 1];
-                        }
-                    }
-#pragma unroll
-                    for (int q = 0; q < KMAX; q++) {
-                        if (q == pos) {
-                            lo[q] = oj;
-                            ls[q] = j;
-                        }
-                    }
-                    n = min(n + 1, KMAX);
-                }
-            }
-            // tangential history from the previous list (both ascending)
-            int pp = 0;
-#pragma unroll
-            for (int q = 0; q < KMAX; q++) {
-                if (q < n) {
-                    float f = 0.0f;
-                    while (pp < oc && a.CP[(int64_t)pp * cap + s] < lo[q]) pp++;
-                    if (pp < oc && a.CP[(int64_t)pp * cap + s] == lo[q]) f = a.CF[(int64_t)pp * cap + s];
-                    lf[q] = f;
-                }
-            }
-        } else {
-            n = min(oc, KMAX);
-#pragma unroll
-            for (int q = 0; q < KMAX; q++) {
-                if (q < n) {
-                    lo[q] = a.CP[(int64_t)q * cap + s];
-                    lf[q] = a.CF[(int64_t)q * cap + s];
-                    ls[q] = a.INV[S.p0 + lo[q]];
-                }
-            }
-        }
-
-        // 2. forces over bond records and contacts merged in ascending partner id
+        // 2. forces over bond records and contacts merged in ascending partner id;
+        //    the new list is written as it is evaluated
         int bq = b_lo;
         auto eval_record = [&](int q) {
             int4 br = a.BR[q];
@@ -416,38 +471,37 @@ __global__ void __launch_bounds__(256) k_step(StepArgs args) {
             a.BF[q] = bf;
         };
 
-#pragma unroll
-        for (int q = 0; q < KMAX; q++) {
-            if (q < n) {
-                if (BONDS) {
-                    while (bq < b_hi && a.BR[bq].x < lo[q]) eval_record(bq++);
-                }
-                const int j = ls[q];
-                const float4 pb = a.P[cur][j];
-                const float2 db = a.D[cur][j];
-                const int mb = a.M[j].x & PF_MAT_MASK;
-                const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
-                const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness,
-                                                   MB.shear_stiffness, me.r + pb.z, lf[q], dd);
-                nfx += f.x;
-                nfy += f.y;
-                nm += f.z;
+        int pp = 0;  // read position in the previous list (ascending ids)
+        for (int q = 0; q < n; q++) {
+            const int ob = a.CPn[q * cap + s];
+            if (BONDS) {
+                while (bq < b_hi && a.BR[bq].x < ob) eval_record(bq++);
             }
+            float ft = 0.0f;
+            if (narrow) {
+                while (pp < oc && a.CP[pp * cap + s] < ob) pp++;
+                if (pp < oc && a.CP[pp * cap + s] == ob) ft = a.CF[pp * cap + s];
+            } else {
+                ft = a.CF[q * cap + s];
+            }
+            const int j = a.INV[S.p0 + ob];
+            const float4 pb = a.P[cur][j];
+            const float2 db = a.D[cur][j];
+            const int mb = a.M[j].x & PF_MAT_MASK;
+            const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
+            const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness,
+                                               MB.shear_stiffness, me.r + pb.z, ft, dd);
+            nfx += f.x;
+            nfy += f.y;
+            nm += f.z;
+            a.CFn[q * cap + s] = ft;
         }
         if (BONDS) {
             while (bq < b_hi) eval_record(bq++);
         }
-        // 3. contact list written back in place (the old list was fully read)
-#pragma unroll
-        for (int q = 0; q < KMAX; q++) {
-            if (q < n) {
-                if (narrow) a.CP[(int64_t)q * cap + s] = lo[q];
-                a.CF[(int64_t)q * cap + s] = lf[q];
-            }
-        }
-        if (narrow && n != oc) a.CC[s] = n;
+        a.CCn[s] = n;
 
-        // 4. store_forces, 2D_Simulation.wgsl:434-478
+        // 3. store_forces, 2D_Simulation.wgsl:434-478
         const float density = MA.density;
         const float rr = p.z;
         const float mass = density * PI_F * rr * rr;
@@ -496,6 +550,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs args) {
             return;
         }
     } else {
+        copy_list(a, s);
         const float4 av = a.A[s];
         ax = av.x;
         ay = av.y;
@@ -518,8 +573,8 @@ __global__ void __launch_bounds__(256) k_step(StepArgs args) {
     const float rr = p.z;
     const float npx = px + dpx, npy = py + dpy;
     if (u.round_bounds == 0) {
-        const float yH = u.vert_bound / 2.0f;
-        const float xW = u.hor_bound / 2.0f;
+        const float yH = S.half_h;
+        const float xW = S.half_w;
         if (pf & PF_WALL_Y) {
             if (npy - rr < -yH) {
                 ivy = -ivy * 0.5f;
@@ -541,8 +596,8 @@ __global__ void __launch_bounds__(256) k_step(StepArgs args) {
     } else if (lenf(npx, npy) + rr > u.bound_radius) {
         const float l = lenf(npx, npy);
         const float nx = npx / l, ny = npy / l;
-        const float dd = dpx * nx + dpy * ny;
-        const float dcx = dd * nx, dcy = dd * ny;
+        const float ddd = dpx * nx + dpy * ny;
+        const float dcx = ddd * nx, dcy = ddd * ny;
         const float vv = ivx * nx + ivy * ny;
         const float cvx = vv * nx, cvy = vv * ny;
         if (pf & PF_WALL_X) {
@@ -572,10 +627,6 @@ __global__ void __launch_bounds__(256) k_step(StepArgs args) {
 __global__ void k_bin(StepArgs args) {
     const DevArrays& a = args.a;
     const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (s <= a.ntiles) {
-        if (s < a.ntiles) a.TILES[s] = 0ull;
-        else *a.TICKET = 0u;
-    }
     const int sid = args.blk_scene[blockIdx.x];
     if (sid < 0) return;
     const SceneDev& S = args.scenes[sid];
@@ -594,15 +645,14 @@ __global__ void k_bin(StepArgs args) {
     a.RANK[s] = atomicAdd(&a.CNT[key], 1);
 }
 
-// ---- k_scan: exclusive prefix of cnt[0..n) into start[0..n], zeroing cnt ----
-// Single pass with decoupled look-back.  Each block takes a ticket so every block it
-// waits on has already started; every wait is bounded (SPIN_LIMIT) and counted.
+// ---- cell scan: exclusive prefix of cnt[0..n) into start[0..n], zeroing cnt ----
+// Reduce-then-scan: k_tile_sums writes one count per tile of SCAN_TILE cells;
+// k_scan_tiles adds the sums of the tiles before its own (read from L2) and scans
+// its tile.  No block waits on another.
 
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
-constexpr unsigned long long FLAG_AGG = 1ull << 62, FLAG_PRE = 2ull << 62, VAL_MASK = (1ull << 62) - 1;
-constexpr int SPIN_LIMIT = 1 << 22;
 
 __device__ __forceinline__ int wave_incl_scan(int x) {
     const int lane = threadIdx.x & 63;
@@ -614,79 +664,98 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
     return x;
 }
 
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan(int32_t* cnt, int32_t* start, int n,
-                                                       unsigned long long* tiles, unsigned int* ticket,
-                                                       unsigned long long* stat) {
-    __shared__ int s_tile;
+__device__ __forceinline__ int wave_sum(int x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+__device__ __forceinline__ void load_tile(const int32_t* cnt, int64_t base, int n, int (&v)[SCAN_ITEMS]) {
+    if (base + SCAN_ITEMS <= n) {
+        const int4* src = reinterpret_cast<const int4*>(cnt + base);
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS / 4; k++) {
+            const int4 q = src[k];
+            v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) v[k] = base + k < n ? cnt[base + k] : 0;
+    }
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_tile_sums(const int32_t* cnt, int n, int32_t* tsum) {
     __shared__ int s_wave[SCAN_THREADS / 64];
-    __shared__ int s_prefix;
-    if (threadIdx.x == 0) s_tile = (int)atomicAdd(ticket, 1u);
-    __syncthreads();
-    const int tile = s_tile;
-    const int64_t base = (int64_t)tile * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS];
+    load_tile(cnt, base, n, v);
     int sum = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        const int64_t i = base + k;
-        v[k] = i < n ? cnt[i] : 0;
-        sum += v[k];
+    for (int k = 0; k < SCAN_ITEMS; k++) sum += v[k];
+    sum = wave_sum(sum);
+    if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+#pragma unroll
+        for (int q = 0; q < SCAN_THREADS / 64; q++) tot += s_wave[q];
+        tsum[blockIdx.x] = tot;
     }
-    const int incl = wave_incl_scan(sum);
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(int32_t* cnt, int32_t* start, int n,
+                                                             const int32_t* tsum) {
+    __shared__ int s_wave[SCAN_THREADS / 64];
+    __shared__ int s_pre[SCAN_THREADS / 64];
+    const int tile = blockIdx.x;
+    // prefix of the tiles before this one
+    int pre = 0;
+    for (int q = threadIdx.x; q < tile; q += SCAN_THREADS) pre += tsum[q];
+    pre = wave_sum(pre);
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) s_pre[wv] = pre;
+    const int64_t base = (int64_t)tile * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    load_tile(cnt, base, n, v);
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) sum += v[k];
+    const int incl = wave_incl_scan(sum);
     if (lane == 63) s_wave[wv] = incl;
     __syncthreads();
-    int wave_off = 0, block_total = 0;
+    int off = 0, tile_pre = 0, block_total = 0;
 #pragma unroll
     for (int q = 0; q < SCAN_THREADS / 64; q++) {
-        if (q < wv) wave_off += s_wave[q];
+        if (q < wv) off += s_wave[q];
+        tile_pre += s_pre[q];
         block_total += s_wave[q];
     }
-    const int thread_off = wave_off + incl - sum;
-    if (threadIdx.x == 0) {
-        int prefix = 0;
-        if (tile == 0) {
-            __hip_atomic_store(&tiles[0], FLAG_PRE | (unsigned long long)(unsigned)block_total, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&tiles[tile], FLAG_AGG | (unsigned long long)(unsigned)block_total, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            int t = tile - 1;
-            int spins = 0;
-            while (t >= 0) {
-                const unsigned long long w =
-                    __hip_atomic_load(&tiles[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned long long f = w & ~VAL_MASK;
-                if (f == FLAG_PRE) {
-                    prefix += (int)(w & VAL_MASK);
-                    break;
-                } else if (f == FLAG_AGG) {
-                    prefix += (int)(w & VAL_MASK);
-                    t--;
-                } else if (++spins > SPIN_LIMIT) {
-                    atomicAdd(&stat[2], 1ull);
-                    break;
-                } else {
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            __hip_atomic_store(&tiles[tile], FLAG_PRE | (unsigned long long)(unsigned)(prefix + block_total),
-                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_prefix = prefix;
-    }
-    __syncthreads();
-    int run = s_prefix + thread_off;
+    int run = tile_pre + off + incl - sum;
+    if (base + SCAN_ITEMS <= n) {
+        int4* dst = reinterpret_cast<int4*>(start + base);
+        int4* zc = reinterpret_cast<int4*>(cnt + base);
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        const int64_t i = base + k;
-        if (i < n) {
-            start[i] = run;
-            cnt[i] = 0;
+        for (int k = 0; k < SCAN_ITEMS / 4; k++) {
+            int4 q;
+            q.x = run; run += v[4 * k];
+            q.y = run; run += v[4 * k + 1];
+            q.z = run; run += v[4 * k + 2];
+            q.w = run; run += v[4 * k + 3];
+            dst[k] = q;
+            zc[k] = make_int4(0, 0, 0, 0);
         }
-        run += v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) {
+            const int64_t i = base + k;
+            if (i < n) {
+                start[i] = run;
+                cnt[i] = 0;
+            }
+            run += v[k];
+        }
     }
-    if (tile == (int)gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) start[n] = s_prefix + block_total;
+    if (tile == (int)gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) start[n] = tile_pre + block_total;
 }
 
 // ---- k_scatter: cell-ordered slot list (positions relative to the scene) ----
@@ -786,13 +855,19 @@ __global__ void k_kinetic(DevArrays a, int cur, int m0, int nm, int p0, int np, 
     if (m.y >= 0 && m.y < np) out_by_id[m.y] = e;
 }
 
-// Sum of the contact counts of a scene (n̄_c) - integer, order independent.
-__global__ void k_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out) {
+// Sum of the contact counts of a scene (mean contacts per particle) - integer,
+// order independent; one atomic per block.
+__global__ void __launch_bounds__(256) k_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out) {
+    __shared__ int s_w[4];
     const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     int c = s < np ? cc[p0 + s] : 0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, (unsigned long long)c);
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        if (tot) atomicAdd(out, (unsigned long long)tot);
+    }
 }
 
 // ---- bond builder (State::regen_bonds rule on the cell list) ----
@@ -863,8 +938,9 @@ hipError_t launch_bin(const StepArgs& args, int nblocks, int block, hipStream_t 
 int scan_tiles(int n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
 hipError_t launch_scan(const DevArrays& a, int ncells, hipStream_t st) {
-    hipLaunchKernelGGL(k_scan, dim3(scan_tiles(ncells)), dim3(SCAN_THREADS), 0, st, a.CNT, a.START, ncells, a.TILES,
-                       a.TICKET, a.STAT);
+    const int nt = scan_tiles(ncells);
+    hipLaunchKernelGGL(k_tile_sums, dim3(nt), dim3(SCAN_THREADS), 0, st, a.CNT, ncells, a.TSUM);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, st, a.CNT, a.START, ncells, a.TSUM);
     return hipGetLastError();
 }
 

@@ -224,14 +224,15 @@ def test_stage_closed_forms_bruteforce():
     orc.lib().orc_cell_keys(n, n, pos.ctypes.data_as(orc.FP), r.ctypes.data_as(orc.FP), C.byref(g),
                             keys.ctypes.data_as(orc.IP))
     cs = np.float32(1.25)
+    inv = np.float32(np.float32(1.0) / cs)
     bx = np.float32(np.float32(-cs * np.float32(16)) * np.float32(0.5))
     by = np.float32(np.float32(cs * np.float32(14)) * np.float32(0.5))
     for i in range(n):
         if r[i] == 0:
             assert keys[i] == 16 * 14
             continue
-        cx = min(max(int(np.float32(np.float32(pos[2 * i] - bx) / cs)), 0), 15)
-        cy = min(max(int(np.float32(np.float32(by - pos[2 * i + 1]) / cs)), 0), 13)
+        cx = min(max(int(np.float32(np.float32(pos[2 * i] - bx) * inv)), 0), 15)
+        cy = min(max(int(np.float32(np.float32(by - pos[2 * i + 1]) * inv)), 0), 13)
         assert keys[i] == cy * 16 + cx
     order = np.zeros(n, dtype=np.int32)
     start = np.zeros(16 * 14 + 1, dtype=np.int32)
