@@ -5,7 +5,9 @@
 // never straddles two scenes (scene slot ranges are padded to the block size), so a
 // block reads its scene descriptor once.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "../../include/dem.h"
 
 namespace psim {
@@ -52,45 +54,61 @@ struct SceneDev {
     int32_t n_real;        // particles of the scene; slots [p0+n_real, p0+np) hold inactive padding
 };
 
+// A device array addressed by 32-bit element index: the byte offset is formed in
+// 32 bits, so loads and stores take the SGPR-base + VGPR-offset form and need no
+// 64-bit address arithmetic per access.  Every pool is below 4 GiB (the engine
+// checks its sizes at allocation).
+template <class T>
+struct Pool {
+    T* p = nullptr;
+    Pool() = default;
+    __host__ __device__ Pool(T* q) : p(q) {}
+    template <class I>
+    __device__ __forceinline__ T& operator[](I i) const {
+        return *reinterpret_cast<T*>(reinterpret_cast<char*>(const_cast<typename std::remove_const<T>::type*>(p)) +
+                                     (uint32_t)i * (uint32_t)sizeof(T));
+    }
+};
+
 struct DevArrays {
     // double-buffered neighbour-visible state (cur = read, nxt = written)
-    float4* P[2];      // {x, y, r, del_rot}
-    float2* D[2];      // {del_x, del_y}
-    float4* V;         // {vx, vy, w, rot}  (after a LOM half: the mid-step velocity)
-    float4* A;         // {ax, ay, alpha, 0}
-    int2* M;           // {material | flags, particle id within the scene}
-    float4* EXT;       // external {fx, fy, moment, 0}
-    float4* DATA0;     // {Σfn, Σft, ΣM, data3}
-    float4* DATA1;     // {v*x, v*y, w*, 0}
+    Pool<float4> P[2];      // {x, y, r, del_rot}
+    Pool<float2> D[2];      // {del_x, del_y}
+    Pool<float4> V;         // {vx, vy, w, rot}  (after a LOM half: the mid-step velocity)
+    Pool<float4> A;         // {ax, ay, alpha, 0}
+    Pool<int2> M;           // {material | flags, particle id within the scene}
+    Pool<float4> EXT;       // external {fx, fy, moment, 0}
+    Pool<float4> DATA0;     // {Σfn, Σft, ΣM, data3}
+    Pool<float4> DATA1;     // {v*x, v*y, w*, 0}
     // contact lists, slot-major [KMAX][cap], double-buffered: every step kernel reads
     // the current lists (CC/CP/CF) and writes the next ones (CCn/CPn/CFn)
-    int32_t* CC;       // count
-    int32_t* CP;       // partner id (scene-local), ascending
-    float* CF;         // tangential force
-    int32_t* CCn;
-    int32_t* CPn;
-    float* CFn;
+    Pool<int32_t> CC;       // count
+    Pool<int32_t> CP;       // partner id (scene-local), ascending
+    Pool<float> CF;         // tangential force
+    Pool<int32_t> CCn;
+    Pool<int32_t> CPn;
+    Pool<float> CFn;
     int64_t cap;       // particle pool size (slot stride of CP / CF)
     // bond endpoint records: CSR indexed by (scene p0 + particle id), each list sorted
     // by partner id; records never move
-    int32_t* BOFF;     // [cap + 1]
-    int4* BR;          // {partner id, bond id, bonded, alive}
-    float4* BF;        // {ft, fbt, theta_b, 0}
-    int2* BS;          // per bond {index word, step of break}
-    float* BL;         // per bond rest length
-    float2* BSTR;      // per bond {normal, shear} strength
-    const dem_material* MAT;
+    Pool<int32_t> BOFF;     // [cap + 1]
+    Pool<int4> BR;          // {partner id, bond id, bonded, alive}
+    Pool<float4> BF;        // {ft, fbt, theta_b, 0}
+    Pool<int2> BS;          // per bond {index word, step of break}
+    Pool<float> BL;         // per bond rest length
+    Pool<float2> BSTR;      // per bond {normal, shear} strength
+    Pool<const dem_material> MAT;
     // broad phase
-    int32_t* KEY;      // cell of each slot (global cell index)
-    int32_t* RANK;     // rank inside the cell from the counting atomics
-    int32_t* CNT;      // [ncells_total] counts, zeroed by the scan
-    int32_t* START;    // [ncells_total + 1]
-    int32_t* SIDX;     // slots in cell order
-    int32_t* INV;      // [cap] slot of (scene p0 + particle id)
+    Pool<int32_t> KEY;      // cell of each slot (global cell index)
+    Pool<int32_t> RANK;     // rank inside the cell from the counting atomics
+    Pool<int32_t> CNT;      // [ncells_total] counts, zeroed by the scan
+    Pool<int32_t> START;    // [ncells_total + 1]
+    Pool<int32_t> SIDX;     // slots in cell order
+    Pool<int32_t> INV;      // [cap] slot of (scene p0 + particle id)
     // counters
-    unsigned long long* STAT;  // [0] contact overflow, [1] bond overflow, [2] scan timeouts
+    Pool<unsigned long long> STAT;  // [0] contact overflow, [1] bond overflow, [2] scan timeouts
     // per-tile cell counts of the scan (reduce-then-scan, no inter-block waits)
-    int32_t* TSUM;
+    Pool<int32_t> TSUM;
     int32_t ntiles;
 };
 

@@ -76,8 +76,12 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     rec_cap_ = std::max<int64_t>(rec_cap, 1);
     mat_cap_ = std::max(mat_cap, 1);
     scene_cap_ = std::max(scene_cap, 1);
-    if (slot_cap_ * KMAX > (int64_t)INT32_MAX || cell_cap_ > (int64_t)INT32_MAX / 2) {
-        set_error("pool too large for 32-bit slot indices");
+    // kernels address every pool with 32-bit byte offsets: each must stay below 4 GiB
+    // (the contact lists, KMAX words per slot, bound the particle count at 2^26)
+    const int64_t lim = (int64_t)1 << 32;
+    if (slot_cap_ * KMAX * 4 >= lim || slot_cap_ * 16 >= lim || cell_cap_ * 4 + 8 >= lim ||
+        rec_cap_ * 16 >= lim || bond_cap_ * 8 >= lim) {
+        set_error("pool too large for 32-bit offsets (at most 2^26 - 1 particle slots per context)");
         return DEM_EINVAL;
     }
     int rc;
@@ -556,7 +560,9 @@ int Engine::reorder(const int2* lf_dev, const std::vector<int32_t>& sids) {
     r.dst.CP = r.src.CPn;
     r.dst.CF = r.src.CFn;
     r.cur = cur_;
-    r.with_ext = 1;
+    r.with_ext = 0;
+    for (const SceneHost& sh : scenes_)
+        if (sh.live && sh.has_ext) r.with_ext = 1;  // EXT is read only where PF_EXT is set
     r.with_data = with_data_ ? 1 : 0;
     if (e == hipSuccess) e = launch_reorder_move(r, (int)(slot_cap_ / block_), block_, stream_);
     if (e == hipSuccess) e = launch_identity(SIDX_, slot_cap_, stream_);

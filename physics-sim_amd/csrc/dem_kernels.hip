@@ -4,16 +4,18 @@
 // half-kick, drift, walls, AABB binning) and Sim (2D_Simulation.wgsl:108-478:
 // narrow phase, contact/bond laws, first half-kick).  Here one step is:
 //
-//   k_step     fused "Sim half of step k" + "LOM half of step k+1" per particle slot:
+//   k_step_plain, k_step_bonds (one body, step_slot)
+//              fused "Sim half of step k" + "LOM half of step k+1" per particle slot:
 //              contact search over the 3x3 neighbour cells of the current cell list,
 //              force laws in ascending partner-id order, store_forces, then the next
-//              drift and its cell key with a per-cell counting atomic (stages 1, 2a,
-//              4, 5).  The first launch of a dem_step call runs only the LOM half and
-//              the last one only the Sim half, so the state between calls is the
-//              reference's post-Sim state.
-//   k_scan     exclusive prefix sum of the per-cell counts (single pass, decoupled
-//              look-back, wave64 prefix in registers + LDS across waves); zeroes the
-//              counts for the next step (stages 2b, 3: cell start = prefix).
+//              drift and its cell key with a per-cell counting atomic, one atomic per
+//              run of equal keys in a wave (stages 1, 2a, 4, 5).  The first launch of
+//              a dem_step call runs only the LOM half and the last one only the Sim
+//              half, so the state between calls is the reference's post-Sim state.
+//   k_tile_sums, k_scan_tiles   exclusive prefix sum of the per-cell counts
+//              (reduce-then-scan over tiles of 4096 cells, wave64 prefix in
+//              registers + LDS across waves); zeroes the counts for the next step
+//              (stages 2b, 3: cell start = prefix).
 //   k_scatter  slot -> position START[key] + RANK in the cell-ordered slot list.
 //   k_cell_sort, k_reorder_move   every few broad phases: sort each cell's entries
 //              by particle id and move every per-particle array into that order.
@@ -225,6 +227,31 @@ __device__ __forceinline__ int xcd_block(int bid, int nb) {
 }
 
 
+// Counting-sort binning: KEY[s] = key and RANK[s] = a unique rank in [0, count)
+// among the slots of that cell.  Lanes of the wave that call together and hold
+// the same key as their lower neighbour form a run; the run's lowest lane does one
+// atomicAdd for the whole run and each lane takes base + its offset in the run.
+// In cell-sorted storage neighbours share cells, and same-line atomics from one
+// wave are serialised in L2, so this halves the atomics of the binning there.
+// Only lanes active at the call take part (the exec mask bounds every run).
+__device__ __forceinline__ void bin_slot(const DevArrays& a, int s, int key) {
+    const uint64_t live = __ballot(1);
+    const int lane = (int)(threadIdx.x & 63);
+    const int below = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || !((live >> (lane - 1)) & 1ull) || below != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);      // lanes 0..lane
+    const int h = 63 - __clzll((long long)(heads & upto));                  // this lane's run head
+    const uint64_t above = heads & ~upto;                                   // heads after this lane
+    const int end = above ? __ffsll((long long)above) - 1 : 64;             // first lane after the run
+    const uint64_t run = (end == 64 ? ~0ull : ((1ull << end) - 1ull)) & ~((1ull << h) - 1ull) & live;
+    int base = 0;
+    if (head) base = atomicAdd(&a.CNT[key], __popcll(run));
+    base = __shfl(base, h, 64);
+    a.KEY[s] = key;
+    a.RANK[s] = base + __popcll(run & ((1ull << lane) - 1ull));  // run lanes below this one
+}
+
 __device__ __forceinline__ void copy_list(const DevArrays& a, int s) {
     const int c = a.CC[s];
     a.CCn[s] = c;
@@ -265,15 +292,14 @@ __device__ __forceinline__ bool overlaps(const float4& pj, const float4& p) {
 // Adds partner id `oj` to the ascending list kept in the next-list buffer
 // (column s, stride cap); the largest drops when KMAX are held.
 __device__ __forceinline__ void list_insert(const DevArrays& a, int s, int oj, int& n) {
-    int32_t* col = a.CPn + s;
     const int cap = (int)a.cap;
     int pos = n;
-    while (pos > 0 && col[(pos - 1) * cap] > oj) {
-        if (pos < KMAX) col[pos * cap] = col[(pos - 1) * cap];
+    while (pos > 0 && a.CPn[(pos - 1) * cap + s] > oj) {
+        if (pos < KMAX) a.CPn[pos * cap + s] = a.CPn[(pos - 1) * cap + s];
         pos--;
     }
     if (pos < KMAX) {
-        col[pos * cap] = oj;
+        a.CPn[pos * cap + s] = oj;
         if (n < KMAX) n++;
         else atomicAdd(&a.STAT[0], 1ull);
     } else {
@@ -293,7 +319,7 @@ __device__ __forceinline__ bool bonded_partner(const DevArrays& a, int b_lo, int
 }
 
 template <bool F64, bool BONDS>
-__global__ void __launch_bounds__(256) k_step(StepArgs args) {
+__device__ __forceinline__ void step_slot(const StepArgs& args) {
     const DevArrays& a = args.a;
     const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int s = (int)(blk * blockDim.x + threadIdx.x);
@@ -382,9 +408,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs args) {
             a.P[nxt][s] = p;
             a.D[nxt][s] = d;
             if (fl & SF_BIN) {
-                const int key = S.c0 + S.w * S.h;
-                a.KEY[s] = key;
-                a.RANK[s] = atomicAdd(&a.CNT[key], 1);
+                bin_slot(a, s, S.c0 + S.w * S.h);
             }
         }
         return;
@@ -617,10 +641,20 @@ __global__ void __launch_bounds__(256) k_step(StepArgs args) {
     a.V[s] = make_float4(ivx, ivy, iw, rot);
     if (fl & SF_BIN) {
         int cx, cy;
-        const int key = S.c0 + cell_of(S, px, py, cx, cy);
-        a.KEY[s] = key;
-        a.RANK[s] = atomicAdd(&a.CNT[key], 1);
+        bin_slot(a, s, S.c0 + cell_of(S, px, py, cx, cy));
     }
+}
+
+// Plain scenes: 8 waves per SIMD (at most 64 VGPRs) to hide the latency of the
+// candidate loop.  Bonded scenes carry the bond-record merge and keep the default
+// register budget (8 waves would spill).
+template <bool F64>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_plain(StepArgs args) {
+    step_slot<F64, false>(args);
+}
+template <bool F64>
+__global__ void __launch_bounds__(256) k_step_bonds(StepArgs args) {
+    step_slot<F64, true>(args);
 }
 
 // Binning of the current positions only (parity views and the bond builder).
@@ -641,8 +675,7 @@ __global__ void k_bin(StepArgs args) {
         int cx, cy;
         key = S.c0 + cell_of(S, p.x, p.y, cx, cy);
     }
-    a.KEY[s] = key;
-    a.RANK[s] = atomicAdd(&a.CNT[key], 1);
+    bin_slot(a, s, key);
 }
 
 // ---- cell scan: exclusive prefix of cnt[0..n) into start[0..n], zeroing cnt ----
@@ -813,9 +846,11 @@ __global__ void k_reorder_move(ReorderArgs r) {
     r.dst.P[cur][j] = r.src.P[cur][s];
     r.dst.D[cur][j] = r.src.D[cur][s];
     r.dst.V[j] = r.src.V[s];
-    r.dst.A[j] = r.src.A[s];
     const int2 m = r.src.M[s];
     r.dst.M[j] = m;
+    // A of an active particle of a moving scene is dead: the re-sort follows a
+    // binning drift, so a force pass rewrites A before the next drift reads it
+    if (!move || (m.x & PF_INACTIVE)) r.dst.A[j] = r.src.A[s];
     if (r.with_ext) r.dst.EXT[j] = r.src.EXT[s];
     if (r.with_data) {
         r.dst.DATA0[j] = r.src.DATA0[s];
@@ -824,8 +859,8 @@ __global__ void k_reorder_move(ReorderArgs r) {
     const int c = r.src.CC[s];
     r.dst.CC[j] = c;
     for (int q = 0; q < c && q < KMAX; q++) {
-        r.dst.CP[(int64_t)q * r.dst.cap + j] = r.src.CP[(int64_t)q * r.src.cap + s];
-        r.dst.CF[(int64_t)q * r.dst.cap + j] = r.src.CF[(int64_t)q * r.src.cap + s];
+        r.dst.CP[q * (int)r.dst.cap + j] = r.src.CP[q * (int)r.src.cap + s];
+        r.dst.CF[q * (int)r.dst.cap + j] = r.src.CF[q * (int)r.src.cap + s];
     }
     r.dst.INV[S.p0 + m.y] = j;
 }
@@ -921,11 +956,11 @@ __global__ void k_bond_pairs(StepArgs args, int reach, int maxb, float factor, i
 hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, hipStream_t st) {
     dim3 g(nblocks), b(block);
     if (f64) {
-        if (bonds) hipLaunchKernelGGL((k_step<true, true>), g, b, 0, st, args);
-        else hipLaunchKernelGGL((k_step<true, false>), g, b, 0, st, args);
+        if (bonds) hipLaunchKernelGGL((k_step_bonds<true>), g, b, 0, st, args);
+        else hipLaunchKernelGGL((k_step_plain<true>), g, b, 0, st, args);
     } else {
-        if (bonds) hipLaunchKernelGGL((k_step<false, true>), g, b, 0, st, args);
-        else hipLaunchKernelGGL((k_step<false, false>), g, b, 0, st, args);
+        if (bonds) hipLaunchKernelGGL((k_step_bonds<false>), g, b, 0, st, args);
+        else hipLaunchKernelGGL((k_step_plain<false>), g, b, 0, st, args);
     }
     return hipGetLastError();
 }
@@ -939,8 +974,8 @@ int scan_tiles(int n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
 hipError_t launch_scan(const DevArrays& a, int ncells, hipStream_t st) {
     const int nt = scan_tiles(ncells);
-    hipLaunchKernelGGL(k_tile_sums, dim3(nt), dim3(SCAN_THREADS), 0, st, a.CNT, ncells, a.TSUM);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, st, a.CNT, a.START, ncells, a.TSUM);
+    hipLaunchKernelGGL(k_tile_sums, dim3(nt), dim3(SCAN_THREADS), 0, st, a.CNT.p, ncells, a.TSUM.p);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, st, a.CNT.p, a.START.p, ncells, a.TSUM.p);
     return hipGetLastError();
 }
 
