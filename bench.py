@@ -36,10 +36,57 @@ def parse():
     ap.add_argument("--cell", type=float, default=0.0, help="grid cell side (0: just above 2 r_max)")
     ap.add_argument("--reorder", type=int, default=16)
     ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--collision-interval", type=int, default=1,
+                    help="narrow phase every k-th step (the reference's collision_interval setting)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-particles", type=int, default=262144)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also time a plain device copy of this many MiB (the reference kernel for the HBM "
+                         "traffic counters and the achievable copy bandwidth)")
     return ap.parse_args()
+
+
+def copy_calibration(mib, reps=5):
+    """A plain device-to-device copy of `mib` MiB, `reps` times, through the HIP
+    runtime the DEM library itself links (its blit copy kernel): known bytes read
+    and written per launch, so that the FETCH_SIZE / WRITE_SIZE counters of a
+    profiled run can be scaled to bytes.  Returns the mean launch time and the copy
+    bandwidth (read + write bytes per second)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    nbytes = mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError("%s failed: hip error %d" % (what, rc))
+    ok(hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)), "hipMalloc")
+    ok(hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)), "hipMalloc")
+    try:
+        ok(hip.hipMemset(src, 0x5a, C.c_size_t(nbytes)), "hipMemset")
+        ok(hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
+        ok(hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
+        D2D = 3
+        ok(hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), D2D, None), "hipMemcpyAsync")
+        ok(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+        ok(hip.hipEventRecord(e0, None), "hipEventRecord")
+        for _ in range(reps):
+            ok(hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), D2D, None), "hipMemcpyAsync")
+        ok(hip.hipEventRecord(e1, None), "hipEventRecord")
+        ok(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+        ms = C.c_float()
+        ok(hip.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+        probe = (C.c_ubyte * 64)()
+        ok(hip.hipMemcpy(probe, C.c_void_p(dst.value + nbytes - 64), C.c_size_t(64), 2), "hipMemcpy")
+        assert all(b == 0x5a for b in probe)
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    t = ms.value / reps
+    return {"mib": mib, "bytes_read": nbytes, "bytes_written": nbytes, "ms": t, "reps": reps,
+            "gbs_read_plus_write": 2 * nbytes / (t * 1e-3) / 1e9}
 
 
 def cpu_baseline(args):
@@ -84,6 +131,7 @@ def main():
     from psim.compute import ComputeProg
 
     s, st = scenes.column_collapse(n=args.particles, per_row=args.per_row)
+    s.physics.collision_interval = args.collision_interval
     grid = ComputeProg.default_grid(s, st, cell_size=args.cell if args.cell > 0 else None)
     prog = ComputeProg(s, st, grid=grid, record_data=False, reorder_interval=args.reorder, block_size=args.block)
     n = st.p_count
@@ -133,6 +181,7 @@ def main():
             "workload": "granular column collapse, %d discs (log-normal r median 0.35 clip [0.175,0.525]), "
                         "4096x2048 box, dt 1e-4" % n,
             "grid": [grid.w, grid.h], "cell_size": grid.cell_size, "reorder_interval": args.reorder,
+            "collision_interval": args.collision_interval,
             "block": args.block, "mean_contacts_per_particle": nc,
             "hbm_frac_whole_step": value * b_alg / (HBM_PEAK_GBS * 1e9),
             "bytes_per_particle_step_alg": b_alg,
@@ -146,6 +195,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }
     prog.close()
+    if args.copy_mib > 0:
+        out["config"]["copy_calibration"] = copy_calibration(args.copy_mib)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

@@ -252,8 +252,8 @@ __device__ __forceinline__ void bin_slot(const DevArrays& a, int s, int key) {
     a.RANK[s] = base + __popcll(run & ((1ull << lane) - 1ull));  // run lanes below this one
 }
 
-__device__ __forceinline__ void copy_list(const DevArrays& a, int s) {
-    const int c = a.CC[s];
+// Carries the contact list of slot s (c entries) over to the next-list buffers.
+__device__ __forceinline__ void copy_list(const DevArrays& a, int s, int c) {
     a.CCn[s] = c;
     const int cap = (int)a.cap;
     for (int q = 0; q < c && q < KMAX; q++) {
@@ -330,7 +330,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const int2 lfw = args.lf[sid];
     const uint32_t fl = (uint32_t)lfw.x;
     if (!(fl & SF_ACTIVE)) {
-        copy_list(a, s);  // the lists flip after every launch
+        copy_list(a, s, a.CC[s]);  // the lists flip after every launch
         return;
     }
     const int cur = args.cur, nxt = cur ^ 1;
@@ -348,6 +348,13 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         b_lo = a.BOFF[S.p0 + oid];
         b_hi = a.BOFF[S.p0 + oid + 1];
     }
+
+    // own-state loads issued before the contact search so that their latency
+    // overlaps it (the compiler keeps them behind the list stores otherwise)
+    float4 v = a.V[s];
+    const int cc_old = a.CC[s];
+    const int mat = (int)(pf & PF_MAT_MASK);
+    const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
 
     // ---- narrow phase: every overlapping partner in the 3x3 stencil ----
     int n = 0;
@@ -370,36 +377,37 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 e1[r] = 0;
             }
         }
+        // the three stencil rows advance together, two candidates each per round: the
+        // slot-id loads of every row are in flight before the first position load, so
+        // a round costs two dependent memory latencies whatever the row count
+        auto test = [&](int j, const float4& q) {
+            if (j != s && overlaps(q, p)) {
+                const int oj = a.M[j].y;
+                if (!bonded_partner<BONDS>(a, b_lo, b_hi, oj)) list_insert(a, s, oj, n);
+            }
+        };
+        int e[3] = {e0[0], e0[1], e0[2]};
+        while (e[0] < e1[0] || e[1] < e1[1] || e[2] < e1[2]) {
+            int j[6];
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
-            int e = e0[r];
-            for (; e + 1 < e1[r]; e += 2) {
-                const int j0 = a.SIDX[e], j1 = a.SIDX[e + 1];
-                const float4 q0 = a.P[cur][j0];
-                const float4 q1 = a.P[cur][j1];
-                if (j0 != s && overlaps(q0, p)) {
-                    const int oj = a.M[j0].y;
-                    if (!bonded_partner<BONDS>(a, b_lo, b_hi, oj)) list_insert(a, s, oj, n);
-                }
-                if (j1 != s && overlaps(q1, p)) {
-                    const int oj = a.M[j1].y;
-                    if (!bonded_partner<BONDS>(a, b_lo, b_hi, oj)) list_insert(a, s, oj, n);
-                }
+            for (int r = 0; r < 3; r++) {
+                j[2 * r] = e[r] < e1[r] ? a.SIDX[e[r]] : -1;
+                j[2 * r + 1] = e[r] + 1 < e1[r] ? a.SIDX[e[r] + 1] : -1;
             }
-            if (e < e1[r]) {
-                const int j0 = a.SIDX[e];
-                const float4 q0 = a.P[cur][j0];
-                if (j0 != s && overlaps(q0, p)) {
-                    const int oj = a.M[j0].y;
-                    if (!bonded_partner<BONDS>(a, b_lo, b_hi, oj)) list_insert(a, s, oj, n);
-                }
-            }
+            float4 q[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) q[k] = j[k] >= 0 ? a.P[cur][j[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int k = 0; k < 6; k++)
+                if (j[k] >= 0) test(j[k], q[k]);
+#pragma unroll
+            for (int r = 0; r < 3; r++) e[r] += 2;
         }
     }
 
     if (!act) {
         // never binned into a real cell, never a partner; state kept in both buffers
-        copy_list(a, s);
+        copy_list(a, s, cc_old);
         if ((fl & SF_SIM) && S.record) {
             const float4 d0 = a.DATA0[s];
             a.DATA0[s] = make_float4(0.0f, 0.0f, 0.0f, d0.w);
@@ -414,9 +422,6 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         return;
     }
 
-    float4 v = a.V[s];
-    const int mat = (int)(pf & PF_MAT_MASK);
-    const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
     float ax, ay, al;
 
     if (fl & SF_SIM) {
@@ -426,7 +431,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         Data3 dd{0.0f, 0.0f, 0.0f};
         float nfx = 0.0f, nfy = 0.0f, nm = 0.0f;
         const int step = lfw.y;
-        const int oc = min(a.CC[s], KMAX);
+        const int oc = min(cc_old, KMAX);
         if (!narrow) {
             // no narrow phase this step: the existing contacts carry over
             n = oc;
@@ -574,7 +579,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             return;
         }
     } else {
-        copy_list(a, s);
+        copy_list(a, s, cc_old);
         const float4 av = a.A[s];
         ax = av.x;
         ay = av.y;
