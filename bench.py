@@ -10,7 +10,8 @@ Prints one JSON line.  `value` is particle-steps/s of the timed region (inputs
 resident in HBM, host clock around K steps bracketed by device syncs).
 `roofline` is for the dominant kernel (the fused step kernel): algorithmic bytes
 per launch = N * (68 + 16 * mean contacts per particle) over its mean duration
-from HIP events on the ctx's stream.  `cpu_baseline` times the CPU oracle's
+from HIP events on the ctx's stream; `traffic` is the kernel's measured HBM bytes per
+fused launch (counter summary under profiles/, default workload only).  `cpu_baseline` times the CPU oracle's
 restatement of the reference algorithm (AABB bucket grid, 14-slot table) on a
 bounded sample of the same scene.
 """
@@ -24,6 +25,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "physics-sim_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per fused step launch measured with the FETCH_SIZE / WRITE_SIZE counters
+# (scaled by a copy of known size) on the default workload
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_step_plain_column_collapse_4M_traffic.md")
+
+
+def measured_traffic(args):
+    """Bytes per launch of the step kernel from the committed counter summary, for the
+    workload that summary covers (the default one); None otherwise."""
+    default = (args.particles == 4194304 and args.per_row == 2048 and args.cell == 0.0 and args.block == 256
+               and args.collision_interval == 1)
+    if not default or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def parse():
@@ -133,7 +150,10 @@ def main():
     s, st = scenes.column_collapse(n=args.particles, per_row=args.per_row)
     s.physics.collision_interval = args.collision_interval
     grid = ComputeProg.default_grid(s, st, cell_size=args.cell if args.cell > 0 else None)
+    t_up = time.perf_counter()
     prog = ComputeProg(s, st, grid=grid, record_data=False, reorder_interval=args.reorder, block_size=args.block)
+    prog.sync()
+    t_up = time.perf_counter() - t_up   # pool allocation + host layout conversion + PCIe upload
     n = st.p_count
     print("bench: %d discs, grid %dx%d cell %.4f; warmup %d" % (n, grid.w, grid.h, grid.cell_size, args.warmup),
           file=sys.stderr, flush=True)
@@ -156,6 +176,9 @@ def main():
     prog.sync()
     tm = prog.timing(False)
     c2 = prog.counters()
+    t_down = time.perf_counter()
+    prog.update_state()                 # PCIe download + rebuild of the reference layouts
+    t_down = time.perf_counter() - t_down
     nc = 0.5 * (c1["contacts"] + c2["contacts"]) / n
     ms_step = 1e3 * wall / args.steps
     value = n * args.steps / wall
@@ -185,6 +208,7 @@ def main():
             "block": args.block, "mean_contacts_per_particle": nc,
             "hbm_frac_whole_step": value * b_alg / (HBM_PEAK_GBS * 1e9),
             "bytes_per_particle_step_alg": b_alg,
+            "upload_ms": 1e3 * t_up, "download_ms": 1e3 * t_down,
             "kernel_ms": {"step": k_ms, "scan": tm["scan_ms"] / max(tm["n_scan"], 1),
                           "scatter": tm["scatter_ms"] / max(tm["n_scatter"], 1),
                           "reorder": tm["reorder_ms"] / max(tm["n_reorder"], 1)},
@@ -192,7 +216,7 @@ def main():
                          for k in c2},
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args)},
     }
     prog.close()
     if args.copy_mib > 0:

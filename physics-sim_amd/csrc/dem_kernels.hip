@@ -896,11 +896,14 @@ __global__ void k_kinetic(DevArrays a, int cur, int m0, int nm, int p0, int np, 
 }
 
 // Sum of the contact counts of a scene (mean contacts per particle) - integer,
-// order independent; one atomic per block.
+// order independent.  A grid of COUNT_BLOCKS strides over the scene, so only that
+// many same-address atomics meet in L2.
+constexpr int COUNT_BLOCKS = 256;
 __global__ void __launch_bounds__(256) k_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out) {
     __shared__ int s_w[4];
-    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    int c = s < np ? cc[p0 + s] : 0;
+    int c = 0;
+    for (int s = (int)(blockIdx.x * blockDim.x + threadIdx.x); s < np; s += (int)(gridDim.x * blockDim.x))
+        c += cc[p0 + s];
     c = wave_sum(c);
     if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
     __syncthreads();
@@ -1016,7 +1019,8 @@ hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, i
 
 hipError_t launch_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out, hipStream_t st) {
     const int b = 256;
-    hipLaunchKernelGGL(k_count_contacts, dim3((np + b - 1) / b), dim3(b), 0, st, cc, p0, np, out);
+    const int nb = (np + b - 1) / b;
+    hipLaunchKernelGGL(k_count_contacts, dim3(nb < COUNT_BLOCKS ? nb : COUNT_BLOCKS), dim3(b), 0, st, cc, p0, np, out);
     return hipGetLastError();
 }
 
