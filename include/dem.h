@@ -217,6 +217,54 @@ int dem_build_bond_pairs(dem_ctx* ctx, int32_t max_bonds, float factor,
  *  out[4] re-sort), and the number of launches of each. */
 int dem_last_step_timing(dem_ctx* ctx, int32_t enable, double* ms_out, int64_t* launches_out);
 
+/* ---- Serving loop: continuous batching of independent scenes ----
+ * No counterpart in the reference (it runs one scene in one window).  One server owns
+ * one set of device pools on one HIP stream.  dem_server_submit queues a scene (its
+ * parameters, initial state and step count; everything is copied).  Each
+ * dem_server_tick admits queued scenes in arrival order while their footprint fits
+ * the pools (up to `bypass` later, smaller requests may pass a head that does not
+ * fit), advances every resident scene by up to `substeps` steps in shared launches,
+ * and retires the finished ones into host staging.  A scene's result is the same as
+ * its solo run through dem_ctx, bit for bit. */
+typedef struct dem_server dem_server;
+
+typedef struct dem_server_params {
+    int64_t slot_capacity;      /* particle slots of the pool (each scene rounds up to block_size) */
+    int64_t cell_capacity;      /* grid cells of the pool (each scene takes w*h + 1) */
+    int64_t bond_capacity;      /* bonds of the pool */
+    int64_t record_capacity;    /* bond endpoint records; 0 = 2*bond_capacity + slot_capacity */
+    int32_t scene_capacity;     /* scenes resident at once, at most */
+    int32_t material_capacity;  /* material records; 0 = 8 per scene */
+    int32_t block_size;         /* 0 = 256 */
+    int32_t record_data;        /* maintain data[7] for the scenes */
+    int32_t substeps;           /* steps per tick for each resident scene (> 0) */
+    int32_t max_resident;       /* <= 0: no limit beyond the pools; 1 = one scene at a time */
+    int32_t bypass;             /* later requests tried when the head does not fit */
+    int32_t contact_slots;      /* contact table width of results whose request gave none; 0 = none */
+} dem_server_params;
+
+#define DEM_REQ_QUEUED   0
+#define DEM_REQ_RESIDENT 1
+#define DEM_REQ_DONE     2
+
+int dem_server_create(const dem_server_params* params, dem_server** out);
+int dem_server_destroy(dem_server* server);
+/* Queues a scene: `params` as for dem_create, `state` as for dem_upload_state, run for
+ * `nsteps` steps.  Writes a ticket. */
+int dem_server_submit(dem_server* server, const dem_params* params, const dem_state_view* state, int32_t nsteps,
+                      int64_t* ticket);
+/* One tick (admit, advance, retire); writes how many scenes finished in it. Synchronous. */
+int dem_server_tick(dem_server* server, int32_t* completed);
+/* DEM_REQ_QUEUED / DEM_REQ_RESIDENT / DEM_REQ_DONE */
+int dem_server_status(dem_server* server, int64_t ticket, int32_t* status);
+/* Copies a finished scene's state into `out` (n must match; NULL arrays skipped),
+ * writes its submit-to-done latency in seconds, and forgets the ticket. */
+int dem_server_take(dem_server* server, int64_t ticket, dem_state_view* out, double* latency_s);
+/* out[0] completed, [1] resident, [2] queued, [3] p50 latency s, [4] p99 latency s,
+ * [5] scenes/s from the first submit to the last completion, [6] particle-steps done,
+ * [7] ticks, [8] step launches (max substeps per tick, summed). */
+int dem_server_stats(dem_server* server, double* out, int32_t n_out);
+
 #ifdef __cplusplus
 }
 #endif

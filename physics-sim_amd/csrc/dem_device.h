@@ -89,9 +89,9 @@ struct DevArrays {
     Pool<int32_t> CPn;
     Pool<float> CFn;
     int64_t cap;       // particle pool size (slot stride of CP / CF)
-    // bond endpoint records: CSR indexed by (scene p0 + particle id), each list sorted
+    // bond endpoint records: ranges indexed by (scene p0 + particle id), each list sorted
     // by partner id; records never move
-    Pool<int32_t> BOFF;     // [cap + 1]
+    Pool<int2> BOFF;        // [cap] record range [x, y) of (scene p0 + particle id)
     Pool<int4> BR;          // {partner id, bond id, bonded, alive}
     Pool<float4> BF;        // {ft, fbt, theta_b, 0}
     Pool<int2> BS;          // per bond {index word, step of break}
@@ -118,6 +118,8 @@ struct StepArgs {
     const int2* lf;            // per scene {step flags, global step index of the Sim half}
     DevArrays a;
     int cur;                   // which P/D buffer holds the current positions
+    int flip;                  // some scene drifts in this launch, so cur flips after it: every slot
+                               // that does not drift carries its P/D over to the other buffer
 };
 
 struct ReorderArgs {

@@ -115,7 +115,9 @@ private:
     StepArgs make_args(const int2* lf) const;
     int write_scene_table();
     int reorder(const int2* lf_dev, const std::vector<int32_t>& sids);
-    int launch_grid_blocks() const;
+    int launch_grid_blocks() const;   // blocks covering the whole slot pool
+    int active_blocks() const;        // blocks covering the slots below the high-water mark
+    void update_high_water();
     int alloc_region(std::vector<std::pair<int64_t, int64_t>>& used, int64_t cap, int64_t need, int64_t align,
                      int64_t* out);
     SceneDev scene_dev(const SceneHost& sh) const;
@@ -125,13 +127,15 @@ private:
     int block_ = 256;
     bool with_data_ = true;
     int64_t slot_cap_ = 0, cell_cap_ = 0, bond_cap_ = 0, rec_cap_ = 0;
+    // ends of the highest live slot and cell regions: launches and the cell scan stop there
+    int64_t hw_slots_ = 0, hw_cells_ = 0;
     int32_t mat_cap_ = 0, scene_cap_ = 0;
     int set_ = 0;   // which DevSet is current
     int cur_ = 0;   // which P/D buffer is current
     DevSet sets_[2];
     ContactLists lists_[2];
     int lcur_ = 0;  // which contact-list buffer is current
-    int32_t* BOFF_ = nullptr;
+    int2* BOFF_ = nullptr;      // per slot (scene p0 + particle id): bond record range [x, y)
     int4* BR_ = nullptr;
     float4* BF_ = nullptr;
     int2* BS_ = nullptr;

@@ -103,7 +103,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
         if ((rc = check(e, "hipMalloc contact lists"))) return rc;
     }
     hipError_t e = hipSuccess;
-    e = e ? e : dalloc(&BOFF_, S + 1); e = e ? e : dalloc(&BR_, rec_cap_); e = e ? e : dalloc(&BF_, rec_cap_);
+    e = e ? e : dalloc(&BOFF_, S); e = e ? e : dalloc(&BR_, rec_cap_); e = e ? e : dalloc(&BF_, rec_cap_);
     e = e ? e : dalloc(&BS_, bond_cap_); e = e ? e : dalloc(&BL_, bond_cap_); e = e ? e : dalloc(&BSTR_, bond_cap_);
     e = e ? e : dalloc(&MAT_, mat_cap_);
     e = e ? e : dalloc(&KEY_, S); e = e ? e : dalloc(&RANK_, S); e = e ? e : dalloc(&CNT_, cell_cap_ + 1);
@@ -122,7 +122,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = hipMemsetAsync(CNT_, 0, sizeof(int32_t) * (cell_cap_ + 1), stream_);
     e = e ? e : hipMemsetAsync(START_, 0, sizeof(int32_t) * (cell_cap_ + 2), stream_);
     e = e ? e : hipMemsetAsync(STAT_, 0, sizeof(unsigned long long) * 8, stream_);
-    e = e ? e : hipMemsetAsync(BOFF_, 0, sizeof(int32_t) * (S + 1), stream_);
+    e = e ? e : hipMemsetAsync(BOFF_, 0, sizeof(int2) * S, stream_);
     for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int2) * S, stream_);
     for (auto& l : lists_) e = e ? e : hipMemsetAsync(l.CC, 0, sizeof(int32_t) * S, stream_);
     std::vector<int32_t> blk(nblk, -1);
@@ -133,6 +133,20 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
 
 int Engine::launch_grid_blocks() const {
     return (int)((slot_cap_ + block_ - 1) / block_);
+}
+
+int Engine::active_blocks() const {
+    return (int)((hw_slots_ + block_ - 1) / block_);
+}
+
+void Engine::update_high_water() {
+    hw_slots_ = 0;
+    hw_cells_ = 0;
+    for (const SceneHost& sh : scenes_) {
+        if (!sh.live) continue;
+        hw_slots_ = std::max<int64_t>(hw_slots_, sh.p0 + sh.slot_cap);
+        hw_cells_ = std::max<int64_t>(hw_cells_, sh.c0 + sh.cell_cap);
+    }
 }
 
 int64_t Engine::free_slots() const {
@@ -235,6 +249,7 @@ int Engine::add_scene(const dem_params& p, int32_t capacity, int32_t bond_cap, i
     } else {
         scenes_[sid] = sh;
     }
+    update_high_water();
     hipError_t e = hipMemcpyAsync(MAT_ + sh.m0, sh.mats.data(), sizeof(dem_material) * sh.nm, hipMemcpyHostToDevice,
                                   stream_);
     if ((rc = check(e, "upload materials"))) return rc;
@@ -252,6 +267,7 @@ int Engine::remove_scene(int32_t sid) {
     if (sh.rec_cap) release_region(used_recs_, sh.r0);
     release_region(used_mats_, sh.m0);
     sh = SceneHost();
+    update_high_water();
     table_dirty_ = true;
     return DEM_OK;
 }
@@ -495,7 +511,11 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     }
     if ((int64_t)BR.size() > sh.rec_cap) { set_error("more bond records than the scene's record capacity"); return DEM_ECAPACITY; }
     sh.nrec = (int32_t)BR.size();
-    for (auto& o : BOFF) o += (int32_t)sh.r0;
+    // per particle id: its record range [begin, end) in the pool (one entry per slot, so
+    // no scene's ranges touch a neighbour region)
+    std::vector<int2> BRANGE(np);
+    for (int32_t i = 0; i < np; i++)
+        BRANGE[i] = make_int2(BOFF[i] + (int32_t)sh.r0, BOFF[i + 1] + (int32_t)sh.r0);
     // device copies: the current set, the current P/D buffer
     hipError_t e = hipSuccess;
     const DevSet& S = sets_[set_];
@@ -521,7 +541,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     }
     h2d(INV_ + p0, INV.data(), sizeof(int32_t) * np);
     h2d(SIDX_ + p0, SIDX.data(), sizeof(int32_t) * np);
-    h2d(BOFF_ + p0, BOFF.data(), sizeof(int32_t) * (np + 1));
+    h2d(BOFF_ + p0, BRANGE.data(), sizeof(int2) * np);
     if (sh.nrec) {
         h2d(BR_ + sh.r0, BR.data(), sizeof(int4) * sh.nrec);
         h2d(BF_ + sh.r0, BF.data(), sizeof(float4) * sh.nrec);
@@ -564,8 +584,8 @@ int Engine::reorder(const int2* lf_dev, const std::vector<int32_t>& sids) {
     for (const SceneHost& sh : scenes_)
         if (sh.live && sh.has_ext) r.with_ext = 1;  // EXT is read only where PF_EXT is set
     r.with_data = with_data_ ? 1 : 0;
-    if (e == hipSuccess) e = launch_reorder_move(r, (int)(slot_cap_ / block_), block_, stream_);
-    if (e == hipSuccess) e = launch_identity(SIDX_, slot_cap_, stream_);
+    if (e == hipSuccess) e = launch_reorder_move(r, active_blocks(), block_, stream_);
+    if (e == hipSuccess) e = launch_identity(SIDX_, hw_slots_, stream_);
     int rc = check(e, "reorder");
     if (rc) return rc;
     set_ ^= 1;
@@ -639,7 +659,7 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
         any_f64 |= scenes_[sid].params.use_f64 != 0;
         any_bonds |= scenes_[sid].nrec > 0;
     }
-    const int nblk = launch_grid_blocks();
+    const int nblk = active_blocks();
     // timing
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
     size_t ev_used = 0;
@@ -665,11 +685,6 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
 
     for (int64_t k = 0; k < nlaunch; k++) {
         const int2* lfk = d_lf_ + k * scene_cap_;
-        StepArgs args = make_args(lfk);
-        e = timed(0, [&]() { return launch_step(args, nblk, block_, any_f64, any_bonds, stream_); });
-        if ((rc = check(e, "step kernel"))) return rc;
-        lcur_ ^= 1;  // the step kernel wrote every live slot's next contact list
-        // the LOM half wrote the next positions
         bool binned = false, lom = false;
         std::vector<int32_t> active;
         for (size_t i = 0; i < sids.size(); i++) {
@@ -677,9 +692,15 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
             if (w.x & SF_LOM) lom = true;
             if (w.x & SF_BIN) { binned = true; active.push_back(sids[i]); }
         }
+        StepArgs args = make_args(lfk);
+        args.flip = lom ? 1 : 0;
+        e = timed(0, [&]() { return launch_step(args, nblk, block_, any_f64, any_bonds, stream_); });
+        if ((rc = check(e, "step kernel"))) return rc;
+        lcur_ ^= 1;  // the step kernel wrote every live slot's next contact list
+        // some LOM half wrote the next positions; every other slot carried its own over
         if (lom) cur_ ^= 1;
         if (binned) {
-            e = timed(1, [&]() { return launch_scan(arrays(set_), (int)cell_cap_, stream_); });
+            e = timed(1, [&]() { return launch_scan(arrays(set_), (int)hw_cells_, stream_); });
             if ((rc = check(e, "scan"))) return rc;
             e = timed(2, [&]() { return launch_scatter(make_args(lfk), nblk, block_, stream_); });
             if ((rc = check(e, "scatter"))) return rc;
@@ -734,7 +755,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     std::vector<int2> M(np);
     std::vector<int32_t> INV(np), CC(np), CP((size_t)np * KMAX);
     std::vector<float> CF((size_t)np * KMAX);
-    std::vector<int32_t> BOFF(np + 1);
+    std::vector<int2> BRANGE(np);
     std::vector<int4> BR(std::max(sh.nrec, 1));
     std::vector<float4> BF(std::max(sh.nrec, 1));
     std::vector<int2> BS(std::max(sh.nb, 1));
@@ -759,7 +780,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
             d2h(CP.data() + (size_t)q * np, L.CP + (int64_t)q * slot_cap_ + p0, sizeof(int32_t) * np);
             d2h(CF.data() + (size_t)q * np, L.CF + (int64_t)q * slot_cap_ + p0, sizeof(float) * np);
         }
-        d2h(BOFF.data(), BOFF_ + p0, sizeof(int32_t) * (np + 1));
+        d2h(BRANGE.data(), BOFF_ + p0, sizeof(int2) * np);
         if (sh.nrec) {
             d2h(BR.data(), BR_ + sh.r0, sizeof(int4) * sh.nrec);
             d2h(BF.data(), BF_ + sh.r0, sizeof(float4) * sh.nrec);
@@ -815,7 +836,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
                 x.bonded = -1;
                 row.push_back(x);
             }
-            for (int32_t q = BOFF[i] - (int32_t)sh.r0; q < BOFF[i + 1] - (int32_t)sh.r0; q++) {
+            for (int32_t q = BRANGE[i].x - (int32_t)sh.r0; q < BRANGE[i].y - (int32_t)sh.r0; q++) {
                 const int4 br = BR[q];
                 if (br.w == 0) continue;
                 const float4 bf = BF[q];
@@ -932,9 +953,9 @@ int Engine::debug_broad_phase(int32_t sid, int32_t* keys, int32_t* sorted_ids, i
     hipError_t e = hipMemcpyAsync(d_lf_, lf.data(), sizeof(int2) * scene_cap_, hipMemcpyHostToDevice, stream_);
     StepArgs args = make_args(d_lf_);
     const int ncl = sh.params.grid.w * sh.params.grid.h;
-    if (e == hipSuccess) e = launch_bin(args, launch_grid_blocks(), block_, stream_);
-    if (e == hipSuccess) e = launch_scan(arrays(set_), (int)cell_cap_, stream_);
-    if (e == hipSuccess) e = launch_scatter(args, launch_grid_blocks(), block_, stream_);
+    if (e == hipSuccess) e = launch_bin(args, active_blocks(), block_, stream_);
+    if (e == hipSuccess) e = launch_scan(arrays(set_), (int)hw_cells_, stream_);
+    if (e == hipSuccess) e = launch_scatter(args, active_blocks(), block_, stream_);
     if (e == hipSuccess) e = launch_cell_sort(args, (int)sh.c0, (int)(sh.c0 + ncl + 1), (int)sh.p0, stream_);
     std::vector<int32_t> KEY(sh.np), SIDX(sh.np), START(ncl + 2);
     std::vector<int2> M(sh.np);
@@ -1006,7 +1027,7 @@ int Engine::bond_pairs(int32_t sid, int32_t max_bonds, float factor, int32_t* pa
     std::vector<int2> lf(scene_cap_, make_int2(0, 0));
     lf[sid] = make_int2((int)SF_ACTIVE, 0);
     if (e == hipSuccess) e = hipMemcpyAsync(d_lf_, lf.data(), sizeof(int2) * scene_cap_, hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess) e = launch_bond_pairs(make_args(d_lf_), launch_grid_blocks(), block_, reach, max_bonds, factor, d_cnt, d_j, stream_);
+    if (e == hipSuccess) e = launch_bond_pairs(make_args(d_lf_), active_blocks(), block_, reach, max_bonds, factor, d_cnt, d_j, stream_);
     std::vector<int32_t> cnt(std::max(n, 1)), jj((size_t)std::max(n, 1) * max_bonds);
     if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(jj.data(), d_j, sizeof(int32_t) * (size_t)n * max_bonds, hipMemcpyDeviceToHost, stream_);

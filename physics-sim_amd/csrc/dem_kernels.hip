@@ -329,11 +329,15 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     if (s < S.p0 || s >= S.p0 + S.np) return;   // whole blocks (np is a multiple of the block)
     const int2 lfw = args.lf[sid];
     const uint32_t fl = (uint32_t)lfw.x;
+    const int cur = args.cur, nxt = cur ^ 1;
     if (!(fl & SF_ACTIVE)) {
         copy_list(a, s, a.CC[s]);  // the lists flip after every launch
+        if (args.flip) {
+            a.P[nxt][s] = a.P[cur][s];
+            a.D[nxt][s] = a.D[cur][s];
+        }
         return;
     }
-    const int cur = args.cur, nxt = cur ^ 1;
     const dem_uniform& u = S.u;
     const float dt = u.dT;
     const int2 mw = a.M[s];
@@ -345,8 +349,9 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const int oid = mw.y;
     int b_lo = 0, b_hi = 0;
     if (BONDS && act) {
-        b_lo = a.BOFF[S.p0 + oid];
-        b_hi = a.BOFF[S.p0 + oid + 1];
+        const int2 br = a.BOFF[S.p0 + oid];
+        b_lo = br.x;
+        b_hi = br.y;
     }
 
     // own-state loads issued before the contact search so that their latency
@@ -412,13 +417,11 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             const float4 d0 = a.DATA0[s];
             a.DATA0[s] = make_float4(0.0f, 0.0f, 0.0f, d0.w);
         }
-        if (fl & SF_LOM) {
+        if (args.flip) {
             a.P[nxt][s] = p;
             a.D[nxt][s] = d;
-            if (fl & SF_BIN) {
-                bin_slot(a, s, S.c0 + S.w * S.h);
-            }
         }
+        if ((fl & SF_LOM) && (fl & SF_BIN)) bin_slot(a, s, S.c0 + S.w * S.h);
         return;
     }
 
@@ -576,6 +579,11 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         if (!(fl & SF_LOM)) {
             a.V[s] = v;
             a.A[s] = make_float4(ax, ay, al, 0.0f);
+            if (args.flip) {
+                // force pass only, while other scenes drift: positions carry over
+                a.P[nxt][s] = p;
+                a.D[nxt][s] = d;
+            }
             return;
         }
     } else {
@@ -650,16 +658,26 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     }
 }
 
+// Whether this block's scene measures distances in f64.  A launch that holds any
+// f64 scene runs the F64 instantiation; its f32 scenes still take the f32 body, so a
+// scene's arithmetic never depends on the other scenes of the launch.
+__device__ __forceinline__ bool block_scene_f64(const StepArgs& args) {
+    const int sid = args.blk_scene[xcd_block((int)blockIdx.x, (int)gridDim.x)];
+    return sid >= 0 && args.scenes[sid].use_f64 != 0;
+}
+
 // Plain scenes: 8 waves per SIMD (at most 64 VGPRs) to hide the latency of the
 // candidate loop.  Bonded scenes carry the bond-record merge and keep the default
 // register budget (8 waves would spill).
 template <bool F64>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_plain(StepArgs args) {
-    step_slot<F64, false>(args);
+    if (F64 && block_scene_f64(args)) step_slot<true, false>(args);
+    else step_slot<false, false>(args);
 }
 template <bool F64>
 __global__ void __launch_bounds__(256) k_step_bonds(StepArgs args) {
-    step_slot<F64, true>(args);
+    if (F64 && block_scene_f64(args)) step_slot<true, true>(args);
+    else step_slot<false, true>(args);
 }
 
 // Binning of the current positions only (parity views and the bond builder).

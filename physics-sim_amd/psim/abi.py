@@ -61,6 +61,16 @@ class DemStateView(C.Structure):
                 ("contact_slots", C.c_int32), ("data", FP)]
 
 
+class DemServerParams(C.Structure):
+    _fields_ = [("slot_capacity", C.c_int64), ("cell_capacity", C.c_int64), ("bond_capacity", C.c_int64),
+                ("record_capacity", C.c_int64), ("scene_capacity", C.c_int32), ("material_capacity", C.c_int32),
+                ("block_size", C.c_int32), ("record_data", C.c_int32), ("substeps", C.c_int32),
+                ("max_resident", C.c_int32), ("bypass", C.c_int32), ("contact_slots", C.c_int32)]
+
+
+DEM_REQ_QUEUED, DEM_REQ_RESIDENT, DEM_REQ_DONE = 0, 1, 2
+
+
 def fptr(a):
     if a is None:
         return None
@@ -107,6 +117,8 @@ EXPORTS = [
     "dem_upload_state", "dem_step", "dem_download_state", "dem_sync", "dem_kinetic_energy",
     "dem_broken_bond_count", "dem_counters", "dem_get_datum", "dem_debug_broad_phase",
     "dem_debug_contacts", "dem_build_bond_pairs", "dem_last_step_timing",
+    "dem_server_create", "dem_server_destroy", "dem_server_submit", "dem_server_tick", "dem_server_status",
+    "dem_server_take", "dem_server_stats",
 ]
 
 _lib = None
@@ -144,6 +156,14 @@ def load_library(path: str = LIB_PATH):
     lib.dem_debug_contacts.argtypes = [C.c_void_p, C.c_int32, IP, IP]
     lib.dem_build_bond_pairs.argtypes = [C.c_void_p, C.c_int32, C.c_float, IP, C.c_int32, IP]
     lib.dem_last_step_timing.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    lib.dem_server_create.argtypes = [C.POINTER(DemServerParams), C.POINTER(C.c_void_p)]
+    lib.dem_server_destroy.argtypes = [C.c_void_p]
+    lib.dem_server_submit.argtypes = [C.c_void_p, C.POINTER(DemParams), C.POINTER(DemStateView), C.c_int32,
+                                      C.POINTER(C.c_int64)]
+    lib.dem_server_tick.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    lib.dem_server_status.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]
+    lib.dem_server_take.argtypes = [C.c_void_p, C.c_int64, C.POINTER(DemStateView), C.POINTER(C.c_double)]
+    lib.dem_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int32]
     _lib = lib
     return lib
 

@@ -1,0 +1,118 @@
+"""Serving loop (continuous batching) on the GPU: every scene served in a mix of
+other scenes - admitted while others are mid-run, its steps split over ticks, its
+neighbours in the launches changing - reproduces its solo run bit for bit.  Needs an
+MI355X."""
+import os
+
+import numpy as np
+import pytest
+
+from psim import abi, scenes
+from psim.compute import ComputeProg
+from psim.serve import Server, footprint
+from psim.settings import Settings
+from psim.state import State
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+FIELDS = ("pos", "vel", "acc", "rot", "rot_vel", "rot_acc", "data", "contacts", "bonds")
+
+
+def _len(st, name):
+    n = st.p_count
+    return {"pos": 2 * n, "vel": 2 * n, "acc": 2 * n, "data": 7 * n, "contacts": st.max_contacts * 6 * n,
+            "bonds": 3 * st.n_bonds}.get(name, n)
+
+
+def assert_same(got: State, want: State, tag):
+    for name in FIELDS:
+        k = _len(want, name)
+        a = np.asarray(getattr(got, name))[:k].view(np.int32)
+        b = np.asarray(getattr(want, name))[:k].view(np.int32)
+        if not np.array_equal(a, b):
+            bad = np.nonzero(a != b)[0]
+            raise AssertionError("%s: %s differs at %d of %d words (first %d)" % (tag, name, len(bad), k, bad[0]))
+
+
+def mixed_scenes():
+    """Granular boxes of several sizes, frictions and packings (f32), plus the
+    reference's parallel-bond experiment state (f64 distances, bond mode 3)."""
+    out = []
+    for k, (n_side, mu, spacing) in enumerate([(16, 0.5, 1.8), (24, 0.2, 1.9), (8, 0.8, 1.7), (32, 0.5, 2.0),
+                                               (12, 0.3, 1.75)]):
+        s, st = scenes.granular_box(n_side=n_side, seed=20 + k, spacing=spacing, mu=mu)
+        out.append(("box%d" % k, s, st, 30 + 11 * k))
+    s = Settings()
+    st = State.load_file(os.path.join(GOLDEN, "saved_states", "PBEXPS.bin"), s)
+    out.append(("PBEXPS", s, st, 45))
+    return out
+
+
+def solo(s, st, n):
+    prog = ComputeProg(s, st.copy(), record_data=True)
+    prog.step(n)
+    got = prog.update_state().copy()
+    prog.close()
+    return got
+
+
+def test_batched_equals_solo_with_continuous_arrivals():
+    sc = mixed_scenes()
+    want = [solo(s, st, n) for _, s, st, n in sc]
+    srv = Server(slot_capacity=8192, cell_capacity=1 << 16, bond_capacity=64, scene_capacity=16, substeps=7)
+    tickets, outs = [], []
+    # two scenes start; the others arrive one per tick while those run
+    for i in range(len(sc)):
+        name, s, st, n = sc[i]
+        outs.append(st.copy())
+        tickets.append(srv.submit(s, outs[i], n))
+        if i >= 1:
+            srv.tick()
+    for _ in range(1000):
+        if all(srv.status(t) == abi.DEM_REQ_DONE for t in tickets):
+            break
+        srv.tick()
+    st = srv.stats()
+    assert st["completed"] == len(sc) and st["queued"] == 0 and st["resident"] == 0
+    assert st["p99_s"] >= st["p50_s"] > 0.0
+    for i, t in enumerate(tickets):
+        got, lat = srv.take(t, outs[i])
+        assert lat > 0.0
+        assert_same(got, want[i], sc[i][0])
+    srv.close()
+
+
+def test_admission_by_footprint():
+    """Pools that hold two of the scenes at a time: the others wait in the queue,
+    are admitted as regions free up, and still reproduce their solo runs."""
+    sc = []
+    for k in range(5):
+        s, st = scenes.granular_box(n_side=32, seed=40 + k, spacing=1.8)
+        sc.append((s, st, 25 + 5 * k))
+    g = ComputeProg.default_grid(sc[0][0], sc[0][1])
+    slots, cells, _ = footprint(sc[0][1], g)
+    srv = Server(slot_capacity=2 * slots, cell_capacity=2 * cells + 8, scene_capacity=8, substeps=10)
+    outs = [st.copy() for _, st, _ in sc]
+    tickets = [srv.submit(s, outs[i], n) for i, (s, st, n) in enumerate(sc)]
+    max_res = 0
+    for _ in range(1000):
+        if all(srv.status(t) == abi.DEM_REQ_DONE for t in tickets):
+            break
+        srv.tick()
+        max_res = max(max_res, sum(srv.status(t) == abi.DEM_REQ_RESIDENT for t in tickets))
+        max_res = max(max_res, int(srv.stats()["resident"]))
+    assert max_res == 2
+    for i, t in enumerate(tickets):
+        got, _ = srv.take(t, outs[i])
+        assert_same(got, solo(*sc[i]), "scene %d" % i)
+    srv.close()
+
+
+def test_oversized_scene_refused():
+    s, st = scenes.granular_box(n_side=20, seed=1)     # 400 discs: two 256-slot blocks
+    srv = Server(slot_capacity=256, cell_capacity=1 << 12, scene_capacity=2, substeps=4)
+    with pytest.raises(abi.DemError) as e:
+        srv.submit(s, st, 10)
+    assert e.value.code == abi.DEM_ECAPACITY
+    srv.close()
