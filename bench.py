@@ -58,6 +58,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-particles", type=int, default=262144)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
+    ap.add_argument("--serve-scenes", type=int, default=96)
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, scenes/s")
+    ap.add_argument("--serve-substeps", type=int, default=50)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also time a plain device copy of this many MiB (the reference kernel for the HBM "
                          "traffic counters and the achievable copy bandwidth)")
@@ -104,6 +108,55 @@ def copy_calibration(mib, reps=5):
     t = ms.value / reps
     return {"mib": mib, "bytes_read": nbytes, "bytes_written": nbytes, "ms": t, "reps": reps,
             "gbs_read_plus_write": 2 * nbytes / (t * 1e-3) / 1e9}
+
+
+def serving_bench(args):
+    """Continuous batching against plain one-scene-at-a-time serving on the same
+    stream (half granular boxes, half bonded chains; log-uniform sizes; seeded
+    per-scene friction and bond strengths).  Each mode runs twice: with Poisson
+    arrivals at --serve-rate (latency under that load) and with every scene queued
+    at t = 0 (service capacity).  "plain" is the same server admitting one scene at
+    a time, so the difference is the batching alone."""
+    from psim import scenes
+    from psim.serve import Server
+    stream = scenes.serving_stream(n_scenes=args.serve_scenes, rate=args.serve_rate)
+    slots = sum((st.p_count + 255) // 256 * 256 for _, _, st, _ in stream)
+    bonds = sum(st.n_bonds for _, _, st, _ in stream)
+    ps_total = sum(st.p_count * n for _, _, st, n in stream)
+
+    import math
+    from psim.compute import ComputeProg
+    grids = [ComputeProg.default_grid(s, st) for _, s, st, _ in stream]
+    cells = sum(g.w * g.h + 1 for g in grids)
+
+    def nearest_rank(sorted_vals, f):
+        k = min(len(sorted_vals), max(1, math.ceil(f * len(sorted_vals))))
+        return sorted_vals[k - 1]
+
+    def run(max_resident, saturated):
+        srv = Server(slot_capacity=slots, cell_capacity=cells, bond_capacity=max(bonds, 1), scene_capacity=len(stream),
+                     substeps=args.serve_substeps, max_resident=max_resident, record_data=False)
+        reqs = [(0.0 if saturated else t, s, st.copy(), n) for t, s, st, n in stream]
+        _, lat, wall = srv.run_stream(reqs)
+        stt = srv.stats()
+        srv.close()
+        lat = sorted(lat)
+        return {"scenes_per_s": len(reqs) / wall, "p50_s": nearest_rank(lat, 0.50), "p99_s": nearest_rank(lat, 0.99),
+                "wall_s": wall,
+                "particle_steps_per_s": ps_total / wall, "ticks": int(stt["ticks"])}
+
+    out = {"stream": {"scenes": len(stream), "arrival_rate": args.serve_rate, "particles_total":
+                      sum(st.p_count for _, _, st, _ in stream), "particle_steps_total": ps_total,
+                      "substeps_per_tick": args.serve_substeps,
+                      "mix": "50/50 granular boxes (f32 contacts) / bonded chains (parallel bonds, Weibull strengths); "
+                             "n log-uniform [1024, 16384], steps U[200, 1000], mu U[0.2, 0.8], strength scale U[20, 80]"}}
+    for name, mr in (("batched", 0), ("plain", 1)):
+        out[name] = {"poisson": run(mr, False), "saturated": run(mr, True)}
+        print("bench: serving %s done" % name, file=sys.stderr, flush=True)
+    out["speedup_scenes_per_s_saturated"] = (out["batched"]["saturated"]["scenes_per_s"] /
+                                             out["plain"]["saturated"]["scenes_per_s"])
+    out["p99_ratio_poisson_plain_over_batched"] = out["plain"]["poisson"]["p99_s"] / out["batched"]["poisson"]["p99_s"]
+    return out
 
 
 def cpu_baseline(args):
@@ -219,6 +272,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args)},
     }
     prog.close()
+    if not args.no_serve:
+        out["serving"] = serving_bench(args)
     if args.copy_mib > 0:
         out["config"]["copy_calibration"] = copy_calibration(args.copy_mib)
     if not args.no_cpu:

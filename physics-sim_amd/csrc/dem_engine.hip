@@ -561,15 +561,9 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
 
 // ---------------------------------------------------------------------------
 int Engine::reorder(const int2* lf_dev, const std::vector<int32_t>& sids) {
-    // all scenes binned in this launch move; the others are copied unchanged
+    // all scenes binned in this launch move into cell order (the order inside a cell is
+    // the binning's; nothing downstream depends on it); the others are copied unchanged
     hipError_t e = hipSuccess;
-    StepArgs args = make_args(lf_dev);
-    for (int32_t sid : sids) {
-        const SceneHost& sh = scenes_[sid];
-        if (e == hipSuccess)
-            e = launch_cell_sort(args, (int)sh.c0, (int)(sh.c0 + sh.params.grid.w * sh.params.grid.h + 1),
-                                 (int)sh.p0, stream_);
-    }
     ReorderArgs r{};
     r.scenes = d_scenes_;
     r.blk_scene = d_blk_scene_;

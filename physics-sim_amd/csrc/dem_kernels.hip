@@ -17,8 +17,9 @@
 //              registers + LDS across waves); zeroes the counts for the next step
 //              (stages 2b, 3: cell start = prefix).
 //   k_scatter  slot -> position START[key] + RANK in the cell-ordered slot list.
-//   k_cell_sort, k_reorder_move   every few broad phases: sort each cell's entries
-//              by particle id and move every per-particle array into that order.
+//   k_reorder_move, k_identity   every few broad phases: move every per-particle
+//              array into cell-list order (k_cell_sort orders each cell by id for the
+//              debug view of the broad phase only).
 //
 // Arithmetic follows the WGSL expression by expression (operand order, no fused
 // multiply-add: the library is built with -ffp-contract=off; sqrt and division are
@@ -829,8 +830,8 @@ __global__ void k_scatter(StepArgs args) {
     else atomicAdd(&a.STAT[3], 1ull);
 }
 
-// ---- storage re-sort ----
-// 1. each cell's entries sorted by particle id: the canonical (cell, id) order
+// ---- (cell, id) order of the cell list (the debug view of stages 1-3) ----
+// Each cell's entries sorted by particle id, one thread per cell.
 __global__ void k_cell_sort(StepArgs args, int cell_lo, int cell_hi, int p0) {
     const DevArrays& a = args.a;
     const int c = cell_lo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -852,8 +853,9 @@ __global__ void k_cell_sort(StepArgs args, int cell_lo, int cell_hi, int p0) {
     }
 }
 
-// 2. move every per-particle array to its sorted position: slot j takes the particle
-//    at position j of the scene's sorted list (the scene's cells, sentinel included,
+// ---- storage re-sort ----
+// 1. move every per-particle array to its cell-list position: slot j takes the
+//    particle at position j of the scene's list (the scene's cells, sentinel included,
 //    hold exactly its np slots, so the list is a permutation of the scene's slots).
 __global__ void k_reorder_move(ReorderArgs r) {
     const int sid = r.blk_scene[blockIdx.x];
@@ -888,7 +890,7 @@ __global__ void k_reorder_move(ReorderArgs r) {
     r.dst.INV[S.p0 + m.y] = j;
 }
 
-// 3. the sorted list becomes the identity
+// 2. the cell list becomes the identity
 __global__ void k_identity(int32_t* sidx, int64_t n) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j < n) sidx[j] = (int32_t)j;

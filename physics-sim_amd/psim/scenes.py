@@ -148,3 +148,50 @@ def bonded_chains(chains: int = 64, per_chain: int = 256, seed: int = 1, striker
     strengths[0::2] = weibull(rng, strength_k, strength_scale, nb_est)
     strengths[1::2] = weibull(rng, strength_k, strength_scale, nb_est)
     return s, st, strengths
+
+
+def chain_bonds(st: State, chains: int, per_chain: int, strengths: np.ndarray = None):
+    """Bonds between consecutive discs of every chain (the pairs State::regen_bonds
+    finds for chains of touching discs: |d| = 1.0 < 1.05 (r_a + r_b)); fills
+    st.bonds, st.contacts and st.bond_strengths."""
+    from .compute import assemble_bonds
+    k = np.arange(chains * per_chain).reshape(chains, per_chain)
+    pairs = np.stack([k[:, :-1].ravel(), k[:, 1:].ravel()], axis=1).astype(np.int32)
+    st.bonds, st.contacts = assemble_bonds(st, pairs, st.max_contacts)
+    if strengths is not None:
+        st.bond_strengths = np.ascontiguousarray(strengths[: 2 * len(pairs)], dtype=np.float32)
+    return st
+
+
+def serving_stream(n_scenes: int = 96, rate: float = 50.0, seed: int = 4, n_min: int = 1024, n_max: int = 16384,
+                   steps_min: int = 200, steps_max: int = 1000):
+    """A stream of independent scenes in the style of the serving configuration:
+    Poisson arrivals at `rate` scenes/s; half granular boxes (plain contacts, f32),
+    half bonded chains struck by a small dense disc (parallel bonds, Weibull(k = 6)
+    strengths); particle counts log-uniform in [n_min, n_max]; per-scene seeded
+    friction mu ~ U[0.2, 0.8] and bond strength scale ~ U[20, 80]; step counts
+    uniform in [steps_min, steps_max].  Scene i uses seed `seed * 100003 + i`.
+    Returns a list of (arrival offset s, settings, state, steps)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_scenes))
+    t -= t[0]
+    out = []
+    for i in range(n_scenes):
+        r = np.random.default_rng(seed * 100003 + i)
+        n = int(round(math.exp(r.uniform(math.log(n_min), math.log(n_max)))))
+        mu = float(r.uniform(0.2, 0.8))
+        steps = int(r.integers(steps_min, steps_max + 1))
+        if i % 2 == 0:
+            side = max(4, int(round(math.sqrt(n))))
+            s, st = granular_box(n_side=side, seed=int(r.integers(1 << 30)), spacing=2.0, mu=mu)
+        else:
+            per_chain = 64
+            chains = max(1, n // per_chain)
+            scale = float(r.uniform(20.0, 80.0))
+            s, st, strengths = bonded_chains(chains=chains, per_chain=per_chain, seed=int(r.integers(1 << 30)),
+                                             striker_r=1.0, striker_v=40.0, striker_density=50.0,
+                                             strength_scale=scale)
+            s.physics.friction_coefficient = f32(mu)
+            chain_bonds(st, chains, per_chain, strengths)
+        out.append((float(t[i]), s, st, steps))
+    return out
