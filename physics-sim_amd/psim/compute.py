@@ -249,7 +249,11 @@ def assemble_bonds(st: State, pairs: np.ndarray, slots: int):
     dx = (pos[j, 0] - pos[i, 0]).astype(np.float32)
     dy = (pos[j, 1] - pos[i, 1]).astype(np.float32)
     mag = np.sqrt((dx * dx + dy * dy).astype(np.float32)).astype(np.float32)
-    ang = np.arctan2(dx / mag, dy / mag).astype(np.float32)
+    # atan2 of the f32 unit vector (state.rs:306-310), evaluated in f64 and rounded once:
+    # numpy's f32 arctan2 is off by up to 2 ulp
+    nx = (dx / mag).astype(np.float32)
+    ny = (dy / mag).astype(np.float32)
+    ang = np.arctan2(nx.astype(np.float64), ny.astype(np.float64)).astype(np.float32)
     nb = len(pairs)
     bonds = np.empty(3 * nb, dtype=np.int32)
     bonds[0::3] = 1

@@ -1,0 +1,93 @@
+"""Bonded-particle path on the GPU: struck chains with per-bond Weibull strengths in
+bond modes 1-3 (intact laws, tearing, broken-bond codes, contacts that take over
+from broken bonds) and the bond builder on the cell list, against the oracle.
+Needs an MI355X."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import orc
+from psim import scenes
+from psim.compute import ComputeProg
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("pos", "vel", "acc", "rot", "rot_vel", "rot_acc")
+
+
+def _n(name, n):
+    return 2 * n if name in ("pos", "vel", "acc") else n
+
+
+def assert_bits(a, b, what):
+    a, b = np.asarray(a).view(np.int32), np.asarray(b).view(np.int32)
+    if not np.array_equal(a, b):
+        bad = np.nonzero(a != b)[0]
+        raise AssertionError("%s differs at %d of %d words (first %d)" % (what, len(bad), len(a), bad[0]))
+
+
+def chain_scene(mode, chains=8, per_chain=32, scale=50.0, seed=3):
+    """Chains of touching r = 0.5 discs bonded to their neighbours, ends pinned,
+    struck from below by a dense r = 1 disc at 40 units/s (dt 5e-5)."""
+    s, st, strengths = scenes.bonded_chains(chains=chains, per_chain=per_chain, seed=seed, striker_r=1.0,
+                                            striker_v=40.0, striker_density=50.0, strength_scale=scale)
+    s.physics.bonds = mode
+    scenes.chain_bonds(st, chains, per_chain, strengths)
+    return s, st
+
+
+@pytest.mark.parametrize("mode", [3, 2, 1])
+def test_struck_chains_match_oracle(mode):
+    s, st = chain_scene(mode)
+    n, nb = st.p_count, st.n_bonds
+    prog = ComputeProg(s, st.copy(), record_data=True)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    for k in (120, 240, 240):       # several calls: the state crosses call boundaries
+        prog.step(k)
+        eng.step(k)
+    got = prog.update_state()
+    for name in FIELDS:
+        assert_bits(getattr(got, name)[: _n(name, n)], getattr(eng.st, name)[: _n(name, n)], name)
+    assert_bits(got.data[: 7 * n], eng.st.data[: 7 * n], "data")
+    assert_bits(got.bonds[: 3 * nb], eng.st.bonds[: 3 * nb], "bond words (break codes)")
+    want = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n]
+    assert_bits(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want, "contact table")
+    broken = int((eng.st.bonds[0:3 * nb:3] < 0).sum())
+    assert prog.broken_bond_count() == broken
+    if mode == 3:
+        assert broken > 0      # the strike tears parallel bonds
+    prog.close()
+
+
+def test_bond_builder_matches_oracle():
+    """dem_build_bond_pairs (pair search on the cell list) + host assembly equal the
+    oracle's State::regen_bonds: the same bonds in the same order, the same rest
+    lengths, the same endpoint slots; the bond angle within one ulp (numpy's and
+    libm's atan2)."""
+    s, st = scenes.granular_box(n_side=48, seed=5, spacing=2.0, rmin=0.8, rmax=1.2)
+    n = st.p_count
+    max_bonds, slots = 6, 14
+    s.setup.max_bonds, s.setup.max_contacts = max_bonds, slots
+    prog = ComputeProg(s, st.copy(), bond_capacity=max_bonds * n)
+    got = prog.regen_bonds(s)
+    nb = got.n_bonds
+    bonds_o = np.zeros(3 * max_bonds * n, dtype=np.int32)
+    cont_o = np.zeros(slots * 6 * n, dtype=np.float32)
+    nb_o = orc.lib().orc_regen_bonds(n, st.pos.ctypes.data_as(orc.FP), st.radii.ctypes.data_as(orc.FP), max_bonds,
+                                     slots, bonds_o.ctypes.data_as(orc.IP), max_bonds * n,
+                                     cont_o.ctypes.data_as(orc.FP))
+    assert nb == nb_o > n          # a dense packing: more bonds than discs
+    b = got.bonds[: 3 * nb].reshape(-1, 3)
+    w = bonds_o[: 3 * nb].reshape(-1, 3)
+    assert np.array_equal(b[:, 0], w[:, 0])
+    assert np.array_equal(b[:, 2], w[:, 2])                       # rest length bits
+    ulp = np.abs(b[:, 1].astype(np.int64) - w[:, 1].astype(np.int64))
+    assert ulp.max() <= 1                                         # angle bits
+    have = got.contacts.view(np.int32).reshape(-1, slots, 6)[:n]
+    assert np.array_equal(have, cont_o.view(np.int32).reshape(-1, slots, 6)[:n])
+    # after the re-upload the device holds the bonds: one step later every bond is intact
+    prog.step(1)
+    after = prog.update_state()
+    assert prog.broken_bond_count() == 0 and np.all(after.bonds[0:3 * nb:3] == 1)
+    prog.close()
