@@ -80,12 +80,12 @@ struct DevArrays {
     Pool<float4> EXT;       // external {fx, fy, moment, 0}
     Pool<float4> DATA0;     // {Σfn, Σft, ΣM, data3}
     Pool<float4> DATA1;     // {v*x, v*y, w*, 0}
-    // contact lists, slot-major [KMAX][cap], double-buffered: every step kernel reads
-    // the current lists (CC/CP/CF) and writes the next ones (CCn/CPn/CFn)
+    // contact lists, slot-major [KMAX][cap]: the partner and force columns are
+    // double-buffered (every step kernel reads CP/CF and writes CPn/CFn); the count is
+    // part of the particle set and is rewritten only when it changes
     Pool<int32_t> CC;       // count
     Pool<int32_t> CP;       // partner id (scene-local), ascending
     Pool<float> CF;         // tangential force
-    Pool<int32_t> CCn;
     Pool<int32_t> CPn;
     Pool<float> CFn;
     int64_t cap;       // particle pool size (slot stride of CP / CF)

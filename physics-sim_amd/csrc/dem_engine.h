@@ -25,10 +25,10 @@ struct DevSet {
     float4* EXT = nullptr;
     float4* DATA0 = nullptr;
     float4* DATA1 = nullptr;
+    int32_t* CC = nullptr;   // contact count (written only when it changes)
 };
 
 struct ContactLists {
-    int32_t* CC = nullptr;
     int32_t* CP = nullptr;
     float* CF = nullptr;
 };

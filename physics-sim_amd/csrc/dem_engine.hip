@@ -46,7 +46,8 @@ Engine::~Engine() {
         hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.D[0]); hipFree(s.D[1]);
         hipFree(s.V); hipFree(s.A); hipFree(s.M); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
     }
-    for (auto& l : lists_) { hipFree(l.CC); hipFree(l.CP); hipFree(l.CF); }
+    for (auto& s : sets_) hipFree(s.CC);
+    for (auto& l : lists_) { hipFree(l.CP); hipFree(l.CF); }
     hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BL_); hipFree(BSTR_); hipFree(MAT_);
     hipFree(KEY_); hipFree(RANK_); hipFree(CNT_); hipFree(START_); hipFree(SIDX_); hipFree(INV_);
     hipFree(STAT_); hipFree(TSUM_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
@@ -93,13 +94,13 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
         e = e ? e : dalloc(&s.P[0], S); e = e ? e : dalloc(&s.P[1], S);
         e = e ? e : dalloc(&s.D[0], S); e = e ? e : dalloc(&s.D[1], S);
         e = e ? e : dalloc(&s.V, S); e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S);
-        e = e ? e : dalloc(&s.EXT, S);
+        e = e ? e : dalloc(&s.EXT, S); e = e ? e : dalloc(&s.CC, S);
         if (with_data) { e = e ? e : dalloc(&s.DATA0, S); e = e ? e : dalloc(&s.DATA1, S); }
         if ((rc = check(e, "hipMalloc particle pools"))) return rc;
     }
     for (auto& l : lists_) {
         hipError_t e = hipSuccess;
-        e = e ? e : dalloc(&l.CC, S); e = e ? e : dalloc(&l.CP, S * KMAX); e = e ? e : dalloc(&l.CF, S * KMAX);
+        e = e ? e : dalloc(&l.CP, S * KMAX); e = e ? e : dalloc(&l.CF, S * KMAX);
         if ((rc = check(e, "hipMalloc contact lists"))) return rc;
     }
     hipError_t e = hipSuccess;
@@ -124,7 +125,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = e ? e : hipMemsetAsync(STAT_, 0, sizeof(unsigned long long) * 8, stream_);
     e = e ? e : hipMemsetAsync(BOFF_, 0, sizeof(int2) * S, stream_);
     for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int2) * S, stream_);
-    for (auto& l : lists_) e = e ? e : hipMemsetAsync(l.CC, 0, sizeof(int32_t) * S, stream_);
+    for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.CC, 0, sizeof(int32_t) * S, stream_);
     std::vector<int32_t> blk(nblk, -1);
     e = e ? e : hipMemcpyAsync(d_blk_scene_, blk.data(), sizeof(int32_t) * nblk, hipMemcpyHostToDevice, stream_);
     e = e ? e : hipStreamSynchronize(stream_);
@@ -346,8 +347,8 @@ DevArrays Engine::arrays(int set) const {
     a.V = s.V; a.A = s.A; a.M = s.M; a.EXT = s.EXT; a.DATA0 = s.DATA0; a.DATA1 = s.DATA1;
     const ContactLists& lc = lists_[lcur_];
     const ContactLists& ln = lists_[lcur_ ^ 1];
-    a.CC = lc.CC; a.CP = lc.CP; a.CF = lc.CF;
-    a.CCn = ln.CC; a.CPn = ln.CP; a.CFn = ln.CF;
+    a.CC = s.CC; a.CP = lc.CP; a.CF = lc.CF;
+    a.CPn = ln.CP; a.CFn = ln.CF;
     a.cap = slot_cap_;
     a.BOFF = BOFF_; a.BR = BR_; a.BF = BF_; a.BS = BS_; a.BL = BL_; a.BSTR = BSTR_; a.MAT = MAT_;
     a.KEY = KEY_; a.RANK = RANK_; a.CNT = CNT_; a.START = START_; a.SIDX = SIDX_; a.INV = INV_;
@@ -534,7 +535,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         h2d(S.DATA1 + p0, D1.data(), sizeof(float4) * np);
     }
     const ContactLists& L = lists_[lcur_];
-    h2d(L.CC + p0, CC.data(), sizeof(int32_t) * np);
+    h2d(S.CC + p0, CC.data(), sizeof(int32_t) * np);
     for (int q = 0; q < KMAX; q++) {
         h2d(L.CP + (int64_t)q * slot_cap_ + p0, CP.data() + (size_t)q * np, sizeof(int32_t) * np);
         h2d(L.CF + (int64_t)q * slot_cap_ + p0, CF.data() + (size_t)q * np, sizeof(float) * np);
@@ -570,7 +571,6 @@ int Engine::reorder(const int2* lf_dev, const std::vector<int32_t>& sids) {
     r.lf = lf_dev;
     r.src = arrays(set_);
     r.dst = arrays(set_ ^ 1);
-    r.dst.CC = r.src.CCn;
     r.dst.CP = r.src.CPn;
     r.dst.CF = r.src.CFn;
     r.cur = cur_;
@@ -769,7 +769,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     }
     if (v.contacts) {
         const ContactLists& L = lists_[lcur_];
-        d2h(CC.data(), L.CC + p0, sizeof(int32_t) * np);
+        d2h(CC.data(), sets_[set_].CC + p0, sizeof(int32_t) * np);
         for (int q = 0; q < KMAX; q++) {
             d2h(CP.data() + (size_t)q * np, L.CP + (int64_t)q * slot_cap_ + p0, sizeof(int32_t) * np);
             d2h(CF.data() + (size_t)q * np, L.CF + (int64_t)q * slot_cap_ + p0, sizeof(float) * np);
@@ -893,7 +893,7 @@ int Engine::counters(int32_t sid, int64_t* out, int32_t n_out) {
     unsigned long long st[8] = {0}, nc = 0;
     hipError_t e = hipMemcpyAsync(st, STAT_, sizeof(st), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipMemsetAsync(d_cnt_, 0, sizeof(unsigned long long), stream_);
-    if (e == hipSuccess) e = launch_count_contacts(lists_[lcur_].CC, (int)sh.p0, sh.np, d_cnt_, stream_);
+    if (e == hipSuccess) e = launch_count_contacts(sets_[set_].CC, (int)sh.p0, sh.np, d_cnt_, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(&nc, d_cnt_, sizeof(nc), hipMemcpyDeviceToHost, stream_);
     int64_t nrec_alive = 0;
     std::vector<int4> BR(std::max(sh.nrec, 1));
@@ -976,7 +976,7 @@ int Engine::debug_contacts(int32_t sid, int32_t max_per, int32_t* counts, int32_
     const SceneHost& sh = scenes_[sid];
     const int np = sh.np;
     std::vector<int32_t> CC(np), CP((size_t)np * KMAX), INV(np);
-    hipError_t e = hipMemcpyAsync(CC.data(), lists_[lcur_].CC + sh.p0, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
+    hipError_t e = hipMemcpyAsync(CC.data(), sets_[set_].CC + sh.p0, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
     for (int q = 0; q < KMAX && e == hipSuccess; q++)
         e = hipMemcpyAsync(CP.data() + (size_t)q * np, lists_[lcur_].CP + (int64_t)q * slot_cap_ + sh.p0,
                            sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);

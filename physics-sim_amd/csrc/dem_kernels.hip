@@ -4,7 +4,7 @@
 // half-kick, drift, walls, AABB binning) and Sim (2D_Simulation.wgsl:108-478:
 // narrow phase, contact/bond laws, first half-kick).  Here one step is:
 //
-//   k_step_plain, k_step_bonds (one body, step_slot)
+//   k_step_plain, k_step_plain_f64, k_step_bonds (one body, step_slot)
 //              fused "Sim half of step k" + "LOM half of step k+1" per particle slot:
 //              contact search over the 3x3 neighbour cells of the current cell list,
 //              force laws in ascending partner-id order, store_forces, then the next
@@ -255,7 +255,6 @@ __device__ __forceinline__ void bin_slot(const DevArrays& a, int s, int key) {
 
 // Carries the contact list of slot s (c entries) over to the next-list buffers.
 __device__ __forceinline__ void copy_list(const DevArrays& a, int s, int c) {
-    a.CCn[s] = c;
     const int cap = (int)a.cap;
     for (int q = 0; q < c && q < KMAX; q++) {
         a.CPn[q * cap + s] = a.CP[q * cap + s];
@@ -532,7 +531,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         if (BONDS) {
             while (bq < b_hi) eval_record(bq++);
         }
-        a.CCn[s] = n;
+        if (n != cc_old) a.CC[s] = n;   // single-buffered: written only when it changes
 
         // 3. store_forces, 2D_Simulation.wgsl:434-478
         const float density = MA.density;
@@ -667,12 +666,14 @@ __device__ __forceinline__ bool block_scene_f64(const StepArgs& args) {
     return sid >= 0 && args.scenes[sid].use_f64 != 0;
 }
 
-// Plain scenes: 8 waves per SIMD (at most 64 VGPRs) to hide the latency of the
-// candidate loop.  Bonded scenes carry the bond-record merge and keep the default
-// register budget (8 waves would spill).
-template <bool F64>
+// Plain f32 scenes: 8 waves per SIMD (at most 64 VGPRs) to hide the latency of the
+// candidate loop.  Launches holding an f64 scene (k_step_plain_f64) and bonded scenes
+// carry more live state and keep the default register budget (8 waves would spill).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_plain(StepArgs args) {
-    if (F64 && block_scene_f64(args)) step_slot<true, false>(args);
+    step_slot<false, false>(args);
+}
+__global__ void __launch_bounds__(256) k_step_plain_f64(StepArgs args) {
+    if (block_scene_f64(args)) step_slot<true, false>(args);
     else step_slot<false, false>(args);
 }
 template <bool F64>
@@ -985,10 +986,10 @@ hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, b
     dim3 g(nblocks), b(block);
     if (f64) {
         if (bonds) hipLaunchKernelGGL((k_step_bonds<true>), g, b, 0, st, args);
-        else hipLaunchKernelGGL((k_step_plain<true>), g, b, 0, st, args);
+        else hipLaunchKernelGGL(k_step_plain_f64, g, b, 0, st, args);
     } else {
         if (bonds) hipLaunchKernelGGL((k_step_bonds<false>), g, b, 0, st, args);
-        else hipLaunchKernelGGL((k_step_plain<false>), g, b, 0, st, args);
+        else hipLaunchKernelGGL(k_step_plain, g, b, 0, st, args);
     }
     return hipGetLastError();
 }
