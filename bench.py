@@ -30,10 +30,11 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_step_plain_column_collapse_4M_traffic.md")
 
 
-def measured_traffic(args):
+def measured_traffic(args, min_cell: bool):
     """Bytes per launch of the step kernel from the committed counter summary, for the
-    workload that summary covers (the default one); None otherwise."""
-    default = (args.particles == 4194304 and args.per_row == 2048 and args.cell == 0.0 and args.block == 256
+    configuration that summary covers (4,194,304 discs, block 256, the smallest cell);
+    None otherwise."""
+    default = (args.particles == 4194304 and args.per_row == 2048 and min_cell and args.block == 256
                and args.collision_interval == 1)
     if not default or not os.path.exists(TRAFFIC_PROFILE):
         return None
@@ -58,6 +59,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-particles", type=int, default=262144)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --block and --cell as given instead of tuning them on the workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
     ap.add_argument("--serve-scenes", type=int, default=96)
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, scenes/s")
@@ -118,15 +121,18 @@ def serving_bench(args):
     at t = 0 (service capacity).  "plain" is the same server admitting one scene at
     a time, so the difference is the batching alone."""
     from psim import scenes
+    from psim.autotune import Tuner
     from psim.serve import Server
     stream = scenes.serving_stream(n_scenes=args.serve_scenes, rate=args.serve_rate)
+    # cell sizes per scene class (kind, size bucket) at the server's block size
+    tuner = Tuner(blocks=(256,), steps=10, warmup=3)
+    grids = [tuner.grid_for("box" if i % 2 == 0 else "chain", s, st, block=256)
+             for i, (_, s, st, _) in enumerate(stream)]
     slots = sum((st.p_count + 255) // 256 * 256 for _, _, st, _ in stream)
     bonds = sum(st.n_bonds for _, _, st, _ in stream)
     ps_total = sum(st.p_count * n for _, _, st, n in stream)
 
     import math
-    from psim.compute import ComputeProg
-    grids = [ComputeProg.default_grid(s, st) for _, s, st, _ in stream]
     cells = sum(g.w * g.h + 1 for g in grids)
 
     def nearest_rank(sorted_vals, f):
@@ -136,7 +142,7 @@ def serving_bench(args):
     def run(max_resident, saturated):
         srv = Server(slot_capacity=slots, cell_capacity=cells, bond_capacity=max(bonds, 1), scene_capacity=len(stream),
                      substeps=args.serve_substeps, max_resident=max_resident, record_data=False)
-        reqs = [(0.0 if saturated else t, s, st.copy(), n) for t, s, st, n in stream]
+        reqs = [(0.0 if saturated else t, s, st.copy(), n, g) for (t, s, st, n), g in zip(stream, grids)]
         _, lat, wall = srv.run_stream(reqs)
         stt = srv.stats()
         srv.close()
@@ -148,6 +154,7 @@ def serving_bench(args):
     out = {"stream": {"scenes": len(stream), "arrival_rate": args.serve_rate, "particles_total":
                       sum(st.p_count for _, _, st, _ in stream), "particle_steps_total": ps_total,
                       "substeps_per_tick": args.serve_substeps,
+                      "tuned_cell_factors": {"%s/2^%d" % k: v["cell_factor"] for k, v in tuner.cache.items()},
                       "mix": "50/50 granular boxes (f32 contacts) / bonded chains (parallel bonds, Weibull strengths); "
                              "n log-uniform [1024, 16384], steps U[200, 1000], mu U[0.2, 0.8], strength scale U[20, 80]"}}
     for name, mr in (("batched", 0), ("plain", 1)):
@@ -202,7 +209,16 @@ def main():
 
     s, st = scenes.column_collapse(n=args.particles, per_row=args.per_row)
     s.physics.collision_interval = args.collision_interval
+    tuned = None
+    if not args.no_autotune:
+        from psim.autotune import cell_size_for, tune
+        tuned = tune(s, st)
+        args.block = tuned["block"]
+        args.cell = cell_size_for(s, st, tuned["cell_factor"])
+        print("bench: autotuned block %d, cell factor %.2f" % (args.block, tuned["cell_factor"]), file=sys.stderr,
+              flush=True)
     grid = ComputeProg.default_grid(s, st, cell_size=args.cell if args.cell > 0 else None)
+    min_cell = grid.cell_size == ComputeProg.default_grid(s, st).cell_size
     t_up = time.perf_counter()
     prog = ComputeProg(s, st, grid=grid, record_data=False, reorder_interval=args.reorder, block_size=args.block)
     prog.sync()
@@ -258,7 +274,7 @@ def main():
                         "4096x2048 box, dt 1e-4" % n,
             "grid": [grid.w, grid.h], "cell_size": grid.cell_size, "reorder_interval": args.reorder,
             "collision_interval": args.collision_interval,
-            "block": args.block, "mean_contacts_per_particle": nc,
+            "block": args.block, "autotune": tuned, "mean_contacts_per_particle": nc,
             "hbm_frac_whole_step": value * b_alg / (HBM_PEAK_GBS * 1e9),
             "bytes_per_particle_step_alg": b_alg,
             "upload_ms": 1e3 * t_up, "download_ms": 1e3 * t_down,
@@ -269,7 +285,7 @@ def main():
                          for k in c2},
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args, min_cell)},
     }
     prog.close()
     if not args.no_serve:

@@ -103,9 +103,10 @@ class Server:
 
     def run_stream(self, requests, poll: float = 0.0):
         """Replays a request stream in real time: `requests` is a list of
-        (arrival offset s, settings, state, nsteps); each is submitted once its arrival
-        time has passed, ticks run back to back (new arrivals join at the next tick),
-        finished scenes are taken into their State.  Returns (tickets, latencies, wall s)."""
+        (arrival offset s, settings, state, nsteps[, grid]); each is submitted once its
+        arrival time has passed, ticks run back to back (new arrivals join at the next
+        tick), finished scenes are taken into their State.  Returns (tickets,
+        latencies, wall s)."""
         order = sorted(range(len(requests)), key=lambda i: requests[i][0])
         tickets = [None] * len(requests)
         lat = [None] * len(requests)
@@ -116,8 +117,9 @@ class Server:
             now = time.perf_counter() - t0
             while k < len(order) and requests[order[k]][0] <= now:
                 i = order[k]
-                _, s, st, n = requests[i]
-                tickets[i] = self.submit(s, st, n)
+                s, st, n = requests[i][1:4]
+                grid = requests[i][4] if len(requests[i]) > 4 else None
+                tickets[i] = self.submit(s, st, n, grid=grid)
                 pending[tickets[i]] = i
                 k += 1
             if not pending:
