@@ -195,3 +195,45 @@ def serving_stream(n_scenes: int = 96, rate: float = 50.0, seed: int = 4, n_min:
             chain_bonds(st, chains, per_chain, strengths)
         out.append((float(t[i]), s, st, steps))
     return out
+
+
+def bonded_assembly(n_side: int = 32, seed: int = 3, spacing: float = 2.0, rmin: float = 0.95, rmax: float = 1.05,
+                    pull_v: float = 0.5, dt: float = 1e-4, kn: float = 1e5, ks: float = 1e5, mu: float = 0.5):
+    """Bonded-particle assembly for a tensile test: an n_side x n_side lattice of discs
+    whose neighbours nearly touch (r ~ U[rmin, rmax] at spacing 2.0, so neighbour gaps
+    lie within +-0.1), parallel bonds (mode 3, tearing on), no gravity; the bottom row
+    and the top row are grips: their y velocity is fixed (fixity y = 1) at -pull_v and
+    +pull_v.  Bonds come from the bond builder (State::regen_bonds rule, factor 1.05)
+    with per-bond strengths drawn afterwards (Weibull, see `assembly_strengths`).
+    Returns (settings, state, bottom-grip ids, top-grip ids)."""
+    rng = np.random.default_rng(seed)
+    n = n_side * n_side
+    box = n_side * spacing * 4.0
+    s = _base_settings(n, box, box, dt, rmax, rmin, kn=2 * kn, ks=2 * ks, mu=mu)
+    s.physics.gravity = False
+    s.physics.bonds = 3
+    s.physics.bond_tearing = True
+    s.physics.bond_normal_stiffness = f32(kn)
+    s.physics.bond_shear_stiffness = f32(ks)
+    st = _new_state(n)
+    ii, jj = np.meshgrid(np.arange(n_side), np.arange(n_side), indexing="xy")
+    st.pos[0:2 * n:2] = (-n_side * spacing / 2 + spacing / 2 + ii.ravel() * spacing).astype(np.float32)
+    st.pos[1:2 * n:2] = (-n_side * spacing / 2 + spacing / 2 + jj.ravel() * spacing).astype(np.float32)
+    st.radii[:n] = rng.uniform(rmin, rmax, n).astype(np.float32)
+    bottom = np.nonzero(jj.ravel() == 0)[0]
+    top = np.nonzero(jj.ravel() == n_side - 1)[0]
+    fx = st.fixity.reshape(-1, 6)
+    for rows, v in ((bottom, -pull_v), (top, pull_v)):
+        fx[rows, 1] = 1
+        fx[rows, 4] = 1
+        st.vel[2 * rows + 1] = np.float32(v)
+    return s, st, bottom, top
+
+
+def assembly_strengths(n_bonds: int, seed: int = 3, k: float = 4.0, scale: float = 30.0):
+    """Per-bond {normal, shear} strengths, Weibull(k, scale) each."""
+    rng = np.random.default_rng(seed + 7919)
+    out = np.empty(2 * n_bonds, dtype=np.float32)
+    out[0::2] = weibull(rng, k, scale, n_bonds)
+    out[1::2] = weibull(rng, k, scale, n_bonds)
+    return out
