@@ -4,12 +4,13 @@ Workload (default): the large polydisperse granular collapse - 4,194,304 discs,
 log-normal radii (median 0.35, clip [0.175, 0.525]; the stated radii scaled by
 0.35 so the discs fit the 4096 x 2048 box), hex column of spacing 1.05, gravity,
 contact law kn_eff = 1e5, mu = 0.5, dt = 1e-4.  A "step" is one full time step of
-every disc (LOM + Sim of the reference, here: step kernel, cell scan, scatter).
+every disc (LOM + Sim of the reference, here: the step kernel, plus the neighbour-list
+rebuild chain - binning, cell scan, scatter, list build - on the steps that need it).
 
 Prints one JSON line.  `value` is particle-steps/s of the timed region (inputs
 resident in HBM, host clock around K steps bracketed by device syncs).
 `roofline` is for the dominant kernel (the fused step kernel): algorithmic bytes
-per launch = N * (68 + 16 * mean contacts per particle) over its mean duration
+per launch = N * (60 + 16 * mean contacts per particle) over its mean duration
 from HIP events on the ctx's stream; `traffic` is the kernel's measured HBM bytes per
 fused launch (counter summary under profiles/, default workload only).  `cpu_baseline` times the CPU oracle's
 restatement of the reference algorithm (AABB bucket grid, 14-slot table) on a
@@ -253,7 +254,7 @@ def main():
     value = n * args.steps / wall
     cells = grid.w * grid.h
     b_alg = 48 + 12 + 16 + 8.0 * cells / n + 16 * nc       # bytes per particle-step, whole step
-    k_bytes = n * (68 + 16 * nc)                            # dominant kernel, per launch
+    k_bytes = n * (60 + 16 * nc)                            # dominant kernel, per launch
     k_ms = tm["step_ms"] / max(tm["n_step"], 1)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     out = {
@@ -278,10 +279,11 @@ def main():
             "hbm_frac_whole_step": value * b_alg / (HBM_PEAK_GBS * 1e9),
             "bytes_per_particle_step_alg": b_alg,
             "upload_ms": 1e3 * t_up, "download_ms": 1e3 * t_down,
-            "kernel_ms": {"step": k_ms, "scan": tm["scan_ms"] / max(tm["n_scan"], 1),
-                          "scatter": tm["scatter_ms"] / max(tm["n_scatter"], 1),
-                          "reorder": tm["reorder_ms"] / max(tm["n_reorder"], 1)},
-            "counters": {k: c2[k] - c0.get(k, 0) if k in ("steps", "broad_phases", "reorders") else c2[k]
+            "kernel_ms": {"step": k_ms,
+                          "rebuild_chain_per_step": (tm["scan_ms"] + tm["scatter_ms"]) / args.steps,
+                          "reorder_per_step": tm["reorder_ms"] / args.steps},
+            "counters": {k: c2[k] - c0.get(k, 0)
+                         if k in ("steps", "broad_phases", "reorders", "list_overflow") else c2[k]
                          for k in c2},
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -77,8 +77,10 @@ typedef struct dem_material {
  * wgpu_prog.rs:698-717): w x h cells of side cell_size centred on the origin,
  * column 0 at x = -cell_size*w/2, row 0 at y = +cell_size*h/2 (rows top-down).
  * Positions outside the grid clamp to the edge cells.  cell_size must be at
- * least twice the largest radius.  cell_cap is kept for layout compatibility;
- * the counting sort has no per-cell capacity. */
+ * least twice the largest radius plus a few ulp of the grid extent; what it exceeds
+ * that by is the skin of the neighbour lists (lists are rebuilt when a particle has
+ * moved 0.45 skin).  cell_cap is kept for layout compatibility; the counting sort
+ * has no per-cell capacity. */
 typedef struct dem_grid {
     float   cell_size;
     int32_t cell_cap;
@@ -93,8 +95,8 @@ typedef struct dem_params {
     dem_grid            grid;
     int32_t use_f64;           /* reference "$ F64" variant: distances in f64 (2D_Simulation.wgsl:264-284) */
     int32_t record_data;       /* maintain data[7] per particle (Σfn, Σft, ΣM, 0, v*x, v*y, ω*) */
-    int32_t reorder_interval;  /* re-sort particle storage by cell every k broad phases; 0: only at the
-                                  first broad phase after an upload; < 0: never */
+    int32_t reorder_interval;  /* re-sort particle storage into cell order (with a neighbour-list
+                                  rebuild) every k steps; 0: only after an upload; < 0: never */
     int32_t block_size;        /* threads per block for particle kernels; 0 = 256 */
 } dem_params;
 
@@ -176,9 +178,12 @@ int dem_sync(dem_ctx* ctx);
 int dem_kinetic_energy(dem_ctx* ctx, double* out);
 /* Number of bonds whose index word is negative (broken). */
 int dem_broken_bond_count(dem_ctx* ctx, int64_t* out);
-/* Counters since upload: [0] steps, [1] broad phases, [2] contact-list overflows,
- * [3] bond-list overflows, [4] active contacts after the last step,
- * [5] active bond endpoints after the last step, [6] storage re-sorts. */
+/* Counters: [0] steps since upload, [1] neighbour-list rebuilds (broad phases),
+ * [2] contact-list overflows, [3] bond-list overflows, [4] active contacts after the
+ * last step, [5] active bond endpoints after the last step, [6] storage re-sorts,
+ * [7] scan timeouts (always 0), [8] scatter errors, [9] particles that took the
+ * stencil fallback because their neighbour list overflowed (summed over rebuilds).
+ * [1], [2], [3], [8] and [9] count over every scene of the context's pools. */
 int dem_counters(dem_ctx* ctx, int64_t* out, int32_t n_out);
 
 /* Sum over the particles flagged in `select` (length N, nonzero = selected) of

@@ -13,6 +13,7 @@
 namespace psim {
 
 constexpr int KMAX = 16;         // contacts (unbonded) per particle
+constexpr int NLMAX = 16;        // neighbour-list candidates per particle (more: stencil fallback)
 constexpr int DATA_WORDS = 7;
 
 // per-particle flag bits (word M.x bits 16..31; bits 0..15 hold the material id)
@@ -52,6 +53,8 @@ struct SceneDev {
     int32_t record;        // maintain data[7]
     int32_t has_strength;  // per-bond strengths present
     int32_t n_real;        // particles of the scene; slots [p0+n_real, p0+np) hold inactive padding
+    float skin;            // neighbour-list skin: cell side minus (2 r_max + rounding margin)
+    float trig2;           // squared displacement since the list build that forces a rebuild
 };
 
 // A device array addressed by 32-bit element index: the byte offset is formed in
@@ -105,6 +108,13 @@ struct DevArrays {
     Pool<int32_t> START;    // [ncells_total + 1]
     Pool<int32_t> SIDX;     // slots in cell order
     Pool<int32_t> INV;      // [cap] slot of (scene p0 + particle id)
+    // neighbour lists (Verlet lists with a skin), rebuilt on the device when a particle
+    // has moved more than 0.45 skin since the last build; part of the particle set
+    // (a re-sort moves them, renaming the candidate slots)
+    Pool<int32_t> NL;       // [NLMAX][cap] candidate slots within r_a + r_b + skin at the build
+    Pool<int32_t> NC;       // [cap] candidate count; -1: more than NLMAX (stencil fallback)
+    Pool<float2> XR;        // [cap] position at the build
+    Pool<int32_t> RB;       // [1] rebuild flag
     // counters
     Pool<unsigned long long> STAT;  // [0] contact overflow, [1] bond overflow, [2] scan timeouts
     // per-tile cell counts of the scan (reduce-then-scan, no inter-block waits)
@@ -128,6 +138,8 @@ struct ReorderArgs {
     const int2* lf;
     DevArrays src;  // current arrays
     DevArrays dst;  // destination arrays
+    int32_t* order;     // [cap] destination slot -> source slot
+    int32_t* newslot;   // [cap] source slot -> destination slot
     int cur;
     int with_ext, with_data;
 };

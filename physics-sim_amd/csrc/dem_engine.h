@@ -26,6 +26,9 @@ struct DevSet {
     float4* DATA0 = nullptr;
     float4* DATA1 = nullptr;
     int32_t* CC = nullptr;   // contact count (written only when it changes)
+    int32_t* NL = nullptr;   // neighbour lists [NLMAX][cap]
+    int32_t* NC = nullptr;   // list lengths
+    float2* XR = nullptr;    // positions at the list build
 };
 
 struct ContactLists {
@@ -60,6 +63,7 @@ struct SceneHost {
     int32_t coll0 = 0, coll1 = 0;
     bool del_override = false;
     bool r0_zero = false;    // particle 0 has radius 0: invocation 0 never updates coll_cont
+    float rmax = 0.0f;       // largest radius (set at upload; radii never change in a run)
     int64_t step = 0;        // Sim halves done since creation (bond-break epochs)
     // counters
     int64_t steps = 0, broad_phases = 0, reorders = 0;
@@ -114,7 +118,9 @@ private:
     DevArrays arrays(int set) const;
     StepArgs make_args(const int2* lf) const;
     int write_scene_table();
-    int reorder(const int2* lf_dev, const std::vector<int32_t>& sids);
+    int reorder(const int2* lf_dev);
+    int rebuild_chain(const int2* lf_dev, bool resort);
+    float skin_of(const SceneHost& sh) const;
     int launch_grid_blocks() const;   // blocks covering the whole slot pool
     int active_blocks() const;        // blocks covering the slots below the high-water mark
     void update_high_water();
@@ -125,6 +131,7 @@ private:
     int device_ = 0;
     hipStream_t stream_ = nullptr;
     int block_ = 256;
+    int cus_ = 256;   // compute units of the device
     bool with_data_ = true;
     int64_t slot_cap_ = 0, cell_cap_ = 0, bond_cap_ = 0, rec_cap_ = 0;
     // ends of the highest live slot and cell regions: launches and the cell scan stop there
@@ -143,6 +150,9 @@ private:
     float2* BSTR_ = nullptr;
     dem_material* MAT_ = nullptr;
     int32_t *KEY_ = nullptr, *RANK_ = nullptr, *CNT_ = nullptr, *START_ = nullptr, *SIDX_ = nullptr, *INV_ = nullptr;
+    int32_t* RB_ = nullptr;
+    int32_t *ORDER_ = nullptr, *NEWSLOT_ = nullptr;   // re-sort permutation
+    bool rb_force_ = true;        // rebuild the neighbour lists after the next drift
     unsigned long long* STAT_ = nullptr;
     int32_t* TSUM_ = nullptr;
     int32_t ntiles_ = 0;

@@ -11,11 +11,16 @@ namespace psim {
 hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, hipStream_t st);
 hipError_t launch_bin(const StepArgs& args, int nblocks, int block, hipStream_t st);
 int scan_tiles(int n);
-hipError_t launch_scan(const DevArrays& a, int ncells, hipStream_t st);
+hipError_t launch_scan(const DevArrays& a, int ncells, const int32_t* gate, hipStream_t st);
+hipError_t launch_rb_bin(const StepArgs& args, int nlb, int grid, int block, hipStream_t st);
+hipError_t launch_rb_scatter(const StepArgs& args, int nlb, int grid, int block, hipStream_t st);
+hipError_t launch_build_nl(const StepArgs& args, int nlb, int grid, int block, hipStream_t st);
+hipError_t launch_rb_store(int32_t* rb, int v, hipStream_t st);
+
 hipError_t launch_scatter(const StepArgs& args, int nblocks, int block, hipStream_t st);
 hipError_t launch_cell_sort(const StepArgs& args, int cell_lo, int cell_hi, int p0, hipStream_t st);
 hipError_t launch_reorder_move(const ReorderArgs& r, int nblocks, int block, hipStream_t st);
-hipError_t launch_identity(int32_t* sidx, int64_t n, hipStream_t st);
+hipError_t launch_partition(const ReorderArgs& r, int nblocks, int block, hipStream_t st);
 hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, int np, double* out, hipStream_t st);
 hipError_t launch_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out, hipStream_t st);
 hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
@@ -46,10 +51,11 @@ Engine::~Engine() {
         hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.D[0]); hipFree(s.D[1]);
         hipFree(s.V); hipFree(s.A); hipFree(s.M); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
     }
-    for (auto& s : sets_) hipFree(s.CC);
+    for (auto& s : sets_) { hipFree(s.CC); hipFree(s.NL); hipFree(s.NC); hipFree(s.XR); }
     for (auto& l : lists_) { hipFree(l.CP); hipFree(l.CF); }
     hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BL_); hipFree(BSTR_); hipFree(MAT_);
     hipFree(KEY_); hipFree(RANK_); hipFree(CNT_); hipFree(START_); hipFree(SIDX_); hipFree(INV_);
+    hipFree(RB_); hipFree(ORDER_); hipFree(NEWSLOT_);
     hipFree(STAT_); hipFree(TSUM_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
     hipFree(d_ke_); hipFree(d_cnt_);
     for (auto e : ev_pool_) hipEventDestroy(e);
@@ -80,13 +86,15 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     // kernels address every pool with 32-bit byte offsets: each must stay below 4 GiB
     // (the contact lists, KMAX words per slot, bound the particle count at 2^26)
     const int64_t lim = (int64_t)1 << 32;
-    if (slot_cap_ * KMAX * 4 >= lim || slot_cap_ * 16 >= lim || cell_cap_ * 4 + 8 >= lim ||
+    if (slot_cap_ * KMAX * 4 >= lim || slot_cap_ * NLMAX * 4 >= lim || slot_cap_ * 16 >= lim || cell_cap_ * 4 + 8 >= lim ||
         rec_cap_ * 16 >= lim || bond_cap_ * 8 >= lim) {
         set_error("pool too large for 32-bit offsets (at most 2^26 - 1 particle slots per context)");
         return DEM_EINVAL;
     }
     int rc;
     if ((rc = check(hipGetDevice(&device_), "hipGetDevice"))) return rc;
+    if ((rc = check(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device_), "CU count")))
+        return rc;
     if ((rc = check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate"))) return rc;
     const int64_t S = slot_cap_;
     for (auto& s : sets_) {
@@ -95,6 +103,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
         e = e ? e : dalloc(&s.D[0], S); e = e ? e : dalloc(&s.D[1], S);
         e = e ? e : dalloc(&s.V, S); e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S);
         e = e ? e : dalloc(&s.EXT, S); e = e ? e : dalloc(&s.CC, S);
+        e = e ? e : dalloc(&s.NL, S * NLMAX); e = e ? e : dalloc(&s.NC, S); e = e ? e : dalloc(&s.XR, S);
         if (with_data) { e = e ? e : dalloc(&s.DATA0, S); e = e ? e : dalloc(&s.DATA1, S); }
         if ((rc = check(e, "hipMalloc particle pools"))) return rc;
     }
@@ -109,6 +118,8 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = e ? e : dalloc(&MAT_, mat_cap_);
     e = e ? e : dalloc(&KEY_, S); e = e ? e : dalloc(&RANK_, S); e = e ? e : dalloc(&CNT_, cell_cap_ + 1);
     e = e ? e : dalloc(&START_, cell_cap_ + 2); e = e ? e : dalloc(&SIDX_, S); e = e ? e : dalloc(&INV_, S);
+    e = e ? e : dalloc(&ORDER_, S); e = e ? e : dalloc(&NEWSLOT_, S);
+    e = e ? e : dalloc(&RB_, 2);
     e = e ? e : dalloc(&STAT_, 8);
     ntiles_ = scan_tiles((int)cell_cap_);
     e = e ? e : dalloc(&TSUM_, ntiles_ + 1);
@@ -123,6 +134,14 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = hipMemsetAsync(CNT_, 0, sizeof(int32_t) * (cell_cap_ + 1), stream_);
     e = e ? e : hipMemsetAsync(START_, 0, sizeof(int32_t) * (cell_cap_ + 2), stream_);
     e = e ? e : hipMemsetAsync(STAT_, 0, sizeof(unsigned long long) * 8, stream_);
+    // the binning accumulates tile totals; the slot lists start as valid slot indices
+    // (every index read from them stays inside the pools even before the first build)
+    e = e ? e : hipMemsetAsync(TSUM_, 0, sizeof(int32_t) * (ntiles_ + 1), stream_);
+    e = e ? e : hipMemsetAsync(SIDX_, 0, sizeof(int32_t) * S, stream_);
+    e = e ? e : hipMemsetAsync(ORDER_, 0, sizeof(int32_t) * S, stream_);
+    e = e ? e : hipMemsetAsync(NEWSLOT_, 0, sizeof(int32_t) * S, stream_);
+    e = e ? e : hipMemsetAsync(RB_, 0, 2 * sizeof(int32_t), stream_);
+    for (auto& st : sets_) e = e ? e : hipMemsetAsync(st.NC, 0, sizeof(int32_t) * S, stream_);
     e = e ? e : hipMemsetAsync(BOFF_, 0, sizeof(int2) * S, stream_);
     for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int2) * S, stream_);
     for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.CC, 0, sizeof(int32_t) * S, stream_);
@@ -280,9 +299,15 @@ int Engine::set_params(int32_t sid, const dem_params& p) {
     SceneHost& sh = scenes_[sid];
     if (p.grid.w * p.grid.h + 1 > sh.cell_cap) { set_error("grid larger than the scene's cell reservation"); return DEM_ECAPACITY; }
     if (p.n_materials > sh.mat_cap) { set_error("too many materials for the scene's reservation"); return DEM_ECAPACITY; }
-    if (p.use_f64 != sh.params.use_f64 || p.record_data != sh.params.record_data) {
-        // fine: kernels read both per launch
+    if (sh.n > 0) {
+        const float cs = p.grid.cell_size;
+        const float ext = cs * (float)std::max(p.grid.w, p.grid.h);
+        if (cs < 2.0f * sh.rmax + 4.0f * (std::nextafter(ext, INFINITY) - ext)) {
+            set_error("grid cell_size is below 2*max radius + rounding margin");
+            return DEM_EINVAL;
+        }
     }
+    rb_force_ = true;   // the grid (and with it the skin) may have changed
     const int32_t n_keep = sh.params.u.particles;
     sh.params = p;
     sh.params.materials = nullptr;
@@ -294,6 +319,16 @@ int Engine::set_params(int32_t sid, const dem_params& p) {
     if ((rc = check(e, "upload materials"))) return rc;
     table_dirty_ = true;
     return DEM_OK;
+}
+
+// Neighbour-list skin of a scene: what the cell side leaves above 2 r_max plus the
+// rounding margin of the cell index (4 ulp of the grid extent).  Zero skin is correct
+// (the lists are rebuilt after every drift), only slower.
+float Engine::skin_of(const SceneHost& sh) const {
+    const float cs = sh.params.grid.cell_size;
+    const float ext = cs * (float)std::max(sh.params.grid.w, sh.params.grid.h);
+    const float need = 2.0f * sh.rmax + 4.0f * (std::nextafter(ext, INFINITY) - ext);
+    return std::max(cs - need, 0.0f);
 }
 
 SceneDev Engine::scene_dev(const SceneHost& sh) const {
@@ -319,6 +354,8 @@ SceneDev Engine::scene_dev(const SceneHost& sh) const {
     d.record = (sh.params.record_data && with_data_) ? 1 : 0;
     d.has_strength = sh.has_strength ? 1 : 0;
     d.n_real = sh.n;
+    d.skin = skin_of(sh);
+    d.trig2 = (0.45f * d.skin) * (0.45f * d.skin);
     return d;
 }
 
@@ -352,6 +389,7 @@ DevArrays Engine::arrays(int set) const {
     a.cap = slot_cap_;
     a.BOFF = BOFF_; a.BR = BR_; a.BF = BF_; a.BS = BS_; a.BL = BL_; a.BSTR = BSTR_; a.MAT = MAT_;
     a.KEY = KEY_; a.RANK = RANK_; a.CNT = CNT_; a.START = START_; a.SIDX = SIDX_; a.INV = INV_;
+    a.NL = s.NL; a.NC = s.NC; a.XR = s.XR; a.RB = RB_;
     a.STAT = STAT_; a.TSUM = TSUM_; a.ntiles = ntiles_;
     return a;
 }
@@ -442,6 +480,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         // (x - base) / cell_size, a few ulp of the grid extent
         float rmax = 0.0f;
         for (int32_t i = 0; i < n; i++) rmax = std::max(rmax, v.radii[i]);
+        sh.rmax = rmax;
         const float cs = sh.params.grid.cell_size;
         const float ext = cs * (float)std::max(sh.params.grid.w, sh.params.grid.h);
         const float need = 2.0f * rmax + 4.0f * (std::nextafter(ext, INFINITY) - ext);
@@ -555,15 +594,18 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     int rc = check(e, "upload");
     if (rc) return rc;
-    sh.since_reorder = 1 << 30;  // re-sort storage at the first broad phase
+    sh.since_reorder = 1 << 30;  // re-sort storage at the first rebuild
+    rb_force_ = true;            // neighbour lists for the new state
     table_dirty_ = true;
     return DEM_OK;
 }
 
 // ---------------------------------------------------------------------------
-int Engine::reorder(const int2* lf_dev, const std::vector<int32_t>& sids) {
-    // all scenes binned in this launch move into cell order (the order inside a cell is
-    // the binning's; nothing downstream depends on it); the others are copied unchanged
+int Engine::reorder(const int2* lf_dev) {
+    // Every live scene, binned by the rebuild this re-sort belongs to and with its
+    // neighbour lists just built, moves into cell order, each block of slots then
+    // partitioned stably into particles with list candidates first (the order inside
+    // a cell is the binning's; results depend on neither).
     hipError_t e = hipSuccess;
     ReorderArgs r{};
     r.scenes = d_scenes_;
@@ -573,20 +615,23 @@ int Engine::reorder(const int2* lf_dev, const std::vector<int32_t>& sids) {
     r.dst = arrays(set_ ^ 1);
     r.dst.CP = r.src.CPn;
     r.dst.CF = r.src.CFn;
+    r.order = ORDER_;
+    r.newslot = NEWSLOT_;
     r.cur = cur_;
     r.with_ext = 0;
     for (const SceneHost& sh : scenes_)
         if (sh.live && sh.has_ext) r.with_ext = 1;  // EXT is read only where PF_EXT is set
     r.with_data = with_data_ ? 1 : 0;
+    if (e == hipSuccess) e = launch_partition(r, active_blocks(), block_, stream_);
     if (e == hipSuccess) e = launch_reorder_move(r, active_blocks(), block_, stream_);
-    if (e == hipSuccess) e = launch_identity(SIDX_, hw_slots_, stream_);
     int rc = check(e, "reorder");
     if (rc) return rc;
     set_ ^= 1;
     lcur_ ^= 1;
-    for (int32_t sid : sids) {
-        scenes_[sid].since_reorder = 0;
-        scenes_[sid].reorders++;
+    for (SceneHost& sh : scenes_) {
+        if (!sh.live) continue;
+        sh.since_reorder = 0;
+        sh.reorders++;
     }
     return DEM_OK;
 }
@@ -677,41 +722,54 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (timing_) { t0 = ev(); t1 = ev(); hipEventRecord(t0, stream_); }
 
+    // the neighbour-list rebuild kernels stride over the slot blocks with one round of
+    // workgroups (32 waves per CU), so that a launch whose rebuild flag is clear is cheap
+    const int rb_grid = std::max(1, std::min(nblk, cus_ * std::max(1, 2048 / block_)));
     for (int64_t k = 0; k < nlaunch; k++) {
         const int2* lfk = d_lf_ + k * scene_cap_;
-        bool binned = false, lom = false;
-        std::vector<int32_t> active;
-        for (size_t i = 0; i < sids.size(); i++) {
-            const int2 w = lf[(size_t)k * scene_cap_ + sids[i]];
-            if (w.x & SF_LOM) lom = true;
-            if (w.x & SF_BIN) { binned = true; active.push_back(sids[i]); }
-        }
+        bool lom = false;
+        for (size_t i = 0; i < sids.size(); i++)
+            if (lf[(size_t)k * scene_cap_ + sids[i]].x & SF_LOM) lom = true;
         StepArgs args = make_args(lfk);
         args.flip = lom ? 1 : 0;
         e = timed(0, [&]() { return launch_step(args, nblk, block_, any_f64, any_bonds, stream_); });
         if ((rc = check(e, "step kernel"))) return rc;
         lcur_ ^= 1;  // the step kernel wrote every live slot's next contact list
         // some LOM half wrote the next positions; every other slot carried its own over
-        if (lom) cur_ ^= 1;
-        if (binned) {
-            e = timed(1, [&]() { return launch_scan(arrays(set_), (int)hw_cells_, stream_); });
-            if ((rc = check(e, "scan"))) return rc;
-            e = timed(2, [&]() { return launch_scatter(make_args(lfk), nblk, block_, stream_); });
-            if ((rc = check(e, "scatter"))) return rc;
-            // storage re-sort: every binned scene moves when one of them is due
-            bool due = false;
-            for (int32_t sid : active) {
-                SceneHost& sh = scenes_[sid];
-                sh.broad_phases++;
-                const int32_t R = sh.params.reorder_interval;
-                if (R > 0 ? sh.since_reorder >= R : (R == 0 && sh.since_reorder >= (1 << 30))) due = true;
-                sh.since_reorder++;
-            }
-            if (due) {
-                e = timed(3, [&]() { return reorder(lfk, active) == DEM_OK ? hipSuccess : hipErrorUnknown; });
-                if (e != hipSuccess) return DEM_EDEVICE;
-            }
+        if (!lom) continue;
+        cur_ ^= 1;
+        // Neighbour-list rebuild after every drift.  The kernels run only when the
+        // rebuild flag is set: by the step kernel (a particle moved more than 0.45
+        // skin since the last build) or here (new state, new parameters, re-sort due).
+        // A due re-sort moves every live scene into the order of the fresh cell list.
+        bool due = false;
+        for (size_t i = 0; i < sids.size(); i++) {
+            if (!(lf[(size_t)k * scene_cap_ + sids[i]].x & SF_LOM)) continue;
+            SceneHost& sh = scenes_[sids[i]];
+            const int32_t R = sh.params.reorder_interval;
+            if (R > 0 ? sh.since_reorder >= R : (R == 0 && sh.since_reorder >= (1 << 30))) due = true;
+            sh.since_reorder++;
         }
+        const bool force = rb_force_ || due;
+        StepArgs ra = make_args(lfk);
+        int32_t* rb = RB_ + cur_;   // this drift's flag (the next step kernel clears it)
+        e = timed(2, [&]() {
+            hipError_t r = force ? launch_rb_store(rb, 1, stream_) : hipSuccess;
+            return r == hipSuccess ? launch_rb_bin(ra, nblk, rb_grid, block_, stream_) : r;
+        });
+        if ((rc = check(e, "rebuild: bin"))) return rc;
+        e = timed(1, [&]() { return launch_scan(arrays(set_), (int)hw_cells_, rb, stream_); });
+        if ((rc = check(e, "rebuild: scan"))) return rc;
+        e = timed(2, [&]() { return launch_rb_scatter(ra, nblk, rb_grid, block_, stream_); });
+        if ((rc = check(e, "rebuild: scatter"))) return rc;
+        e = timed(2, [&]() { return launch_build_nl(ra, nblk, rb_grid, block_, stream_); });
+        if ((rc = check(e, "rebuild: lists"))) return rc;
+        if (due) {
+            // the lists move with their particles
+            e = timed(3, [&]() { return reorder(lfk) == DEM_OK ? hipSuccess : hipErrorUnknown; });
+            if (e != hipSuccess) return DEM_EDEVICE;
+        }
+        rb_force_ = false;
     }
     if (timing_) {
         hipEventRecord(t1, stream_);
@@ -902,9 +960,9 @@ int Engine::counters(int32_t sid, int64_t* out, int32_t n_out) {
     int rc = check(e, "counters");
     if (rc) return rc;
     for (int32_t q = 0; q < sh.nrec; q++) nrec_alive += BR[q].w != 0;
-    const int64_t vals[9] = {sh.steps, sh.broad_phases, (int64_t)st[0], (int64_t)st[1], (int64_t)nc, nrec_alive,
-                             sh.reorders, (int64_t)st[2], (int64_t)st[3]};
-    for (int32_t i = 0; i < n_out && i < 9; i++) out[i] = vals[i];
+    const int64_t vals[10] = {sh.steps, (int64_t)st[4], (int64_t)st[0], (int64_t)st[1], (int64_t)nc, nrec_alive,
+                              sh.reorders, (int64_t)st[2], (int64_t)st[3], (int64_t)st[5]};
+    for (int32_t i = 0; i < n_out && i < 10; i++) out[i] = vals[i];
     return DEM_OK;
 }
 
@@ -942,13 +1000,16 @@ int Engine::debug_broad_phase(int32_t sid, int32_t* keys, int32_t* sorted_ids, i
     int rc = write_scene_table();
     if (rc) return rc;
     const SceneHost& sh = scenes_[sid];
+    // this re-bins one scene over the whole cell scan: the other scenes' cell lists
+    // (the neighbour-list fallback reads them) are rebuilt after the next drift
+    rb_force_ = true;
     std::vector<int2> lf(scene_cap_, make_int2(0, 0));
     lf[sid] = make_int2((int)(SF_ACTIVE | SF_BIN), 0);
     hipError_t e = hipMemcpyAsync(d_lf_, lf.data(), sizeof(int2) * scene_cap_, hipMemcpyHostToDevice, stream_);
     StepArgs args = make_args(d_lf_);
     const int ncl = sh.params.grid.w * sh.params.grid.h;
     if (e == hipSuccess) e = launch_bin(args, active_blocks(), block_, stream_);
-    if (e == hipSuccess) e = launch_scan(arrays(set_), (int)hw_cells_, stream_);
+    if (e == hipSuccess) e = launch_scan(arrays(set_), (int)hw_cells_, nullptr, stream_);
     if (e == hipSuccess) e = launch_scatter(args, active_blocks(), block_, stream_);
     if (e == hipSuccess) e = launch_cell_sort(args, (int)sh.c0, (int)(sh.c0 + ncl + 1), (int)sh.p0, stream_);
     std::vector<int32_t> KEY(sh.np), SIDX(sh.np), START(ncl + 2);
@@ -1003,9 +1064,7 @@ int Engine::bond_pairs(int32_t sid, int32_t max_bonds, float factor, int32_t* pa
     const SceneHost& sh = scenes_[sid];
     const int n = sh.n;
     // bin the current positions (also refreshes the cell lists)
-    int32_t dummy = 0;
     int rc = debug_broad_phase(sid, nullptr, nullptr, nullptr);
-    (void)dummy;
     if (rc) return rc;
     std::vector<float4> P(sh.np);
     hipError_t e = hipMemcpyAsync(P.data(), sets_[set_].P[cur_] + sh.p0, sizeof(float4) * sh.np, hipMemcpyDeviceToHost, stream_);

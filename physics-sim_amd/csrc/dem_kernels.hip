@@ -6,20 +6,25 @@
 //
 //   k_step_plain, k_step_plain_f64, k_step_bonds (one body, step_slot)
 //              fused "Sim half of step k" + "LOM half of step k+1" per particle slot:
-//              contact search over the 3x3 neighbour cells of the current cell list,
-//              force laws in ascending partner-id order, store_forces, then the next
-//              drift and its cell key with a per-cell counting atomic, one atomic per
-//              run of equal keys in a wave (stages 1, 2a, 4, 5).  The first launch of
-//              a dem_step call runs only the LOM half and the last one only the Sim
-//              half, so the state between calls is the reference's post-Sim state.
-//   k_tile_sums, k_scan_tiles   exclusive prefix sum of the per-cell counts
-//              (reduce-then-scan over tiles of 4096 cells, wave64 prefix in
-//              registers + LDS across waves); zeroes the counts for the next step
-//              (stages 2b, 3: cell start = prefix).
-//   k_scatter  slot -> position START[key] + RANK in the cell-ordered slot list.
-//   k_reorder_move, k_identity   every few broad phases: move every per-particle
-//              array into cell-list order (k_cell_sort orders each cell by id for the
-//              debug view of the broad phase only).
+//              contact search over the particle's neighbour list (the slots within
+//              r_a + r_b + skin at the last rebuild; the 3x3 stencil of that
+//              rebuild's cell list when a list overflows), force laws in ascending
+//              partner-id order, store_forces, then the next drift, which raises the
+//              rebuild flag when the particle has moved more than 0.45 skin since the
+//              rebuild (stages 2a, 4, 5).  The first launch of a dem_step call runs
+//              only the LOM half and the last one only the Sim half, so the state
+//              between calls is the reference's post-Sim state.
+//   neighbour-list rebuild, after every drift, each kernel a no-op unless the flag
+//   is set: k_rb_bin (cell key + rank by a per-cell counting atomic, one atomic per
+//              run of equal keys in a wave, tile totals likewise), k_scan_tiles
+//              (exclusive prefix of the cell counts over tiles of 4096 cells, wave64
+//              prefix in registers + LDS across waves; zeroes the counts), k_rb_scatter
+//              (slot -> position START[key] + RANK in the cell-ordered slot list) and
+//              k_build_nl (the lists) - stages 1, 2b, 3 of the reference's broad phase.
+//   k_partition, k_reorder_move   every few rebuilds: move every per-particle
+//              array, the neighbour lists included, into cell-list order, each block
+//              partitioned into particles with list candidates first (k_cell_sort
+//              orders each cell by id for the debug view of the broad phase only).
 //
 // Arithmetic follows the WGSL expression by expression (operand order, no fused
 // multiply-add: the library is built with -ffp-contract=off; sqrt and division are
@@ -28,6 +33,8 @@
 // partner id).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 #include "dem_device.h"
 
@@ -228,6 +235,14 @@ __device__ __forceinline__ int xcd_block(int bid, int nb) {
 }
 
 
+// Cell scan tiles: the exclusive prefix of the cell counts is taken over tiles of
+// SCAN_TILE cells, whose totals the binning accumulates in TSUM.
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__host__ __device__ inline int tile_count(int n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
 // Counting-sort binning: KEY[s] = key and RANK[s] = a unique rank in [0, count)
 // among the slots of that cell.  Lanes of the wave that call together and hold
 // the same key as their lower neighbour form a run; the run's lowest lane does one
@@ -251,6 +266,22 @@ __device__ __forceinline__ void bin_slot(const DevArrays& a, int s, int key) {
     base = __shfl(base, h, 64);
     a.KEY[s] = key;
     a.RANK[s] = base + __popcll(run & ((1ull << lane) - 1ull));  // run lanes below this one
+    // the same aggregation over scan tiles: one atomic per run of lanes in one tile
+    const int tile = key / SCAN_TILE;
+    const int tbelow = __shfl_up(tile, 1, 64);
+    const bool thead = lane == 0 || !((live >> (lane - 1)) & 1ull) || tbelow != tile;
+    const uint64_t theads = __ballot(thead);
+    const uint64_t tabove = theads & ~upto;
+    const int tend = tabove ? __ffsll((long long)tabove) - 1 : 64;
+    const int th = 63 - __clzll((long long)(theads & upto));
+    const uint64_t trun = (tend == 64 ? ~0ull : ((1ull << tend) - 1ull)) & ~((1ull << th) - 1ull) & live;
+    if (thead) atomicAdd(&a.TSUM[tile], __popcll(trun));
+}
+
+// Clears the tile totals for the next binning (after the scan has read them).
+__device__ __forceinline__ void clear_tile_sums(const DevArrays& a) {
+    const int stride = (int)(gridDim.x * blockDim.x);
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < a.ntiles; i += stride) a.TSUM[i] = 0;
 }
 
 // Carries the contact list of slot s (c entries) over to the next-list buffers.
@@ -361,12 +392,34 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const int mat = (int)(pf & PF_MAT_MASK);
     const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
 
-    // ---- narrow phase: every overlapping partner in the 3x3 stencil ----
+    // ---- narrow phase: every overlapping partner ----
     int n = 0;
     const bool narrow = (fl & SF_SIM) && (fl & SF_NARROW);
-    if (narrow && act) {
+    auto test = [&](int j, const float4& q) {
+        if (j != s && overlaps(q, p)) {
+            const int oj = a.M[j].y;
+            if (!bonded_partner<BONDS>(a, b_lo, b_hi, oj)) list_insert(a, s, oj, n);
+        }
+    };
+    const int nc = (narrow && act) ? a.NC[s] : 0;
+    if (narrow && act && nc >= 0) {
+        // neighbour list: every particle within r_a + r_b + skin at the last build; none
+        // outside it can touch s while no particle has moved more than 0.45 skin since
+        for (int k = 0; k < nc; k += 2) {
+            const int j0 = a.NL[k * cap + s];
+            const int j1 = k + 1 < nc ? a.NL[(k + 1) * cap + s] : -1;
+            const float4 q0 = a.P[cur][j0];
+            const float4 q1 = j1 >= 0 ? a.P[cur][j1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            test(j0, q0);
+            if (j1 >= 0) test(j1, q1);
+        }
+    } else if (narrow && act) {
+        // more candidates than a list holds: the 3x3 stencil around the particle's cell
+        // at the last build, over the last cell list (cells are wider than
+        // 2 r_max + skin, so it holds every particle that can touch s until the rebuild)
+        const float2 xr0 = a.XR[s];
         int cx, cy;
-        cell_of(S, p.x, p.y, cx, cy);
+        cell_of(S, xr0.x, xr0.y, cx, cy);
         const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
         const int pos0 = S.p0 - a.START[S.c0];  // cell-list position -> slot-list index
         const int rlo = max(cy - 1, 0), rhi = min(cy + 1, S.h - 1);
@@ -383,14 +436,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             }
         }
         // the three stencil rows advance together, two candidates each per round: the
-        // slot-id loads of every row are in flight before the first position load, so
-        // a round costs two dependent memory latencies whatever the row count
-        auto test = [&](int j, const float4& q) {
-            if (j != s && overlaps(q, p)) {
-                const int oj = a.M[j].y;
-                if (!bonded_partner<BONDS>(a, b_lo, b_hi, oj)) list_insert(a, s, oj, n);
-            }
-        };
+        // slot-id loads of every row are in flight before the first position load
         int e[3] = {e0[0], e0[1], e0[2]};
         while (e[0] < e1[0] || e[1] < e1[1] || e[2] < e1[2]) {
             int j[6];
@@ -421,7 +467,6 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             a.P[nxt][s] = p;
             a.D[nxt][s] = d;
         }
-        if ((fl & SF_LOM) && (fl & SF_BIN)) bin_slot(a, s, S.c0 + S.w * S.h);
         return;
     }
 
@@ -652,10 +697,13 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     a.P[nxt][s] = make_float4(px, py, rr, dr);
     a.D[nxt][s] = make_float2(dpx, dpy);
     a.V[s] = make_float4(ivx, ivy, iw, rot);
-    if (fl & SF_BIN) {
-        int cx, cy;
-        bin_slot(a, s, S.c0 + cell_of(S, px, py, cx, cy));
-    }
+    // moved more than 0.45 skin since the neighbour lists were built: rebuild them after
+    // this launch (every writer stores 1)
+    const float2 xr = a.XR[s];
+    const float mx = px - xr.x, my = py - xr.y;
+    const float mxx = mx * mx;
+    const float myy = my * my;
+    if (mxx + myy > S.trig2) a.RB[nxt] = 1;
 }
 
 // Whether this block's scene measures distances in f64.  A launch that holds any
@@ -704,13 +752,10 @@ __global__ void k_bin(StepArgs args) {
 }
 
 // ---- cell scan: exclusive prefix of cnt[0..n) into start[0..n], zeroing cnt ----
-// Reduce-then-scan: k_tile_sums writes one count per tile of SCAN_TILE cells;
-// k_scan_tiles adds the sums of the tiles before its own (read from L2) and scans
+// Reduce-then-scan: the binning has added every cell count to its tile's total;
+// k_scan_tiles adds the totals of the tiles before its own (read from L2) and scans
 // its tile.  No block waits on another.
 
-constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 16;
-constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
 __device__ __forceinline__ int wave_incl_scan(int x) {
     const int lane = threadIdx.x & 63;
@@ -742,30 +787,12 @@ __device__ __forceinline__ void load_tile(const int32_t* cnt, int64_t base, int 
     }
 }
 
-__global__ void __launch_bounds__(SCAN_THREADS) k_tile_sums(const int32_t* cnt, int n, int32_t* tsum) {
-    __shared__ int s_wave[SCAN_THREADS / 64];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-    int v[SCAN_ITEMS];
-    load_tile(cnt, base, n, v);
-    int sum = 0;
-#pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) sum += v[k];
-    sum = wave_sum(sum);
-    if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int tot = 0;
-#pragma unroll
-        for (int q = 0; q < SCAN_THREADS / 64; q++) tot += s_wave[q];
-        tsum[blockIdx.x] = tot;
-    }
-}
-
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(int32_t* cnt, int32_t* start, int n,
-                                                             const int32_t* tsum) {
+// Exclusive scan of tile `tile` (of nt) into start, zeroing its counts; the last tile
+// also writes start[n] = total.
+__device__ __forceinline__ void scan_tile_body(int32_t* cnt, int32_t* start, int n, const int32_t* tsum, int tile,
+                                               int nt) {
     __shared__ int s_wave[SCAN_THREADS / 64];
     __shared__ int s_pre[SCAN_THREADS / 64];
-    const int tile = blockIdx.x;
     // prefix of the tiles before this one
     int pre = 0;
     for (int q = threadIdx.x; q < tile; q += SCAN_THREADS) pre += tsum[q];
@@ -813,12 +840,20 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(int32_t* cnt, int32
             run += v[k];
         }
     }
-    if (tile == (int)gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) start[n] = tile_pre + block_total;
+    if (tile == nt - 1 && threadIdx.x == SCAN_THREADS - 1) start[n] = tile_pre + block_total;
+    __syncthreads();   // s_wave / s_pre are reused by the block's next tile
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(int32_t* cnt, int32_t* start, int n,
+                                                             const int32_t* tsum, const int32_t* gate) {
+    if (gate && gate[0] == 0) return;
+    scan_tile_body(cnt, start, n, tsum, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ---- k_scatter: cell-ordered slot list (positions relative to the scene) ----
 __global__ void k_scatter(StepArgs args) {
     const DevArrays& a = args.a;
+    clear_tile_sums(a);
     const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int sid = args.blk_scene[blockIdx.x];
     if (sid < 0) return;
@@ -829,6 +864,135 @@ __global__ void k_scatter(StepArgs args) {
     const int pos = a.START[a.KEY[s]] - a.START[S.c0] + S.p0 + a.RANK[s];
     if (pos >= S.p0 && pos < S.p0 + S.np) a.SIDX[pos] = s;
     else atomicAdd(&a.STAT[3], 1ull);
+}
+
+// ---- neighbour-list rebuild ----
+// Every kernel returns at once unless RB[cur] is set.  The grid strides over logical
+// blocks of blockDim.x slots (the scene table's blocks), so a launch with the flag
+// clear costs a few workgroups.  The rebuild bins every live scene.
+
+__global__ void k_rb_store(int32_t* rb, int v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) rb[0] = v;
+}
+
+// The per-slot stages of a rebuild; lbs is the scene table's block of slots.
+__device__ __forceinline__ void rb_bin_slot(const StepArgs& args, int s, int lbs) {
+    const DevArrays& a = args.a;
+    const int sid = args.blk_scene[s / lbs];
+    if (sid < 0) return;   // whole waves: lbs is a multiple of 64
+    const SceneDev& S = args.scenes[sid];
+    const int2 m = a.M[s];
+    int key = S.c0 + S.w * S.h;
+    if (!(m.x & PF_INACTIVE)) {
+        const float4 p = a.P[args.cur][s];
+        int cx, cy;
+        key = S.c0 + cell_of(S, p.x, p.y, cx, cy);
+    }
+    bin_slot(a, s, key);
+}
+
+__device__ __forceinline__ void rb_scatter_slot(const StepArgs& args, int s, int lbs) {
+    const DevArrays& a = args.a;
+    const int sid = args.blk_scene[s / lbs];
+    if (sid < 0) return;
+    const SceneDev& S = args.scenes[sid];
+    const int pos = a.START[a.KEY[s]] - a.START[S.c0] + S.p0 + a.RANK[s];
+    if (pos >= S.p0 && pos < S.p0 + S.np) a.SIDX[pos] = s;
+    else atomicAdd(&a.STAT[3], 1ull);
+}
+
+// Neighbour list of slot s: the slots within r_a + r_b + skin (x 1.0001) in the 3x3
+// stencil of its cell; more than NLMAX marks the particle for the stencil fallback
+// (NC = -1).  Records the build position.
+__device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int lbs) {
+    const DevArrays& a = args.a;
+    const int sid = args.blk_scene[s / lbs];
+    if (sid < 0) return;
+    const SceneDev& S = args.scenes[sid];
+    const int cap = (int)a.cap;
+    const int cur = args.cur;
+    const int2 m = a.M[s];
+    const float4 p = a.P[cur][s];
+    a.XR[s] = make_float2(p.x, p.y);
+    if (m.x & PF_INACTIVE) {
+        a.NC[s] = 0;
+        return;
+    }
+    int cx, cy;
+    cell_of(S, p.x, p.y, cx, cy);
+    const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
+    const int pos0 = S.p0 - a.START[S.c0];
+    const int rlo = max(cy - 1, 0), rhi = min(cy + 1, S.h - 1);
+    // the three stencil rows advance together, two candidates each per round, so
+    // that six slot-id loads and then six position loads are in flight at once
+    // (the list's order is free: the step kernel inserts partners in id order)
+    int e[3], e1[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const int ry = rlo + r;
+        if (ry <= rhi) {
+            e[r] = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
+            e1[r] = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
+        } else {
+            e[r] = 0;
+            e1[r] = 0;
+        }
+    }
+    const float lr = p.z + S.skin;
+    int cnt = 0;
+    while (e[0] < e1[0] || e[1] < e1[1] || e[2] < e1[2]) {
+        int j[6];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            j[2 * r] = e[r] < e1[r] ? a.SIDX[e[r]] : -1;
+            j[2 * r + 1] = e[r] + 1 < e1[r] ? a.SIDX[e[r] + 1] : -1;
+        }
+        float4 q[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) q[k] = j[k] >= 0 ? a.P[cur][j[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            if (j[k] < 0 || j[k] == s) continue;
+            const float dx = q[k].x - p.x, dy = q[k].y - p.y;
+            const float xx = dx * dx;
+            const float yy = dy * dy;
+            const float L = (lr + q[k].z) * 1.0001f;
+            if (xx + yy < L * L) {
+                if (cnt < NLMAX) a.NL[cnt * cap + s] = j[k];
+                cnt++;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++) e[r] += 2;
+    }
+    a.NC[s] = cnt <= NLMAX ? cnt : -1;
+    if (cnt > NLMAX) atomicAdd(&a.STAT[5], 1ull);   // particles on the stencil fallback
+}
+
+// The rebuild flag is double-buffered by position buffer: a drifting launch sets
+// RB[nxt] for the rebuild that follows it, and that rebuild's first kernel clears
+// the other flag, whose rebuild has run (no kernel of this chain reads it).
+__global__ void k_rb_bin(StepArgs args, int nlb) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) args.a.RB[args.cur ^ 1] = 0;
+    if (args.a.RB[args.cur] == 0) return;
+    const int lbs = (int)blockDim.x;
+    for (int lb = (int)blockIdx.x; lb < nlb; lb += (int)gridDim.x) rb_bin_slot(args, lb * lbs + (int)threadIdx.x, lbs);
+}
+
+__global__ void k_rb_scatter(StepArgs args, int nlb) {
+    if (args.a.RB[args.cur] == 0) return;
+    clear_tile_sums(args.a);
+    const int lbs = (int)blockDim.x;
+    for (int lb = (int)blockIdx.x; lb < nlb; lb += (int)gridDim.x)
+        rb_scatter_slot(args, lb * lbs + (int)threadIdx.x, lbs);
+}
+
+__global__ void k_build_nl(StepArgs args, int nlb) {
+    if (args.a.RB[args.cur] == 0) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&args.a.STAT[4], 1ull);   // rebuilds
+    const int lbs = (int)blockDim.x;
+    for (int lb = (int)blockIdx.x; lb < nlb; lb += (int)gridDim.x)
+        build_nl_slot(args, lb * lbs + (int)threadIdx.x, lbs);
 }
 
 // ---- (cell, id) order of the cell list (the debug view of stages 1-3) ----
@@ -855,28 +1019,23 @@ __global__ void k_cell_sort(StepArgs args, int cell_lo, int cell_hi, int p0) {
 }
 
 // ---- storage re-sort ----
-// 1. move every per-particle array to its cell-list position: slot j takes the
-//    particle at position j of the scene's list (the scene's cells, sentinel included,
-//    hold exactly its np slots, so the list is a permutation of the scene's slots).
+// 1. move every per-particle array to its destination: slot j takes the particle
+//    order[j] (the scene's cells, sentinel included, hold exactly its np slots, so the
+//    list, and with it the order, is a permutation of the scene's slots).
 __global__ void k_reorder_move(ReorderArgs r) {
     const int sid = r.blk_scene[blockIdx.x];
     if (sid < 0) return;
     const SceneDev& S = r.scenes[sid];
-    const uint32_t fl = (uint32_t)r.lf[sid].x;
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (j < S.p0 || j >= S.p0 + S.np) return;
-    const bool move = (fl & SF_ACTIVE) && (fl & SF_BIN);
-    int s = move ? r.src.SIDX[j] : j;
-    if (s < S.p0 || s >= S.p0 + S.np) s = j;
+    const int s = r.order[j];
     const int cur = r.cur;
     r.dst.P[cur][j] = r.src.P[cur][s];
     r.dst.D[cur][j] = r.src.D[cur][s];
     r.dst.V[j] = r.src.V[s];
     const int2 m = r.src.M[s];
     r.dst.M[j] = m;
-    // A of an active particle of a moving scene is dead: the re-sort follows a
-    // binning drift, so a force pass rewrites A before the next drift reads it
-    if (!move || (m.x & PF_INACTIVE)) r.dst.A[j] = r.src.A[s];
+    r.dst.A[j] = r.src.A[s];
     if (r.with_ext) r.dst.EXT[j] = r.src.EXT[s];
     if (r.with_data) {
         r.dst.DATA0[j] = r.src.DATA0[s];
@@ -884,17 +1043,49 @@ __global__ void k_reorder_move(ReorderArgs r) {
     }
     const int c = r.src.CC[s];
     r.dst.CC[j] = c;
+    const int cap = (int)r.src.cap;
     for (int q = 0; q < c && q < KMAX; q++) {
-        r.dst.CP[q * (int)r.dst.cap + j] = r.src.CP[q * (int)r.src.cap + s];
-        r.dst.CF[q * (int)r.dst.cap + j] = r.src.CF[q * (int)r.src.cap + s];
+        r.dst.CP[q * cap + j] = r.src.CP[q * cap + s];
+        r.dst.CF[q * cap + j] = r.src.CF[q * cap + s];
     }
+    // the neighbour list, its candidates renamed to their destination slots
+    const int nc = r.src.NC[s];
+    r.dst.NC[j] = nc;
+    r.dst.XR[j] = r.src.XR[s];
+    for (int q = 0; q < nc; q++) r.dst.NL[q * cap + j] = r.newslot[r.src.NL[q * cap + s]];
     r.dst.INV[S.p0 + m.y] = j;
 }
 
-// 2. the cell list becomes the identity
-__global__ void k_identity(int32_t* sidx, int64_t n) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) sidx[j] = (int32_t)j;
+// 0. the destination of every slot: each block of list positions (the scene table's
+//    blocks, so no block straddles scenes) keeps its positions, partitioned stably
+//    into particles with neighbour-list candidates first.  The candidate lanes of a
+//    block then fill whole waves and the other waves skip the narrow phase.  The cell
+//    list is renamed to the destination slots in place.
+__global__ void k_partition(ReorderArgs r) {
+    const int lb = (int)blockIdx.x;
+    const int sid = r.blk_scene[lb];
+    if (sid < 0) return;
+    const SceneDev& S = r.scenes[sid];
+    const int j = lb * (int)blockDim.x + (int)threadIdx.x;
+    if (j < S.p0 || j >= S.p0 + S.np) return;   // whole blocks (np is a multiple of the block)
+    int s = r.src.SIDX[j];
+    if (s < S.p0 || s >= S.p0 + S.np) s = j;     // a damaged list keeps the slot in place
+    const bool f = r.src.NC[s] != 0;
+    __shared__ int s_cnt[16];                    // flagged lanes per wave (blockDim <= 1024)
+    const uint64_t bf = __ballot(f);
+    const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+    if (lane == 0) s_cnt[wv] = __popcll(bf);
+    __syncthreads();
+    int pre = 0, tot = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
+        if (q < wv) pre += s_cnt[q];
+        tot += s_cnt[q];
+    }
+    const int rf = pre + __popcll(bf & ((1ull << lane) - 1ull));   // flagged lanes before this one
+    const int dst = lb * (int)blockDim.x + (f ? rf : tot + ((int)threadIdx.x - rf));
+    r.order[dst] = s;
+    r.newslot[s] = dst;
+    r.src.SIDX[j] = dst;
 }
 
 // ---- observables ----
@@ -999,12 +1190,34 @@ hipError_t launch_bin(const StepArgs& args, int nblocks, int block, hipStream_t 
     return hipGetLastError();
 }
 
-int scan_tiles(int n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
-hipError_t launch_scan(const DevArrays& a, int ncells, hipStream_t st) {
+int scan_tiles(int n) { return tile_count(n); }
+
+// gate: NULL to scan unconditionally, else the rebuild flag
+hipError_t launch_scan(const DevArrays& a, int ncells, const int32_t* gate, hipStream_t st) {
     const int nt = scan_tiles(ncells);
-    hipLaunchKernelGGL(k_tile_sums, dim3(nt), dim3(SCAN_THREADS), 0, st, a.CNT.p, ncells, a.TSUM.p);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, st, a.CNT.p, a.START.p, ncells, a.TSUM.p);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, st, a.CNT.p, a.START.p, ncells, a.TSUM.p,
+                       gate);
+    return hipGetLastError();
+}
+
+hipError_t launch_rb_bin(const StepArgs& args, int nlb, int grid, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_rb_bin, dim3(grid), dim3(block), 0, st, args, nlb);
+    return hipGetLastError();
+}
+
+hipError_t launch_rb_scatter(const StepArgs& args, int nlb, int grid, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_rb_scatter, dim3(grid), dim3(block), 0, st, args, nlb);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_nl(const StepArgs& args, int nlb, int grid, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_build_nl, dim3(grid), dim3(block), 0, st, args, nlb);
+    return hipGetLastError();
+}
+
+hipError_t launch_rb_store(int32_t* rb, int v, hipStream_t st) {
+    hipLaunchKernelGGL(k_rb_store, dim3(1), dim3(64), 0, st, rb, v);
     return hipGetLastError();
 }
 
@@ -1026,9 +1239,8 @@ hipError_t launch_reorder_move(const ReorderArgs& r, int nblocks, int block, hip
     return hipGetLastError();
 }
 
-hipError_t launch_identity(int32_t* sidx, int64_t n, hipStream_t st) {
-    const int b = 256;
-    hipLaunchKernelGGL(k_identity, dim3((unsigned)((n + b - 1) / b)), dim3(b), 0, st, sidx, n);
+hipError_t launch_partition(const ReorderArgs& r, int nblocks, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_partition, dim3(nblocks), dim3(block), 0, st, r);
     return hipGetLastError();
 }
 

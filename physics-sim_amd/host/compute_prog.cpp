@@ -19,6 +19,8 @@ void check(int rc) {
 
 }  // namespace
 
+constexpr double kSkinFraction = 0.1;
+
 float min_cell_size(float rmax, float extent) {
     // 2 r_max plus room for the rounding of (x - base) / cell_size, a few ulp of the extent
     const float e = extent;
@@ -32,8 +34,9 @@ dem_grid ComputeProg::default_grid(const Settings& settings, const State& state,
                     : (double)settings.setup.max_radius;
     rmax = std::max(rmax, 1e-6);
     const double ext = std::max((double)settings.simulation.hor_bound, (double)settings.simulation.vert_bound) + 8.0 * rmax;
-    float cs = min_cell_size((float)rmax, (float)ext);
-    if (cell_size > 0.0f) cs = std::max(cs, cell_size);
+    const float lo = min_cell_size((float)rmax, (float)ext);
+    // default: room for a neighbour-list skin of kSkinFraction * 2 r_max
+    const float cs = cell_size > 0.0f ? std::max(lo, cell_size) : (float)((double)lo + kSkinFraction * 2.0 * rmax);
     return settings.box_grid(cs);
 }
 

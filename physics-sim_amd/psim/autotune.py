@@ -2,8 +2,9 @@
 
 The two launch parameters that change the step's speed without changing its results
 are the threads per block of the particle kernels and the grid's cell side (any cell
-of at least 2 r_max plus the rounding margin finds the same contacts; bigger cells
-mean fewer cells to scan and more candidates per disc).  `tune` times a short burst of
+of at least 2 r_max plus the rounding margin finds the same contacts; what a cell
+leaves above that is the neighbour-list skin: bigger cells mean fewer cells, longer
+lists and rarer rebuilds).  `tune` times a short burst of
 a scene for every pair on the device (HIP events around every launch) and returns
 the fastest; `Tuner` caches the choice per scene class (kind and size bucket), so a
 serving loop tunes once per class, not per scene.
@@ -13,7 +14,7 @@ from __future__ import annotations
 import math
 from typing import Dict, Iterable, Optional, Tuple
 
-from .compute import ComputeProg, min_cell_size
+from .compute import ComputeProg, default_cell_size
 from .settings import Settings
 from .state import State
 
@@ -22,10 +23,11 @@ CELL_FACTORS = (1.0, 1.25, 1.5)
 
 
 def cell_size_for(settings: Settings, state: State, factor: float) -> float:
+    """`factor` times the default cell side (2 r_max + margin + neighbour-list skin)."""
     n = state.p_count
     rmax = float(max(state.radii[:n].max(), 1e-6)) if n else float(settings.setup.max_radius)
     ext = max(settings.simulation.hor_bound, settings.simulation.vert_bound) + 8.0 * rmax
-    return min_cell_size(rmax, ext) * factor
+    return default_cell_size(rmax, ext) * factor
 
 
 def time_config(settings: Settings, state: State, block: int, factor: float, steps: int = 20, warmup: int = 5) -> float:

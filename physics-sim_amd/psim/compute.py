@@ -55,16 +55,19 @@ class ComputeProg:
     # -- configuration -----------------------------------------------------
     @staticmethod
     def default_grid(settings: Settings, state: State, cell_size: Optional[float] = None) -> abi.DemGrid:
-        """Cells of side just above 2 * (largest radius) over the walled box plus one
-        cell of margin.  Any grid whose cells clear min_cell_size() gives the same
-        contacts; cell_size picks a coarser one."""
+        """Cells over the walled box plus one cell of margin, of side
+        default_cell_size() (2 r_max, the rounding margin and a neighbour-list skin of
+        SKIN_FRACTION * 2 r_max) or `cell_size` (at least min_cell_size()).  Every such
+        grid gives the same contacts; the cell side only trades cells for candidates
+        and rebuild frequency."""
         n = state.p_count
         rmax = float(np.max(state.radii[:n])) if n else float(settings.setup.max_radius)
         rmax = max(rmax, 1e-6)
         ext = max(settings.simulation.hor_bound, settings.simulation.vert_bound) + 8.0 * rmax
-        cs = min_cell_size(rmax, ext)
         if cell_size is not None:
-            cs = max(cs, float(cell_size))
+            cs = max(min_cell_size(rmax, ext), float(cell_size))
+        else:
+            cs = default_cell_size(rmax, ext)
         return settings.box_grid(cell_size=cs)
 
     def _params(self, settings: Settings) -> abi.DemParams:
@@ -126,10 +129,10 @@ class ComputeProg:
         return out.value
 
     def counters(self) -> dict:
-        out = (C.c_int64 * 9)()
-        abi.check(self.lib.dem_counters(self.ctx, out, 9))
+        out = (C.c_int64 * 10)()
+        abi.check(self.lib.dem_counters(self.ctx, out, 10))
         keys = ["steps", "broad_phases", "contact_overflow", "bond_overflow", "contacts", "bond_endpoints",
-                "reorders", "scan_timeouts", "scatter_errors"]
+                "reorders", "scan_timeouts", "scatter_errors", "list_overflow"]
         return {k: int(out[i]) for i, k in enumerate(keys)}
 
     def datum(self, selections) -> Optional[np.ndarray]:
@@ -227,6 +230,15 @@ class ComputeProg:
             self.close()
         except Exception:
             pass
+
+
+# Neighbour-list skin of the default grid, as a fraction of 2 r_max: the lists are
+# rebuilt when a particle has moved 0.45 skin since the last build.
+SKIN_FRACTION = 0.1
+
+
+def default_cell_size(rmax: float, extent: float) -> float:
+    return float(np.float32(min_cell_size(rmax, extent) + SKIN_FRACTION * 2.0 * rmax))
 
 
 def min_cell_size(rmax: float, extent: float) -> float:

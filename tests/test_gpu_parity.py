@@ -132,3 +132,56 @@ def test_experiment_replay_matches_oracle_and_traces(scene):
             for c in range(1, 7):
                 scale = max(np.abs(ref[:n, c]).max(), 1e-30)
                 assert np.abs(ref[:n, c] - gpu[stem][:n, c]).max() / scale < 1e-3, (stem, c)
+
+
+def _run_both(s, st, calls, **kw):
+    prog = ComputeProg(s, st.copy(), **kw)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    for k in calls:
+        prog.step(k)
+        eng.step(k)
+    got = prog.update_state()
+    c = prog.counters()
+    prog.close()
+    return got, eng, c
+
+
+def test_neighbour_list_overflow_takes_stencil_fallback():
+    """One r = 4 disc in a dense bed of r ~ U[0.3, 0.45] discs: the big disc has more
+    neighbour-list candidates than a list holds and is searched over the cell list of
+    the last rebuild instead; the run stays bit-exact with the oracle."""
+    s, st = scenes.granular_box(n_side=24, seed=11, spacing=1.0, rmin=0.3, rmax=0.45)
+    n = st.p_count
+    p = st.pos[: 2 * n].reshape(-1, 2)
+    d = np.hypot(p[:, 0], p[:, 1])
+    big = int(np.argmin(d))
+    keep = (d > 4.0 + 0.5) | (np.arange(n) == big)
+    st2 = scenes._new_state(int(keep.sum()))
+    idx = np.nonzero(keep)[0]
+    st2.pos[: 2 * len(idx)] = p[idx].ravel()
+    st2.radii[: len(idx)] = st.radii[idx]
+    j = int(np.nonzero(idx == big)[0][0])
+    st2.pos[2 * j: 2 * j + 2] = 0.0
+    st2.radii[j] = 4.0
+    s.setup.particles = st2.p_count
+    got, eng, c = _run_both(s, st2, (60, 120, 120), reorder_interval=16)
+    assert_bits_equal(got, eng.st, st2.p_count)
+    assert c["list_overflow"] > 0 and c["broad_phases"] >= 1 and c["contact_overflow"] == 0
+
+
+@pytest.mark.parametrize("reorder", [8, -1])
+def test_fast_motion_rebuilds_lists_often(reorder):
+    """Discs at ~30 units/s (dt 1e-4): the lists are rebuilt every few dozen steps
+    as particles cross 0.45 skin; every step stays bit-exact with the oracle.  With
+    re-sorts every 8 steps the rebuilds alternate between the launch chain of a
+    re-sort and the single cooperative launch; without re-sorts every rebuild after
+    the first is the cooperative one."""
+    s, st = scenes.granular_box(n_side=24, seed=12, spacing=1.9)
+    n = st.p_count
+    st.vel[: 2 * n] = np.random.default_rng(12).normal(0.0, 30.0, 2 * n).astype(np.float32)
+    got, eng, c = _run_both(s, st, (100, 100, 100), reorder_interval=reorder)
+    assert_bits_equal(got, eng.st, n)
+    assert np.array_equal(got.data[: 7 * n].view(np.int32), eng.st.data[: 7 * n].view(np.int32))
+    assert c["broad_phases"] >= 5
+    if reorder < 0:
+        assert c["reorders"] == 0
