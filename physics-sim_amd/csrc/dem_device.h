@@ -14,6 +14,20 @@ namespace psim {
 
 constexpr int KMAX = 16;         // contacts (unbonded) per particle
 constexpr int NLMAX = 16;        // neighbour-list candidates per particle (more: stencil fallback)
+
+// Row 0 of a neighbour list: the candidate count in the top bits (NL_OVERFLOW: more
+// than NLMAX, stencil fallback) and the first candidate's slot in the low 26 bits
+// (slots stay below 2^26), so one load gives the count and the first candidate.
+constexpr int NL_SHIFT = 26;
+constexpr int NL_OVERFLOW = 31;
+__host__ __device__ inline int nl_word(int count, int first) {
+    return ((count < 0 ? NL_OVERFLOW : count) << NL_SHIFT) | first;
+}
+__host__ __device__ inline int nl_count(int w) {
+    const int c = (int)((unsigned)w >> NL_SHIFT);
+    return c == NL_OVERFLOW ? -1 : c;
+}
+__host__ __device__ inline int nl_first(int w) { return w & ((1 << NL_SHIFT) - 1); }
 constexpr int DATA_WORDS = 7;
 
 // per-particle flag bits (word M.x bits 16..31; bits 0..15 hold the material id)
@@ -111,8 +125,8 @@ struct DevArrays {
     // neighbour lists (Verlet lists with a skin), rebuilt on the device when a particle
     // has moved more than 0.45 skin since the last build; part of the particle set
     // (a re-sort moves them, renaming the candidate slots)
-    Pool<int32_t> NL;       // [NLMAX][cap] candidate slots within r_a + r_b + skin at the build
-    Pool<int32_t> NC;       // [cap] candidate count; -1: more than NLMAX (stencil fallback)
+    Pool<int32_t> NL;       // [NLMAX][cap] candidate slots within r_a + r_b + skin at the build;
+                            // row 0 packs the count with the first slot (nl_word)
     Pool<float2> XR;        // [cap] position at the build
     Pool<int32_t> RB;       // [1] rebuild flag
     // counters

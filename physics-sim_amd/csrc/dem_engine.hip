@@ -51,7 +51,7 @@ Engine::~Engine() {
         hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.D[0]); hipFree(s.D[1]);
         hipFree(s.V); hipFree(s.A); hipFree(s.M); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
     }
-    for (auto& s : sets_) { hipFree(s.CC); hipFree(s.NL); hipFree(s.NC); hipFree(s.XR); }
+    for (auto& s : sets_) { hipFree(s.CC); hipFree(s.NL); hipFree(s.XR); }
     for (auto& l : lists_) { hipFree(l.CP); hipFree(l.CF); }
     hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BL_); hipFree(BSTR_); hipFree(MAT_);
     hipFree(KEY_); hipFree(RANK_); hipFree(CNT_); hipFree(START_); hipFree(SIDX_); hipFree(INV_);
@@ -103,7 +103,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
         e = e ? e : dalloc(&s.D[0], S); e = e ? e : dalloc(&s.D[1], S);
         e = e ? e : dalloc(&s.V, S); e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S);
         e = e ? e : dalloc(&s.EXT, S); e = e ? e : dalloc(&s.CC, S);
-        e = e ? e : dalloc(&s.NL, S * NLMAX); e = e ? e : dalloc(&s.NC, S); e = e ? e : dalloc(&s.XR, S);
+        e = e ? e : dalloc(&s.NL, S * NLMAX); e = e ? e : dalloc(&s.XR, S);
         if (with_data) { e = e ? e : dalloc(&s.DATA0, S); e = e ? e : dalloc(&s.DATA1, S); }
         if ((rc = check(e, "hipMalloc particle pools"))) return rc;
     }
@@ -141,7 +141,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = e ? e : hipMemsetAsync(ORDER_, 0, sizeof(int32_t) * S, stream_);
     e = e ? e : hipMemsetAsync(NEWSLOT_, 0, sizeof(int32_t) * S, stream_);
     e = e ? e : hipMemsetAsync(RB_, 0, 2 * sizeof(int32_t), stream_);
-    for (auto& st : sets_) e = e ? e : hipMemsetAsync(st.NC, 0, sizeof(int32_t) * S, stream_);
+    for (auto& st : sets_) e = e ? e : hipMemsetAsync(st.NL, 0, sizeof(int32_t) * S, stream_);   // empty lists
     e = e ? e : hipMemsetAsync(BOFF_, 0, sizeof(int2) * S, stream_);
     for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int2) * S, stream_);
     for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.CC, 0, sizeof(int32_t) * S, stream_);
@@ -389,7 +389,7 @@ DevArrays Engine::arrays(int set) const {
     a.cap = slot_cap_;
     a.BOFF = BOFF_; a.BR = BR_; a.BF = BF_; a.BS = BS_; a.BL = BL_; a.BSTR = BSTR_; a.MAT = MAT_;
     a.KEY = KEY_; a.RANK = RANK_; a.CNT = CNT_; a.START = START_; a.SIDX = SIDX_; a.INV = INV_;
-    a.NL = s.NL; a.NC = s.NC; a.XR = s.XR; a.RB = RB_;
+    a.NL = s.NL; a.XR = s.XR; a.RB = RB_;
     a.STAT = STAT_; a.TSUM = TSUM_; a.ntiles = ntiles_;
     return a;
 }

@@ -401,19 +401,34 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             if (!bonded_partner<BONDS>(a, b_lo, b_hi, oj)) list_insert(a, s, oj, n);
         }
     };
-    const int nc = (narrow && act) ? a.NC[s] : 0;
-    if (narrow && act && nc >= 0) {
+    const int w0 = (narrow && act) ? a.NL[s] : 0;
+    const int nc = nl_count(w0);
+    if (narrow && act && nc > 0) {
         // neighbour list: every particle within r_a + r_b + skin at the last build; none
-        // outside it can touch s while no particle has moved more than 0.45 skin since
-        for (int k = 0; k < nc; k += 2) {
-            const int j0 = a.NL[k * cap + s];
-            const int j1 = k + 1 < nc ? a.NL[(k + 1) * cap + s] : -1;
-            const float4 q0 = a.P[cur][j0];
-            const float4 q1 = j1 >= 0 ? a.P[cur][j1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            test(j0, q0);
-            if (j1 >= 0) test(j1, q1);
+        // outside it can touch s while no particle has moved more than 0.45 skin since.
+        // The first four candidates' positions are loaded together.
+        const int j0 = nl_first(w0);
+        const int j1 = nc > 1 ? a.NL[cap + s] : -1;
+        const int j2 = nc > 2 ? a.NL[2 * cap + s] : -1;
+        const int j3 = nc > 3 ? a.NL[3 * cap + s] : -1;
+        const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 q0 = a.P[cur][j0];
+        const float4 q1 = j1 >= 0 ? a.P[cur][j1] : z;
+        const float4 q2 = j2 >= 0 ? a.P[cur][j2] : z;
+        const float4 q3 = j3 >= 0 ? a.P[cur][j3] : z;
+        test(j0, q0);
+        if (j1 >= 0) test(j1, q1);
+        if (j2 >= 0) test(j2, q2);
+        if (j3 >= 0) test(j3, q3);
+        for (int k = 4; k < nc; k += 2) {
+            const int ja = a.NL[k * cap + s];
+            const int jb = k + 1 < nc ? a.NL[(k + 1) * cap + s] : -1;
+            const float4 qa = a.P[cur][ja];
+            const float4 qb = jb >= 0 ? a.P[cur][jb] : z;
+            test(ja, qa);
+            if (jb >= 0) test(jb, qb);
         }
-    } else if (narrow && act) {
+    } else if (narrow && act && nc < 0) {
         // more candidates than a list holds: the 3x3 stencil around the particle's cell
         // at the last build, over the last cell list (cells are wider than
         // 2 r_max + skin, so it holds every particle that can touch s until the rebuild)
@@ -903,7 +918,7 @@ __device__ __forceinline__ void rb_scatter_slot(const StepArgs& args, int s, int
 
 // Neighbour list of slot s: the slots within r_a + r_b + skin (x 1.0001) in the 3x3
 // stencil of its cell; more than NLMAX marks the particle for the stencil fallback
-// (NC = -1).  Records the build position.
+// (count NL_OVERFLOW).  Records the build position.
 __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int lbs) {
     const DevArrays& a = args.a;
     const int sid = args.blk_scene[s / lbs];
@@ -915,7 +930,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
     const float4 p = a.P[cur][s];
     a.XR[s] = make_float2(p.x, p.y);
     if (m.x & PF_INACTIVE) {
-        a.NC[s] = 0;
+        a.NL[s] = nl_word(0, 0);
         return;
     }
     int cx, cy;
@@ -939,7 +954,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
         }
     }
     const float lr = p.z + S.skin;
-    int cnt = 0;
+    int cnt = 0, first = 0;
     while (e[0] < e1[0] || e[1] < e1[1] || e[2] < e1[2]) {
         int j[6];
 #pragma unroll
@@ -958,14 +973,15 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
             const float yy = dy * dy;
             const float L = (lr + q[k].z) * 1.0001f;
             if (xx + yy < L * L) {
-                if (cnt < NLMAX) a.NL[cnt * cap + s] = j[k];
+                if (cnt == 0) first = j[k];
+                else if (cnt < NLMAX) a.NL[cnt * cap + s] = j[k];
                 cnt++;
             }
         }
 #pragma unroll
         for (int r = 0; r < 3; r++) e[r] += 2;
     }
-    a.NC[s] = cnt <= NLMAX ? cnt : -1;
+    a.NL[s] = nl_word(cnt <= NLMAX ? cnt : -1, first);
     if (cnt > NLMAX) atomicAdd(&a.STAT[5], 1ull);   // particles on the stencil fallback
 }
 
@@ -1049,10 +1065,11 @@ __global__ void k_reorder_move(ReorderArgs r) {
         r.dst.CF[q * cap + j] = r.src.CF[q * cap + s];
     }
     // the neighbour list, its candidates renamed to their destination slots
-    const int nc = r.src.NC[s];
-    r.dst.NC[j] = nc;
+    const int w0 = r.src.NL[s];
+    const int nc = nl_count(w0);
+    r.dst.NL[j] = nl_word(nc, nc > 0 ? r.newslot[nl_first(w0)] : 0);
     r.dst.XR[j] = r.src.XR[s];
-    for (int q = 0; q < nc; q++) r.dst.NL[q * cap + j] = r.newslot[r.src.NL[q * cap + s]];
+    for (int q = 1; q < nc; q++) r.dst.NL[q * cap + j] = r.newslot[r.src.NL[q * cap + s]];
     r.dst.INV[S.p0 + m.y] = j;
 }
 
@@ -1070,7 +1087,7 @@ __global__ void k_partition(ReorderArgs r) {
     if (j < S.p0 || j >= S.p0 + S.np) return;   // whole blocks (np is a multiple of the block)
     int s = r.src.SIDX[j];
     if (s < S.p0 || s >= S.p0 + S.np) s = j;     // a damaged list keeps the slot in place
-    const bool f = r.src.NC[s] != 0;
+    const bool f = nl_count(r.src.NL[s]) != 0;
     __shared__ int s_cnt[16];                    // flagged lanes per wave (blockDim <= 1024)
     const uint64_t bf = __ballot(f);
     const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
