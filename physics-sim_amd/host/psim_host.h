@@ -146,7 +146,7 @@ public:
         int32_t capacity = 0;        // 0: the state's particle count
         int32_t bond_capacity = 0;   // 0: the state's bond count
         bool record_data = true;
-        int32_t reorder_interval = 8;
+        int32_t reorder_interval = 64;
         int32_t block_size = 256;
         float cell_size = 0.0f;      // 0: just above 2 r_max (default_grid)
         int use_f64 = -1;            // -1: settings.simulation.use_f64

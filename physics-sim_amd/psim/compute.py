@@ -30,7 +30,7 @@ from .state import State
 class ComputeProg:
     def __init__(self, settings: Settings, state: State, capacity: Optional[int] = None,
                  bond_capacity: Optional[int] = None, grid: Optional[abi.DemGrid] = None,
-                 use_f64: Optional[bool] = None, record_data: bool = True, reorder_interval: int = 8,
+                 use_f64: Optional[bool] = None, record_data: bool = True, reorder_interval: int = 64,
                  block_size: int = 256):
         self.lib = abi.load_library()
         if self.lib.dem_device_count() <= 0:

@@ -23,7 +23,7 @@ from .state import State
 
 
 def scene_params(settings: Settings, state: State, grid: Optional[abi.DemGrid] = None,
-                 use_f64: Optional[bool] = None, record_data: bool = True, reorder_interval: int = 8,
+                 use_f64: Optional[bool] = None, record_data: bool = True, reorder_interval: int = 64,
                  block_size: int = 256):
     """dem_params for one scene (as ComputeProg builds them); returns (params, keep-alive)."""
     p = abi.DemParams()
@@ -67,7 +67,7 @@ class Server:
         self._shape = {}
 
     def submit(self, settings: Settings, state: State, nsteps: int, grid: Optional[abi.DemGrid] = None,
-               use_f64: Optional[bool] = None, reorder_interval: int = 8) -> int:
+               use_f64: Optional[bool] = None, reorder_interval: int = 64) -> int:
         p, _keep = scene_params(settings, state, grid, use_f64, self.record_data, reorder_interval, self.block_size)
         v = state.view(state.p_count, with_data=self.record_data)
         t = C.c_int64()

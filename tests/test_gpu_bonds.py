@@ -39,7 +39,7 @@ def chain_scene(mode, chains=8, per_chain=32, scale=50.0, seed=3):
 def test_struck_chains_match_oracle(mode):
     s, st = chain_scene(mode)
     n, nb = st.p_count, st.n_bonds
-    prog = ComputeProg(s, st.copy(), record_data=True)
+    prog = ComputeProg(s, st.copy(), record_data=True, reorder_interval=8)
     eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
     for k in (120, 240, 240):       # several calls: the state crosses call boundaries
         prog.step(k)
@@ -99,7 +99,7 @@ def test_tensile_assembly_matches_oracle():
     on the downloaded state."""
     s, st, bottom, top = scenes.bonded_assembly(n_side=24, pull_v=0.02)
     n = st.p_count
-    prog = ComputeProg(s, st.copy(), bond_capacity=6 * n, record_data=True)
+    prog = ComputeProg(s, st.copy(), bond_capacity=6 * n, record_data=True, reorder_interval=8)
     built = prog.regen_bonds(s)
     nb = built.n_bonds
     built.bond_strengths = scenes.assembly_strengths(nb)
