@@ -403,6 +403,14 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     };
     const int w0 = (narrow && act) ? a.NL[s] : 0;
     const int nc = nl_count(w0);
+    // Register path (plain kernels): at most four candidates and four old contacts.
+    // The hits stay in registers as {partner id, slot, material word}, sorted by id,
+    // and the force loop reads each partner by slot: no id -> slot lookup and no
+    // re-read of the new list.  Same partners, same order, same arithmetic.
+    const bool fast = !BONDS && narrow && act && nc > 0 && nc <= 4 && cc_old <= 4;
+    int fo[4] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
+    int fj[4] = {0, 0, 0, 0};
+    int fm[4] = {0, 0, 0, 0};
     if (narrow && act && nc > 0) {
         // neighbour list: every particle within r_a + r_b + skin at the last build; none
         // outside it can touch s while no particle has moved more than 0.45 skin since.
@@ -416,10 +424,38 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         const float4 q1 = j1 >= 0 ? a.P[cur][j1] : z;
         const float4 q2 = j2 >= 0 ? a.P[cur][j2] : z;
         const float4 q3 = j3 >= 0 ? a.P[cur][j3] : z;
-        test(j0, q0);
-        if (j1 >= 0) test(j1, q1);
-        if (j2 >= 0) test(j2, q2);
-        if (j3 >= 0) test(j3, q3);
+        if (fast) {
+            const int jj[4] = {j0, j1, j2, j3};
+            const float4 qq[4] = {q0, q1, q2, q3};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (jj[k] >= 0 && jj[k] != s && overlaps(qq[k], p)) {
+                    const int2 mj = a.M[jj[k]];
+                    fo[k] = mj.y;
+                    fj[k] = jj[k];
+                    fm[k] = mj.x;
+                    n++;
+                }
+            }
+            // ascending partner id (misses carry INT32_MAX and sort last)
+            auto cswap = [&](int x, int y) {
+                if (fo[y] < fo[x]) {
+                    const int to = fo[x], tj = fj[x], tm = fm[x];
+                    fo[x] = fo[y]; fj[x] = fj[y]; fm[x] = fm[y];
+                    fo[y] = to; fj[y] = tj; fm[y] = tm;
+                }
+            };
+            cswap(0, 1);
+            cswap(2, 3);
+            cswap(0, 2);
+            cswap(1, 3);
+            cswap(1, 2);
+        } else {
+            test(j0, q0);
+            if (j1 >= 0) test(j1, q1);
+            if (j2 >= 0) test(j2, q2);
+            if (j3 >= 0) test(j3, q3);
+        }
         for (int k = 4; k < nc; k += 2) {
             const int ja = a.NL[k * cap + s];
             const int jb = k + 1 < nc ? a.NL[(k + 1) * cap + s] : -1;
@@ -563,8 +599,43 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             a.BF[q] = bf;
         };
 
+        if (fast) {
+            // previous list (at most four entries) for the tangential-force carry-over
+            int co[4];
+            float cf[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                co[k] = k < oc ? a.CP[k * cap + s] : -1;
+                cf[k] = k < oc ? a.CF[k * cap + s] : 0.0f;
+            }
+            for (int k = 0; k < n; k++) {
+                // the hit of rank k sits in element 0 (the arrays shift after each)
+                const int ob = fo[0], j = fj[0], mw = fm[0];
+                float ft = 0.0f;
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+                    if (m < oc && co[m] == ob) ft = cf[m];
+                const float4 pb = a.P[cur][j];
+                const float2 db = a.D[cur][j];
+                const int mb = mw & PF_MAT_MASK;
+                const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
+                const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness,
+                                                   MB.shear_stiffness, me.r + pb.z, ft, dd);
+                nfx += f.x;
+                nfy += f.y;
+                nm += f.z;
+                a.CPn[k * cap + s] = ob;
+                a.CFn[k * cap + s] = ft;
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    fo[m] = fo[m + 1];
+                    fj[m] = fj[m + 1];
+                    fm[m] = fm[m + 1];
+                }
+            }
+        }
         int pp = 0;  // read position in the previous list (ascending ids)
-        for (int q = 0; q < n; q++) {
+        for (int q = 0; q < (fast ? 0 : n); q++) {
             const int ob = a.CPn[q * cap + s];
             if (BONDS) {
                 while (bq < b_hi && a.BR[bq].x < ob) eval_record(bq++);
