@@ -131,6 +131,10 @@ private:
     hipStream_t stream_ = nullptr;
     int block_ = 256;
     int cus_ = 256;   // compute units of the device
+    // pinned host staging for uploads and downloads (grown on demand)
+    char* stage_ = nullptr;
+    size_t stage_cap_ = 0;
+    char* staging(size_t bytes);
     bool with_data_ = true;
     int64_t slot_cap_ = 0, cell_cap_ = 0, bond_cap_ = 0, rec_cap_ = 0;
     // ends of the highest live slot and cell regions: launches and the cell scan stop there

@@ -31,6 +31,20 @@ inline int32_t fbits(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
 inline float bitsf(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
 inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
+// Carves typed, 256-byte aligned arrays out of one buffer; with a null base it only
+// adds up the size.
+struct Carve {
+    char* base = nullptr;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
 template <typename T>
 hipError_t dalloc(T** p, int64_t n) {
     *p = nullptr;
@@ -58,6 +72,7 @@ Engine::~Engine() {
     hipFree(RB_); hipFree(ORDER_); hipFree(NEWSLOT_);
     hipFree(STAT_); hipFree(TSUM_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
     hipFree(d_ke_); hipFree(d_cnt_);
+    if (stage_) hipHostFree(stage_);
     for (auto e : ev_pool_) hipEventDestroy(e);
     if (stream_) hipStreamDestroy(stream_);
 }
@@ -149,6 +164,23 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = e ? e : hipMemcpyAsync(d_blk_scene_, blk.data(), sizeof(int32_t) * nblk, hipMemcpyHostToDevice, stream_);
     e = e ? e : hipStreamSynchronize(stream_);
     return check(e, "pool init");
+}
+
+char* Engine::staging(size_t bytes) {
+    if (bytes > stage_cap_) {
+        // every earlier copy from or into the buffer has completed (each call syncs)
+        if (hipStreamSynchronize(stream_) != hipSuccess) return nullptr;
+        if (stage_) hipHostFree(stage_);
+        stage_ = nullptr;
+        stage_cap_ = 0;
+        const size_t cap = std::max(bytes, (size_t)(1 << 20));
+        if (hipHostMalloc((void**)&stage_, cap, hipHostMallocDefault) != hipSuccess) {
+            stage_ = nullptr;
+            return nullptr;
+        }
+        stage_cap_ = cap;
+    }
+    return stage_;
 }
 
 int Engine::launch_grid_blocks() const {
@@ -424,12 +456,34 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     sh.np = np;
     sh.params.u.particles = n;
     const int32_t nmat = sh.nm;
-    std::vector<float4> P(np), V(np), A(np), EXT(np);
-    std::vector<float2> D(np);
-    std::vector<int2> M(np);
-    std::vector<float4> D0(np), D1(np);
-    std::vector<int32_t> CC(np, 0), CP((size_t)np * KMAX, 0), INV(np), SIDX(np);
-    std::vector<float> CF((size_t)np * KMAX, 0.0f);
+    // the device layout is built in pinned staging memory and copied from there
+    Carve cv;
+    auto carve = [&](Carve& c) {
+        c.take<float4>(np); c.take<float2>(np); c.take<float4>(np); c.take<float4>(np); c.take<int2>(np);
+        c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<int32_t>(np);
+        c.take<int32_t>((size_t)np * KMAX); c.take<float>((size_t)np * KMAX); c.take<int32_t>(np);
+        c.take<int32_t>(np); c.take<int2>(np);
+    };
+    carve(cv);
+    cv.base = staging(cv.off);
+    if (!cv.base) { set_error("upload: pinned staging allocation failed"); return DEM_ENOMEM; }
+    cv.off = 0;
+    float4* P = cv.take<float4>(np);
+    float2* D = cv.take<float2>(np);
+    float4* V = cv.take<float4>(np);
+    float4* A = cv.take<float4>(np);
+    int2* M = cv.take<int2>(np);
+    float4* EXT = cv.take<float4>(np);
+    float4* D0 = cv.take<float4>(np);
+    float4* D1 = cv.take<float4>(np);
+    int32_t* CC = cv.take<int32_t>(np);
+    int32_t* CP = cv.take<int32_t>((size_t)np * KMAX);   // rows beyond a slot's count are never read
+    float* CF = cv.take<float>((size_t)np * KMAX);
+    int32_t* INV = cv.take<int32_t>(np);
+    int32_t* SIDX = cv.take<int32_t>(np);
+    int2* BRANGE = cv.take<int2>(np);
+    std::fill(CC, CC + np, 0);
+    int32_t rows = 0;   // contact-list rows in use
     sh.fixity.assign(v.fixity, v.fixity + 6 * (size_t)n);
     sh.forces.assign(v.forces, v.forces + 6 * (size_t)n);
     sh.matptr.assign(v.material_pointers, v.material_pointers + n);
@@ -534,6 +588,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
                 return DEM_ECAPACITY;
             }
             CC[i] = (int32_t)con.size();
+            rows = std::max(rows, CC[i]);
             for (size_t q = 0; q < con.size(); q++) {
                 const Slot& s = tab[(size_t)i * K + con[q].second];
                 CP[(size_t)q * np + i] = s.b;
@@ -553,7 +608,6 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     sh.nrec = (int32_t)BR.size();
     // per particle id: its record range [begin, end) in the pool (one entry per slot, so
     // no scene's ranges touch a neighbour region)
-    std::vector<int2> BRANGE(np);
     for (int32_t i = 0; i < np; i++)
         BRANGE[i] = make_int2(BOFF[i] + (int32_t)sh.r0, BOFF[i + 1] + (int32_t)sh.r0);
     // device copies: the current set, the current P/D buffer
@@ -563,25 +617,29 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     auto h2d = [&](void* dst, const void* src, size_t bytes) {
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_);
     };
-    h2d(S.P[cur_] + p0, P.data(), sizeof(float4) * np);
-    h2d(S.D[cur_] + p0, D.data(), sizeof(float2) * np);
-    h2d(S.V + p0, V.data(), sizeof(float4) * np);
-    h2d(S.A + p0, A.data(), sizeof(float4) * np);
-    h2d(S.M + p0, M.data(), sizeof(int2) * np);
-    h2d(S.EXT + p0, EXT.data(), sizeof(float4) * np);
+    h2d(S.P[cur_] + p0, P, sizeof(float4) * np);
+    h2d(S.D[cur_] + p0, D, sizeof(float2) * np);
+    h2d(S.V + p0, V, sizeof(float4) * np);
+    h2d(S.A + p0, A, sizeof(float4) * np);
+    h2d(S.M + p0, M, sizeof(int2) * np);
+    h2d(S.EXT + p0, EXT, sizeof(float4) * np);
     if (with_data_) {
-        h2d(S.DATA0 + p0, D0.data(), sizeof(float4) * np);
-        h2d(S.DATA1 + p0, D1.data(), sizeof(float4) * np);
+        h2d(S.DATA0 + p0, D0, sizeof(float4) * np);
+        h2d(S.DATA1 + p0, D1, sizeof(float4) * np);
     }
     const ContactLists& L = lists_[lcur_];
-    h2d(S.CC + p0, CC.data(), sizeof(int32_t) * np);
-    for (int q = 0; q < KMAX; q++) {
-        h2d(L.CP + (int64_t)q * slot_cap_ + p0, CP.data() + (size_t)q * np, sizeof(int32_t) * np);
-        h2d(L.CF + (int64_t)q * slot_cap_ + p0, CF.data() + (size_t)q * np, sizeof(float) * np);
+    h2d(S.CC + p0, CC, sizeof(int32_t) * np);
+    if (rows > 0 && e == hipSuccess) {
+        // rows [0, rows) of the slot-major lists in one strided copy each
+        e = hipMemcpy2DAsync(L.CP + p0, sizeof(int32_t) * slot_cap_, CP, sizeof(int32_t) * np, sizeof(int32_t) * np,
+                             rows, hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(L.CF + p0, sizeof(float) * slot_cap_, CF, sizeof(float) * np, sizeof(float) * np,
+                                 rows, hipMemcpyHostToDevice, stream_);
     }
-    h2d(INV_ + p0, INV.data(), sizeof(int32_t) * np);
-    h2d(SIDX_ + p0, SIDX.data(), sizeof(int32_t) * np);
-    h2d(BOFF_ + p0, BRANGE.data(), sizeof(int2) * np);
+    h2d(INV_ + p0, INV, sizeof(int32_t) * np);
+    h2d(SIDX_ + p0, SIDX, sizeof(int32_t) * np);
+    h2d(BOFF_ + p0, BRANGE, sizeof(int2) * np);
     if (sh.nrec) {
         h2d(BR_ + sh.r0, BR.data(), sizeof(int4) * sh.nrec);
         h2d(BF_ + sh.r0, BF.data(), sizeof(float4) * sh.nrec);
@@ -803,42 +861,64 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     if (v.n != n) { set_error("view.n does not match the scene's particle count"); return DEM_EINVAL; }
     const DevSet& S = sets_[set_];
     const int64_t p0 = sh.p0;
-    std::vector<float4> P(np), V(np), A(np), D0(np), D1(np);
-    std::vector<int2> M(np);
-    std::vector<int32_t> INV(np), CC(np), CP((size_t)np * KMAX);
-    std::vector<float> CF((size_t)np * KMAX);
-    std::vector<int2> BRANGE(np);
-    std::vector<int4> BR(std::max(sh.nrec, 1));
-    std::vector<float4> BF(std::max(sh.nrec, 1));
-    std::vector<int2> BS(std::max(sh.nb, 1));
+    // copies land in pinned staging memory
+    const size_t nrec = (size_t)std::max(sh.nrec, 1), nbs = (size_t)std::max(sh.nb, 1);
+    Carve cv;
+    auto carve = [&](Carve& c) {
+        c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np);
+        c.take<int2>(np); c.take<int32_t>(np); c.take<int32_t>(np); c.take<int32_t>((size_t)np * KMAX);
+        c.take<float>((size_t)np * KMAX); c.take<int2>(np); c.take<int4>(nrec); c.take<float4>(nrec);
+        c.take<int2>(nbs);
+    };
+    carve(cv);
+    cv.base = staging(cv.off);
+    if (!cv.base) { set_error("download: pinned staging allocation failed"); return DEM_ENOMEM; }
+    cv.off = 0;
+    float4* P = cv.take<float4>(np);
+    float4* V = cv.take<float4>(np);
+    float4* A = cv.take<float4>(np);
+    float4* D0 = cv.take<float4>(np);
+    float4* D1 = cv.take<float4>(np);
+    int2* M = cv.take<int2>(np);
+    int32_t* INV = cv.take<int32_t>(np);
+    int32_t* CC = cv.take<int32_t>(np);
+    int32_t* CP = cv.take<int32_t>((size_t)np * KMAX);
+    float* CF = cv.take<float>((size_t)np * KMAX);
+    int2* BRANGE = cv.take<int2>(np);
+    int4* BR = cv.take<int4>(nrec);
+    float4* BF = cv.take<float4>(nrec);
+    int2* BS = cv.take<int2>(nbs);
     hipError_t e = hipSuccess;
     auto d2h = [&](void* dst, const void* src, size_t bytes) {
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_);
     };
-    d2h(P.data(), S.P[cur_] + p0, sizeof(float4) * np);
-    d2h(V.data(), S.V + p0, sizeof(float4) * np);
-    d2h(A.data(), S.A + p0, sizeof(float4) * np);
-    d2h(M.data(), S.M + p0, sizeof(int2) * np);
-    d2h(INV.data(), INV_ + p0, sizeof(int32_t) * np);
+    d2h(P, S.P[cur_] + p0, sizeof(float4) * np);
+    d2h(V, S.V + p0, sizeof(float4) * np);
+    d2h(A, S.A + p0, sizeof(float4) * np);
+    d2h(M, S.M + p0, sizeof(int2) * np);
+    d2h(INV, INV_ + p0, sizeof(int32_t) * np);
     const bool rec = with_data_ && sh.params.record_data && v.data;
     if (rec) {
-        d2h(D0.data(), S.DATA0 + p0, sizeof(float4) * np);
-        d2h(D1.data(), S.DATA1 + p0, sizeof(float4) * np);
+        d2h(D0, S.DATA0 + p0, sizeof(float4) * np);
+        d2h(D1, S.DATA1 + p0, sizeof(float4) * np);
     }
     if (v.contacts) {
         const ContactLists& L = lists_[lcur_];
-        d2h(CC.data(), sets_[set_].CC + p0, sizeof(int32_t) * np);
-        for (int q = 0; q < KMAX; q++) {
-            d2h(CP.data() + (size_t)q * np, L.CP + (int64_t)q * slot_cap_ + p0, sizeof(int32_t) * np);
-            d2h(CF.data() + (size_t)q * np, L.CF + (int64_t)q * slot_cap_ + p0, sizeof(float) * np);
-        }
-        d2h(BRANGE.data(), BOFF_ + p0, sizeof(int2) * np);
+        d2h(CC, sets_[set_].CC + p0, sizeof(int32_t) * np);
+        // the slot-major lists, KMAX rows in one strided copy each
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(CP, sizeof(int32_t) * np, L.CP + p0, sizeof(int32_t) * slot_cap_,
+                                 sizeof(int32_t) * np, KMAX, hipMemcpyDeviceToHost, stream_);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(CF, sizeof(float) * np, L.CF + p0, sizeof(float) * slot_cap_, sizeof(float) * np,
+                                 KMAX, hipMemcpyDeviceToHost, stream_);
+        d2h(BRANGE, BOFF_ + p0, sizeof(int2) * np);
         if (sh.nrec) {
-            d2h(BR.data(), BR_ + sh.r0, sizeof(int4) * sh.nrec);
-            d2h(BF.data(), BF_ + sh.r0, sizeof(float4) * sh.nrec);
+            d2h(BR, BR_ + sh.r0, sizeof(int4) * sh.nrec);
+            d2h(BF, BF_ + sh.r0, sizeof(float4) * sh.nrec);
         }
     }
-    if (sh.nb) d2h(BS.data(), BS_ + sh.b0, sizeof(int2) * sh.nb);
+    if (sh.nb) d2h(BS, BS_ + sh.b0, sizeof(int2) * sh.nb);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     int rc = check(e, "download");
     if (rc) return rc;
