@@ -168,6 +168,32 @@ def serving_bench(args):
     return out
 
 
+def graph_bench(args):
+    """Repeated dem_step(50) calls on one small scene (a 16,384-disc granular box, the
+    reference's frame loop: gen_per_frame steps per frame), launched directly and
+    replayed from HIP graphs; microseconds per step over 100 calls."""
+    from psim import scenes
+    from psim.compute import ComputeProg
+    out = {"scene": "granular box, 16384 discs, dem_step(50) x 100, re-sort off"}
+    s, st = scenes.granular_box(n_side=128, seed=3, spacing=1.9)
+    for g in (False, True):
+        prog = ComputeProg(s, st.copy(), reorder_interval=-1, graph=g, record_data=False)
+        for _ in range(4):
+            prog.step(50)
+        prog.sync()
+        t = time.perf_counter()
+        for _ in range(100):
+            prog.step(50)
+        prog.sync()
+        dt = time.perf_counter() - t
+        c = prog.counters()
+        prog.close()
+        out["graph" if g else "direct"] = {"us_per_step": dt / 5000 * 1e6, "captures": c["graph_captures"],
+                                           "replays": c["graph_replays"]}
+    out["speedup"] = out["direct"]["us_per_step"] / out["graph"]["us_per_step"]
+    return out
+
+
 def cpu_baseline(args):
     """Oracle (reference algorithm, f32) on a bounded sample: the first
     `cpu_particles` discs of the same column generator, timed over enough steps
@@ -293,6 +319,7 @@ def main():
     prog.close()
     if not args.no_serve:
         out["serving"] = serving_bench(args)
+        out["graphs"] = graph_bench(args)
     if args.copy_mib > 0:
         out["config"]["copy_calibration"] = copy_calibration(args.copy_mib)
     if not args.no_cpu:

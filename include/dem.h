@@ -98,6 +98,9 @@ typedef struct dem_params {
     int32_t reorder_interval;  /* re-sort particle storage into cell order (with a neighbour-list
                                   rebuild) every k steps; 0: only after an upload; < 0: never */
     int32_t block_size;        /* threads per block for particle kernels; 0 = 256 */
+    int32_t graph;             /* 1: capture each dem_step call's launches in a HIP graph and replay
+                                  it when a later call repeats the same launches (same step count,
+                                  buffers, rebuild and re-sort pattern); 0: launch directly */
 } dem_params;
 
 /* Caller-owned arrays in the reference host layouts (state.rs:69-93).
@@ -182,8 +185,9 @@ int dem_broken_bond_count(dem_ctx* ctx, int64_t* out);
  * [2] contact-list overflows, [3] bond-list overflows, [4] active contacts after the
  * last step, [5] active bond endpoints after the last step, [6] storage re-sorts,
  * [7] scan timeouts (always 0), [8] scatter errors, [9] particles that took the
- * stencil fallback because their neighbour list overflowed (summed over rebuilds).
- * [1], [2], [3], [8] and [9] count over every scene of the context's pools. */
+ * stencil fallback because their neighbour list overflowed (summed over rebuilds),
+ * [10] HIP graphs captured, [11] HIP graph replays.
+ * [1], [2], [3], [8], [9], [10] and [11] count over every scene of the context. */
 int dem_counters(dem_ctx* ctx, int64_t* out, int32_t n_out);
 
 /* Sum over the particles flagged in `select` (length N, nonzero = selected) of
@@ -246,6 +250,8 @@ typedef struct dem_server_params {
     int32_t max_resident;       /* <= 0: no limit beyond the pools; 1 = one scene at a time */
     int32_t bypass;             /* later requests tried when the head does not fit */
     int32_t contact_slots;      /* contact table width of results whose request gave none; 0 = none */
+    int32_t graph;              /* 1: a tick's launches run as a HIP graph, replayed when a later
+                                   tick repeats them (dem_params.graph) */
 } dem_server_params;
 
 #define DEM_REQ_QUEUED   0

@@ -64,6 +64,7 @@ int dem_create(const dem_params* params, int32_t capacity, int32_t bond_capacity
             delete c;
             return rc;
         }
+        c->eng.enable_graphs(params->graph != 0);
         *out = c;
         return DEM_OK;
     })
@@ -80,7 +81,9 @@ int dem_destroy(dem_ctx* ctx) {
 int dem_set_params(dem_ctx* ctx, const dem_params* params) {
     DEM_GUARD({
         if (!ctx || !params) { set_error("dem_set_params: NULL argument"); return DEM_EINVAL; }
-        return ctx->eng.set_params(ctx->sid, *params);
+        const int rc = ctx->eng.set_params(ctx->sid, *params);
+        if (rc == DEM_OK) ctx->eng.enable_graphs(params->graph != 0);
+        return rc;
     })
 }
 

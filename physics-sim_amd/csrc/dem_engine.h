@@ -69,6 +69,29 @@ struct SceneHost {
     int32_t since_reorder = 1 << 30;
 };
 
+// One planned launch of a step sequence.  Operations are built zero-filled (padding
+// included) so that two plans compare bytewise: a repeated plan replays its graph.
+enum OpKind { OP_STEP, OP_RB_STORE, OP_RB_BIN, OP_SCAN, OP_RB_SCATTER, OP_BUILD_NL, OP_PARTITION, OP_MOVE };
+static const char* const kOpNames[] = {"step kernel",      "rebuild: flag",  "rebuild: bin",
+                                       "rebuild: scan",    "rebuild: scatter", "rebuild: lists",
+                                       "re-sort: partition", "re-sort: move"};
+struct LaunchOp {
+    int32_t kind, timed;
+    int32_t grid, block, nlb, ncells, value, f64, bonds;
+    int32_t* rb;
+    const int32_t* gate;
+    StepArgs sa;
+    ReorderArgs ra;
+    DevArrays da;
+};
+
+struct GraphEntry {
+    std::vector<LaunchOp> ops;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t used = 0;
+};
+
 class Engine {
 public:
     Engine() = default;
@@ -102,6 +125,13 @@ public:
 
     // timing of the last step() call
     void enable_timing(bool on) { timing_ = on; }
+    // capture step sequences in HIP graphs and replay them when a sequence repeats
+    void enable_graphs(bool on) {
+        if (!on) clear_graphs();
+        use_graph_ = on;
+    }
+    int64_t graph_captures() const { return graph_captures_; }
+    int64_t graph_replays() const { return graph_replays_; }
     double t_total = 0, t_step = 0, t_scan = 0, t_scatter = 0, t_reorder = 0;
     int64_t n_step = 0, n_scan = 0, n_scatter = 0, n_reorder = 0;
 
@@ -117,7 +147,11 @@ private:
     DevArrays arrays(int set) const;
     StepArgs make_args(const int2* lf) const;
     int write_scene_table();
-    int reorder(const int2* lf_dev);
+    void plan_reorder(const int2* lf_dev, std::vector<LaunchOp>& ops);
+    static LaunchOp& new_op(std::vector<LaunchOp>& ops, int kind, int timed);
+    hipError_t run_op(const LaunchOp& op);
+    int run_ops(const std::vector<LaunchOp>& ops);
+    void clear_graphs();
     int rebuild_chain(const int2* lf_dev, bool resort);
     float skin_of(const SceneHost& sh) const;
     int launch_grid_blocks() const;   // blocks covering the whole slot pool
@@ -168,6 +202,13 @@ private:
     std::vector<SceneHost> scenes_;
     std::vector<std::pair<int64_t, int64_t>> used_slots_, used_cells_, used_bonds_, used_recs_, used_mats_;
     bool timing_ = false;
+    // graphs of step sequences (at most kGraphCache, least recently used dropped)
+    static constexpr size_t kGraphMaxOps = 4096;
+    static constexpr size_t kGraphCache = 32;
+    bool use_graph_ = false;
+    std::vector<GraphEntry> graphs_;
+    uint64_t graph_clock_ = 0;
+    int64_t graph_captures_ = 0, graph_replays_ = 0;
     std::vector<hipEvent_t> ev_pool_;
     bool table_dirty_ = true;
 };

@@ -73,6 +73,7 @@ Engine::~Engine() {
     hipFree(STAT_); hipFree(TSUM_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
     hipFree(d_ke_); hipFree(d_cnt_);
     if (stage_) hipHostFree(stage_);
+    clear_graphs();
     for (auto e : ev_pool_) hipEventDestroy(e);
     if (stream_) hipStreamDestroy(stream_);
 }
@@ -659,12 +660,11 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
 }
 
 // ---------------------------------------------------------------------------
-int Engine::reorder(const int2* lf_dev) {
+void Engine::plan_reorder(const int2* lf_dev, std::vector<LaunchOp>& ops) {
     // Every live scene, binned by the rebuild this re-sort belongs to and with its
     // neighbour lists just built, moves into cell order, each block of slots then
     // partitioned stably into particles with list candidates first (the order inside
     // a cell is the binning's; results depend on neither).
-    hipError_t e = hipSuccess;
     ReorderArgs r{};
     r.scenes = d_scenes_;
     r.blk_scene = d_blk_scene_;
@@ -680,10 +680,14 @@ int Engine::reorder(const int2* lf_dev) {
     for (const SceneHost& sh : scenes_)
         if (sh.live && sh.has_ext) r.with_ext = 1;  // EXT is read only where PF_EXT is set
     r.with_data = with_data_ ? 1 : 0;
-    if (e == hipSuccess) e = launch_partition(r, active_blocks(), block_, stream_);
-    if (e == hipSuccess) e = launch_reorder_move(r, active_blocks(), block_, stream_);
-    int rc = check(e, "reorder");
-    if (rc) return rc;
+    LaunchOp& a = new_op(ops, OP_PARTITION, 3);
+    a.ra = r;
+    a.nlb = active_blocks();
+    a.block = block_;
+    LaunchOp& b = new_op(ops, OP_MOVE, 3);
+    b.ra = r;
+    b.nlb = active_blocks();
+    b.block = block_;
     set_ ^= 1;
     lcur_ ^= 1;
     for (SceneHost& sh : scenes_) {
@@ -691,7 +695,127 @@ int Engine::reorder(const int2* lf_dev) {
         sh.since_reorder = 0;
         sh.reorders++;
     }
-    return DEM_OK;
+}
+
+LaunchOp& Engine::new_op(std::vector<LaunchOp>& ops, int kind, int timed) {
+    ops.emplace_back();
+    LaunchOp& op = ops.back();
+    std::memset(static_cast<void*>(&op), 0, sizeof(LaunchOp));   // padding too: ops compare bytewise
+    op.kind = kind;
+    op.timed = timed;
+    return op;
+}
+
+hipError_t Engine::run_op(const LaunchOp& op) {
+    switch (op.kind) {
+        case OP_STEP: return launch_step(op.sa, op.nlb, op.block, op.f64 != 0, op.bonds != 0, stream_);
+        case OP_RB_STORE: return launch_rb_store(op.rb, op.value, stream_);
+        case OP_RB_BIN: return launch_rb_bin(op.sa, op.nlb, op.grid, op.block, stream_);
+        case OP_SCAN: return launch_scan(op.da, op.ncells, op.gate, stream_);
+        case OP_RB_SCATTER: return launch_rb_scatter(op.sa, op.nlb, op.grid, op.block, stream_);
+        case OP_BUILD_NL: return launch_build_nl(op.sa, op.nlb, op.grid, op.block, stream_);
+        case OP_PARTITION: return launch_partition(op.ra, op.nlb, op.block, stream_);
+        case OP_MOVE: return launch_reorder_move(op.ra, op.nlb, op.block, stream_);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+static bool same_ops(const std::vector<LaunchOp>& a, const std::vector<LaunchOp>& b) {
+    return a.size() == b.size() && std::memcmp(a.data(), b.data(), sizeof(LaunchOp) * a.size()) == 0;
+}
+
+void Engine::clear_graphs() {
+    for (GraphEntry& g : graphs_) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        if (g.graph) hipGraphDestroy(g.graph);
+    }
+    graphs_.clear();
+}
+
+// Runs the planned launches: eagerly (timing, graphs off, long sequences), or as a HIP
+// graph, captured the first time a sequence occurs and replayed when it repeats.
+int Engine::run_ops(const std::vector<LaunchOp>& ops) {
+    const bool graph = use_graph_ && !timing_ && ops.size() <= kGraphMaxOps;
+    if (!graph) {
+        std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
+        size_t ev_used = 0;
+        auto ev = [&]() -> hipEvent_t {
+            if (ev_used >= ev_pool_.size()) {
+                hipEvent_t x;
+                hipEventCreate(&x);
+                ev_pool_.push_back(x);
+            }
+            return ev_pool_[ev_used++];
+        };
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (timing_) { t0 = ev(); t1 = ev(); hipEventRecord(t0, stream_); }
+        for (const LaunchOp& op : ops) {
+            hipEvent_t a = nullptr, b = nullptr;
+            if (timing_) { a = ev(); b = ev(); hipEventRecord(a, stream_); }
+            const hipError_t e = run_op(op);
+            if (timing_) { hipEventRecord(b, stream_); evs.push_back({op.timed, {a, b}}); }
+            if (int rc = check(e, kOpNames[op.kind])) return rc;
+        }
+        if (timing_) {
+            hipEventRecord(t1, stream_);
+            hipEventSynchronize(t1);
+            float ms = 0;
+            hipEventElapsedTime(&ms, t0, t1);
+            t_total = ms;
+            t_step = t_scan = t_scatter = t_reorder = 0;
+            n_step = n_scan = n_scatter = n_reorder = 0;
+            for (auto& x : evs) {
+                float m = 0;
+                hipEventElapsedTime(&m, x.second.first, x.second.second);
+                switch (x.first) {
+                    case 0: t_step += m; n_step++; break;
+                    case 1: t_scan += m; n_scan++; break;
+                    case 2: t_scatter += m; n_scatter++; break;
+                    default: t_reorder += m; n_reorder++; break;
+                }
+            }
+        }
+        return DEM_OK;
+    }
+    GraphEntry* hit = nullptr;
+    for (GraphEntry& g : graphs_)
+        if (same_ops(g.ops, ops)) hit = &g;
+    if (!hit) {
+        if (graphs_.size() >= kGraphCache) {
+            // drop the least recently used
+            size_t lru = 0;
+            for (size_t i = 1; i < graphs_.size(); i++)
+                if (graphs_[i].used < graphs_[lru].used) lru = i;
+            hipGraphExecDestroy(graphs_[lru].exec);
+            hipGraphDestroy(graphs_[lru].graph);
+            graphs_.erase(graphs_.begin() + (long)lru);
+        }
+        GraphEntry g;
+        g.ops = ops;
+        hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+        if (int rc = check(e, "graph capture")) return rc;
+        hipError_t le = hipSuccess;
+        for (const LaunchOp& op : ops)
+            if (le == hipSuccess) le = run_op(op);
+        e = hipStreamEndCapture(stream_, &g.graph);
+        if (int rc = check(le, "graph capture: launch")) {
+            if (g.graph) hipGraphDestroy(g.graph);
+            return rc;
+        }
+        if (int rc = check(e, "graph capture: end")) return rc;
+        e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+        if (int rc = check(e, "graph instantiate")) {
+            hipGraphDestroy(g.graph);
+            return rc;
+        }
+        graphs_.push_back(std::move(g));
+        hit = &graphs_.back();
+        graph_captures_++;
+    } else {
+        graph_replays_++;
+    }
+    hit->used = ++graph_clock_;
+    return check(hipGraphLaunch(hit->exec, stream_), "graph launch");
 }
 
 int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& nsteps) {
@@ -757,41 +881,23 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
         any_bonds |= scenes_[sid].nrec > 0;
     }
     const int nblk = active_blocks();
-    // timing
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
-    size_t ev_used = 0;
-    auto ev = [&]() -> hipEvent_t {
-        if (ev_used >= ev_pool_.size()) {
-            hipEvent_t x;
-            hipEventCreate(&x);
-            ev_pool_.push_back(x);
-        }
-        return ev_pool_[ev_used++];
-    };
-    auto timed = [&](int kind, auto&& fn) -> hipError_t {
-        if (!timing_) return fn();
-        hipEvent_t a = ev(), b = ev();
-        hipEventRecord(a, stream_);
-        hipError_t r = fn();
-        hipEventRecord(b, stream_);
-        evs.push_back({kind, {a, b}});
-        return r;
-    };
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    if (timing_) { t0 = ev(); t1 = ev(); hipEventRecord(t0, stream_); }
-
     // the neighbour-list rebuild kernels stride over the slot blocks with one round of
     // workgroups (32 waves per CU), so that a launch whose rebuild flag is clear is cheap
     const int rb_grid = std::max(1, std::min(nblk, cus_ * std::max(1, 2048 / block_)));
+    std::vector<LaunchOp> ops;
+    ops.reserve((size_t)nlaunch * 6);
     for (int64_t k = 0; k < nlaunch; k++) {
         const int2* lfk = d_lf_ + k * scene_cap_;
         bool lom = false;
         for (size_t i = 0; i < sids.size(); i++)
             if (lf[(size_t)k * scene_cap_ + sids[i]].x & SF_LOM) lom = true;
-        StepArgs args = make_args(lfk);
-        args.flip = lom ? 1 : 0;
-        e = timed(0, [&]() { return launch_step(args, nblk, block_, any_f64, any_bonds, stream_); });
-        if ((rc = check(e, "step kernel"))) return rc;
+        LaunchOp& st = new_op(ops, OP_STEP, 0);
+        st.sa = make_args(lfk);
+        st.sa.flip = lom ? 1 : 0;
+        st.nlb = nblk;
+        st.block = block_;
+        st.f64 = any_f64 ? 1 : 0;
+        st.bonds = any_bonds ? 1 : 0;
         lcur_ ^= 1;  // the step kernel wrote every live slot's next contact list
         // some LOM half wrote the next positions; every other slot carried its own over
         if (!lom) continue;
@@ -800,55 +906,46 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
         // rebuild flag is set: by the step kernel (a particle moved more than 0.45
         // skin since the last build) or here (new state, new parameters, re-sort due).
         // A due re-sort moves every live scene into the order of the fresh cell list.
+        // With graphs a due re-sort waits for the first drift of a call, so that calls
+        // repeat a few launch patterns (results do not depend on when re-sorts happen).
+        const bool may_sort = k == 0 || !(use_graph_ && !timing_);
         bool due = false;
         for (size_t i = 0; i < sids.size(); i++) {
             if (!(lf[(size_t)k * scene_cap_ + sids[i]].x & SF_LOM)) continue;
             SceneHost& sh = scenes_[sids[i]];
             const int32_t R = sh.params.reorder_interval;
-            if (R > 0 ? sh.since_reorder >= R : (R == 0 && sh.since_reorder >= (1 << 30))) due = true;
+            if (may_sort && (R > 0 ? sh.since_reorder >= R : (R == 0 && sh.since_reorder >= (1 << 30)))) due = true;
             sh.since_reorder++;
         }
-        const bool force = rb_force_ || due;
-        StepArgs ra = make_args(lfk);
-        int32_t* rb = RB_ + cur_;   // this drift's flag (the next step kernel clears it)
-        e = timed(2, [&]() {
-            hipError_t r = force ? launch_rb_store(rb, 1, stream_) : hipSuccess;
-            return r == hipSuccess ? launch_rb_bin(ra, nblk, rb_grid, block_, stream_) : r;
-        });
-        if ((rc = check(e, "rebuild: bin"))) return rc;
-        e = timed(1, [&]() { return launch_scan(arrays(set_), (int)hw_cells_, rb, stream_); });
-        if ((rc = check(e, "rebuild: scan"))) return rc;
-        e = timed(2, [&]() { return launch_rb_scatter(ra, nblk, rb_grid, block_, stream_); });
-        if ((rc = check(e, "rebuild: scatter"))) return rc;
-        e = timed(2, [&]() { return launch_build_nl(ra, nblk, rb_grid, block_, stream_); });
-        if ((rc = check(e, "rebuild: lists"))) return rc;
-        if (due) {
-            // the lists move with their particles
-            e = timed(3, [&]() { return reorder(lfk) == DEM_OK ? hipSuccess : hipErrorUnknown; });
-            if (e != hipSuccess) return DEM_EDEVICE;
+        const StepArgs ra = make_args(lfk);
+        int32_t* rb = RB_ + cur_;   // this drift's flag (the next chain's first kernel clears the other)
+        if (rb_force_ || due) {
+            LaunchOp& o = new_op(ops, OP_RB_STORE, 2);
+            o.rb = rb;
+            o.value = 1;
         }
-        rb_force_ = false;
-    }
-    if (timing_) {
-        hipEventRecord(t1, stream_);
-        hipEventSynchronize(t1);
-        float ms = 0;
-        hipEventElapsedTime(&ms, t0, t1);
-        t_total = ms;
-        t_step = t_scan = t_scatter = t_reorder = 0;
-        n_step = n_scan = n_scatter = n_reorder = 0;
-        for (auto& x : evs) {
-            float m = 0;
-            hipEventElapsedTime(&m, x.second.first, x.second.second);
-            switch (x.first) {
-                case 0: t_step += m; n_step++; break;
-                case 1: t_scan += m; n_scan++; break;
-                case 2: t_scatter += m; n_scatter++; break;
-                default: t_reorder += m; n_reorder++; break;
+        const int chain[3] = {OP_RB_BIN, OP_SCAN, OP_RB_SCATTER};
+        for (int kind : chain) {
+            LaunchOp& o = new_op(ops, kind, kind == OP_SCAN ? 1 : 2);
+            o.sa = ra;
+            o.nlb = nblk;
+            o.grid = rb_grid;
+            o.block = block_;
+            if (kind == OP_SCAN) {
+                o.da = arrays(set_);
+                o.ncells = (int)hw_cells_;
+                o.gate = rb;
             }
         }
+        LaunchOp& bl = new_op(ops, OP_BUILD_NL, 2);
+        bl.sa = ra;
+        bl.nlb = nblk;
+        bl.grid = rb_grid;
+        bl.block = block_;
+        if (due) plan_reorder(lfk, ops);   // the lists move with their particles
+        rb_force_ = false;
     }
-    return DEM_OK;
+    return run_ops(ops);
 }
 
 int Engine::sync() { return check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
@@ -1040,9 +1137,10 @@ int Engine::counters(int32_t sid, int64_t* out, int32_t n_out) {
     int rc = check(e, "counters");
     if (rc) return rc;
     for (int32_t q = 0; q < sh.nrec; q++) nrec_alive += BR[q].w != 0;
-    const int64_t vals[10] = {sh.steps, (int64_t)st[4], (int64_t)st[0], (int64_t)st[1], (int64_t)nc, nrec_alive,
-                              sh.reorders, (int64_t)st[2], (int64_t)st[3], (int64_t)st[5]};
-    for (int32_t i = 0; i < n_out && i < 10; i++) out[i] = vals[i];
+    const int64_t vals[12] = {sh.steps, (int64_t)st[4], (int64_t)st[0], (int64_t)st[1], (int64_t)nc, nrec_alive,
+                              sh.reorders, (int64_t)st[2], (int64_t)st[3], (int64_t)st[5], graph_captures_,
+                              graph_replays_};
+    for (int32_t i = 0; i < n_out && i < 12; i++) out[i] = vals[i];
     return DEM_OK;
 }
 

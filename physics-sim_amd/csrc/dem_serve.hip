@@ -205,6 +205,7 @@ int dem_server_create(const dem_server_params* p, dem_server** out) {
             delete s;
             return rc;
         }
+        s->eng.enable_graphs(p->graph != 0);
         *out = s;
         return DEM_OK;
     })

@@ -55,6 +55,7 @@ dem_params ComputeProg::params(const Settings& s) {
     p.record_data = opt_.record_data ? 1 : 0;
     p.reorder_interval = opt_.reorder_interval;
     p.block_size = opt_.block_size;
+    p.graph = opt_.graph ? 1 : 0;
     return p;
 }
 

@@ -150,6 +150,7 @@ public:
         int32_t block_size = 256;
         float cell_size = 0.0f;      // 0: just above 2 r_max (default_grid)
         int use_f64 = -1;            // -1: settings.simulation.use_f64
+        bool graph = false;          // replay repeated dem_step launch sequences as HIP graphs
     };
 
     // WGPUComputeProg::new (buffers) + restore

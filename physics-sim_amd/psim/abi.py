@@ -47,7 +47,7 @@ class DemGrid(C.Structure):
 class DemParams(C.Structure):
     _fields_ = [("u", DemUniform), ("materials", C.POINTER(DemMaterial)), ("n_materials", C.c_int32),
                 ("grid", DemGrid), ("use_f64", C.c_int32), ("record_data", C.c_int32),
-                ("reorder_interval", C.c_int32), ("block_size", C.c_int32)]
+                ("reorder_interval", C.c_int32), ("block_size", C.c_int32), ("graph", C.c_int32)]
 
 
 FP = C.POINTER(C.c_float)
@@ -65,7 +65,8 @@ class DemServerParams(C.Structure):
     _fields_ = [("slot_capacity", C.c_int64), ("cell_capacity", C.c_int64), ("bond_capacity", C.c_int64),
                 ("record_capacity", C.c_int64), ("scene_capacity", C.c_int32), ("material_capacity", C.c_int32),
                 ("block_size", C.c_int32), ("record_data", C.c_int32), ("substeps", C.c_int32),
-                ("max_resident", C.c_int32), ("bypass", C.c_int32), ("contact_slots", C.c_int32)]
+                ("max_resident", C.c_int32), ("bypass", C.c_int32), ("contact_slots", C.c_int32),
+                ("graph", C.c_int32)]
 
 
 DEM_REQ_QUEUED, DEM_REQ_RESIDENT, DEM_REQ_DONE = 0, 1, 2

@@ -31,7 +31,7 @@ class ComputeProg:
     def __init__(self, settings: Settings, state: State, capacity: Optional[int] = None,
                  bond_capacity: Optional[int] = None, grid: Optional[abi.DemGrid] = None,
                  use_f64: Optional[bool] = None, record_data: bool = True, reorder_interval: int = 64,
-                 block_size: int = 256):
+                 block_size: int = 256, graph: bool = False):
         self.lib = abi.load_library()
         if self.lib.dem_device_count() <= 0:
             raise abi.DemError(abi.DEM_EDEVICE, "no HIP device visible")
@@ -40,6 +40,7 @@ class ComputeProg:
         self.record_data = record_data
         self.reorder_interval = reorder_interval
         self.block_size = block_size
+        self.graph = bool(graph)
         self.grid = grid if grid is not None else self.default_grid(settings, state)
         self.capacity = int(capacity if capacity is not None else max(state.p_count, 1))
         nb = state.n_bonds
@@ -81,6 +82,7 @@ class ComputeProg:
         p.record_data = int(self.record_data)
         p.reorder_interval = int(self.reorder_interval)
         p.block_size = int(self.block_size)
+        p.graph = int(self.graph)
         return p
 
     def set_params(self, settings: Settings = None):
@@ -129,10 +131,10 @@ class ComputeProg:
         return out.value
 
     def counters(self) -> dict:
-        out = (C.c_int64 * 10)()
-        abi.check(self.lib.dem_counters(self.ctx, out, 10))
+        out = (C.c_int64 * 12)()
+        abi.check(self.lib.dem_counters(self.ctx, out, 12))
         keys = ["steps", "broad_phases", "contact_overflow", "bond_overflow", "contacts", "bond_endpoints",
-                "reorders", "scan_timeouts", "scatter_errors", "list_overflow"]
+                "reorders", "scan_timeouts", "scatter_errors", "list_overflow", "graph_captures", "graph_replays"]
         return {k: int(out[i]) for i, k in enumerate(keys)}
 
     def datum(self, selections) -> Optional[np.ndarray]:

@@ -48,7 +48,7 @@ def footprint(state: State, grid: abi.DemGrid, block_size: int = 256):
 class Server:
     def __init__(self, slot_capacity: int, cell_capacity: int, bond_capacity: int = 0, scene_capacity: int = 64,
                  substeps: int = 50, max_resident: int = 0, bypass: int = 8, record_data: bool = True,
-                 block_size: int = 256, contact_slots: int = 14, record_capacity: int = 0):
+                 block_size: int = 256, contact_slots: int = 14, record_capacity: int = 0, graph: bool = False):
         self.lib = abi.load_library()
         if self.lib.dem_device_count() <= 0:
             raise abi.DemError(abi.DEM_EDEVICE, "no HIP device visible")
@@ -58,6 +58,7 @@ class Server:
         sp.scene_capacity, sp.material_capacity = int(scene_capacity), 0
         sp.block_size, sp.record_data, sp.substeps = int(block_size), int(record_data), int(substeps)
         sp.max_resident, sp.bypass, sp.contact_slots = int(max_resident), int(bypass), int(contact_slots)
+        sp.graph = int(graph)
         self.params = sp
         self.block_size = int(block_size)
         self.record_data = bool(record_data)

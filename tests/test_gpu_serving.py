@@ -57,10 +57,14 @@ def solo(s, st, n):
     return got
 
 
-def test_batched_equals_solo_with_continuous_arrivals():
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_equals_solo_with_continuous_arrivals(graph):
+    """Scenes arriving while others run reproduce their solo runs; with `graph` the
+    ticks' launches run as HIP graphs (captured once per launch pattern, replayed after)."""
     sc = mixed_scenes()
     want = [solo(s, st, n) for _, s, st, n in sc]
-    srv = Server(slot_capacity=8192, cell_capacity=1 << 16, bond_capacity=64, scene_capacity=16, substeps=7)
+    srv = Server(slot_capacity=8192, cell_capacity=1 << 16, bond_capacity=64, scene_capacity=16, substeps=7,
+                 graph=graph)
     tickets, outs = [], []
     # two scenes start; the others arrive one per tick while those run
     for i in range(len(sc)):
