@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--serve-scenes", type=int, default=96)
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, scenes/s")
     ap.add_argument("--serve-substeps", type=int, default=50)
+    ap.add_argument("--no-bonded", action="store_true", help="skip the 1M-disc bonded-assembly leg")
+    ap.add_argument("--bonded-steps", type=int, default=100)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also time a plain device copy of this many MiB (the reference kernel for the HBM "
                          "traffic counters and the achievable copy bandwidth)")
@@ -166,6 +168,48 @@ def serving_bench(args):
                                              out["plain"]["saturated"]["scenes_per_s"])
     out["p99_ratio_poisson_plain_over_batched"] = out["plain"]["poisson"]["p99_s"] / out["batched"]["poisson"]["p99_s"]
     return out
+
+
+def bonded_bench(args):
+    """The bonded-particle assembly at 1,048,576 discs (a 1024 x 1024 lattice at spacing
+    2.0, r ~ U[0.95, 1.05]; every pair closer than 1.05 (r_a + r_b) bonded by the device
+    bond builder, parallel bonds with Weibull(k = 4, 30) strengths; bottom and top rows
+    gripped and pulled apart): bond-builder time, then particle-steps/s of the bonded
+    step over --bonded-steps steps."""
+    from psim import scenes
+    from psim.compute import ComputeProg
+    s, st, bottom, top = scenes.bonded_assembly(n_side=1024, pull_v=0.02)
+    n = st.p_count
+    prog = ComputeProg(s, st, bond_capacity=6 * n, record_data=False, reorder_interval=args.reorder)
+    t = time.perf_counter()
+    built = prog.regen_bonds(s)
+    t_build = time.perf_counter() - t
+    built.bond_strengths = scenes.assembly_strengths(built.n_bonds)
+    prog.restore(built)
+    prog.step(10)
+    prog.sync()
+    t = time.perf_counter()
+    prog.step(args.bonded_steps)
+    prog.sync()
+    wall = time.perf_counter() - t
+    prog.timing(True)
+    prog.step(args.bonded_steps)
+    prog.sync()
+    tm = prog.timing(False)
+    c = prog.counters()
+    broken = prog.broken_bond_count()
+    prog.close()
+    nc = c["contacts"] / n
+    nb_end = 2.0 * built.n_bonds / n
+    k_ms = tm["step_ms"] / max(tm["n_step"], 1)
+    k_bytes = n * (60 + 16 * nc + 32 * nb_end)
+    return {"workload": "bonded assembly, %d discs, %d bonds (parallel bonds, Weibull(4, 30) strengths), "
+                        "rows gripped and pulled at 0.02 units/s, dt 1e-4" % (n, built.n_bonds),
+            "value": n * args.bonded_steps / wall, "unit": "particle-steps/s", "steps": args.bonded_steps,
+            "ms_per_step": 1e3 * wall / args.bonded_steps, "bond_builder_s": t_build,
+            "bond_endpoints_per_particle": nb_end, "mean_contacts_per_particle": nc,
+            "kernel_ms": {"step": k_ms}, "step_kernel_alg_gbs": k_bytes / (k_ms * 1e-3) / 1e9,
+            "broken_bonds": int(broken)}
 
 
 def graph_bench(args):
@@ -317,6 +361,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args, min_cell)},
     }
     prog.close()
+    if not args.no_bonded:
+        out["bonded"] = bonded_bench(args)
+        print("bench: bonded assembly done", file=sys.stderr, flush=True)
     if not args.no_serve:
         out["serving"] = serving_bench(args)
         out["graphs"] = graph_bench(args)

@@ -115,6 +115,12 @@ struct DevArrays {
     Pool<float> BL;         // per bond rest length
     Pool<float2> BSTR;      // per bond {normal, shear} strength
     Pool<const dem_material> MAT;
+    // bond pass: owner of each record (global particle id, -1: none) and its results
+    Pool<int32_t> RO;
+    Pool<int32_t> RS, RP;   // slots of each record's owner (-1: none) and partner
+    Pool<int2> BRS;         // [cap] per slot: its record range (moves with the particle)
+    Pool<float4> RF;        // {fx, fy, moment, alive}
+    Pool<float4> RD;        // [2 per record] data[0..2] updates (with data recording)
     // broad phase
     Pool<int32_t> KEY;      // cell of each slot (global cell index)
     Pool<int32_t> RANK;     // rank inside the cell from the counting atomics

@@ -28,6 +28,7 @@ struct DevSet {
     int32_t* CC = nullptr;   // contact count (written only when it changes)
     int32_t* NL = nullptr;   // neighbour lists [NLMAX][cap], row 0 packs the count
     float2* XR = nullptr;    // positions at the list build
+    int2* BRS = nullptr;     // bond record range of each slot
 };
 
 struct ContactLists {
@@ -71,10 +72,12 @@ struct SceneHost {
 
 // One planned launch of a step sequence.  Operations are built zero-filled (padding
 // included) so that two plans compare bytewise: a repeated plan replays its graph.
-enum OpKind { OP_STEP, OP_RB_STORE, OP_RB_BIN, OP_SCAN, OP_RB_SCATTER, OP_BUILD_NL, OP_PARTITION, OP_MOVE };
+enum OpKind { OP_STEP, OP_RB_STORE, OP_RB_BIN, OP_SCAN, OP_RB_SCATTER, OP_BUILD_NL, OP_PARTITION, OP_MOVE, OP_BONDS,
+              OP_RSLOTS };
 static const char* const kOpNames[] = {"step kernel",      "rebuild: flag",  "rebuild: bin",
                                        "rebuild: scan",    "rebuild: scatter", "rebuild: lists",
-                                       "re-sort: partition", "re-sort: move"};
+                                       "re-sort: partition", "re-sort: move", "bond pass",
+                                       "re-sort: record slots"};
 struct LaunchOp {
     int32_t kind, timed;
     int32_t grid, block, nlb, ncells, value, f64, bonds;
@@ -188,6 +191,10 @@ private:
     dem_material* MAT_ = nullptr;
     int32_t *KEY_ = nullptr, *RANK_ = nullptr, *CNT_ = nullptr, *START_ = nullptr, *SIDX_ = nullptr, *INV_ = nullptr;
     int32_t* RB_ = nullptr;
+    int32_t* RO_ = nullptr;   // bond pass: record owners, their slots and results
+    int32_t *RS_ = nullptr, *RP_ = nullptr;
+    float4 *RF_ = nullptr, *RD_ = nullptr;
+    int64_t hw_recs_ = 0;     // end of the highest live record region
     int32_t *ORDER_ = nullptr, *NEWSLOT_ = nullptr;   // re-sort permutation
     bool rb_force_ = true;        // rebuild the neighbour lists after the next drift
     unsigned long long* STAT_ = nullptr;

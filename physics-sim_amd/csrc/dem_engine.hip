@@ -21,6 +21,8 @@ hipError_t launch_scatter(const StepArgs& args, int nblocks, int block, hipStrea
 hipError_t launch_cell_sort(const StepArgs& args, int cell_lo, int cell_hi, int p0, hipStream_t st);
 hipError_t launch_reorder_move(const ReorderArgs& r, int nblocks, int block, hipStream_t st);
 hipError_t launch_partition(const ReorderArgs& r, int nblocks, int block, hipStream_t st);
+hipError_t launch_bond_forces(const StepArgs& args, int nrec, int lbs, bool f64, hipStream_t st);
+hipError_t launch_record_slots(const StepArgs& args, int nrec, int lbs, hipStream_t st);
 hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, int np, double* out, hipStream_t st);
 hipError_t launch_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out, hipStream_t st);
 hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
@@ -65,11 +67,12 @@ Engine::~Engine() {
         hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.D[0]); hipFree(s.D[1]);
         hipFree(s.V); hipFree(s.A); hipFree(s.M); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
     }
-    for (auto& s : sets_) { hipFree(s.CC); hipFree(s.NL); hipFree(s.XR); }
+    for (auto& s : sets_) { hipFree(s.CC); hipFree(s.NL); hipFree(s.XR); hipFree(s.BRS); }
     for (auto& l : lists_) { hipFree(l.CP); hipFree(l.CF); }
     hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BL_); hipFree(BSTR_); hipFree(MAT_);
     hipFree(KEY_); hipFree(RANK_); hipFree(CNT_); hipFree(START_); hipFree(SIDX_); hipFree(INV_);
-    hipFree(RB_); hipFree(ORDER_); hipFree(NEWSLOT_);
+    hipFree(RB_); hipFree(ORDER_); hipFree(NEWSLOT_); hipFree(RO_); hipFree(RF_); hipFree(RD_);
+    hipFree(RS_); hipFree(RP_);
     hipFree(STAT_); hipFree(TSUM_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
     hipFree(d_ke_); hipFree(d_cnt_);
     if (stage_) hipHostFree(stage_);
@@ -119,7 +122,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
         e = e ? e : dalloc(&s.D[0], S); e = e ? e : dalloc(&s.D[1], S);
         e = e ? e : dalloc(&s.V, S); e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S);
         e = e ? e : dalloc(&s.EXT, S); e = e ? e : dalloc(&s.CC, S);
-        e = e ? e : dalloc(&s.NL, S * NLMAX); e = e ? e : dalloc(&s.XR, S);
+        e = e ? e : dalloc(&s.NL, S * NLMAX); e = e ? e : dalloc(&s.XR, S); e = e ? e : dalloc(&s.BRS, S);
         if (with_data) { e = e ? e : dalloc(&s.DATA0, S); e = e ? e : dalloc(&s.DATA1, S); }
         if ((rc = check(e, "hipMalloc particle pools"))) return rc;
     }
@@ -130,6 +133,9 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     }
     hipError_t e = hipSuccess;
     e = e ? e : dalloc(&BOFF_, S); e = e ? e : dalloc(&BR_, rec_cap_); e = e ? e : dalloc(&BF_, rec_cap_);
+    e = e ? e : dalloc(&RO_, rec_cap_); e = e ? e : dalloc(&RF_, rec_cap_);
+    e = e ? e : dalloc(&RS_, rec_cap_); e = e ? e : dalloc(&RP_, rec_cap_);
+    e = e ? e : dalloc(&RD_, with_data ? 2 * rec_cap_ : 2);
     e = e ? e : dalloc(&BS_, bond_cap_); e = e ? e : dalloc(&BL_, bond_cap_); e = e ? e : dalloc(&BSTR_, bond_cap_);
     e = e ? e : dalloc(&MAT_, mat_cap_);
     e = e ? e : dalloc(&KEY_, S); e = e ? e : dalloc(&RANK_, S); e = e ? e : dalloc(&CNT_, cell_cap_ + 1);
@@ -157,6 +163,10 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = e ? e : hipMemsetAsync(ORDER_, 0, sizeof(int32_t) * S, stream_);
     e = e ? e : hipMemsetAsync(NEWSLOT_, 0, sizeof(int32_t) * S, stream_);
     e = e ? e : hipMemsetAsync(RB_, 0, 2 * sizeof(int32_t), stream_);
+    e = e ? e : hipMemsetAsync(RO_, 0xff, sizeof(int32_t) * rec_cap_, stream_);   // no records
+    e = e ? e : hipMemsetAsync(RS_, 0xff, sizeof(int32_t) * rec_cap_, stream_);
+    e = e ? e : hipMemsetAsync(RP_, 0, sizeof(int32_t) * rec_cap_, stream_);
+    for (auto& st : sets_) e = e ? e : hipMemsetAsync(st.BRS, 0, sizeof(int2) * S, stream_);
     for (auto& st : sets_) e = e ? e : hipMemsetAsync(st.NL, 0, sizeof(int32_t) * S, stream_);   // empty lists
     e = e ? e : hipMemsetAsync(BOFF_, 0, sizeof(int2) * S, stream_);
     for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int2) * S, stream_);
@@ -195,10 +205,12 @@ int Engine::active_blocks() const {
 void Engine::update_high_water() {
     hw_slots_ = 0;
     hw_cells_ = 0;
+    hw_recs_ = 0;
     for (const SceneHost& sh : scenes_) {
         if (!sh.live) continue;
         hw_slots_ = std::max<int64_t>(hw_slots_, sh.p0 + sh.slot_cap);
         hw_cells_ = std::max<int64_t>(hw_cells_, sh.c0 + sh.cell_cap);
+        if (sh.rec_cap) hw_recs_ = std::max<int64_t>(hw_recs_, sh.r0 + sh.rec_cap);
     }
 }
 
@@ -317,7 +329,13 @@ int Engine::remove_scene(int32_t sid) {
     release_region(used_slots_, sh.p0);
     release_region(used_cells_, sh.c0);
     if (sh.bond_cap) release_region(used_bonds_, sh.b0);
-    if (sh.rec_cap) release_region(used_recs_, sh.r0);
+    if (sh.rec_cap) {
+        release_region(used_recs_, sh.r0);
+        // the bond pass skips the region until a scene uploads records into it
+        hipError_t e = hipMemsetAsync(RO_ + sh.r0, 0xff, sizeof(int32_t) * sh.rec_cap, stream_);
+        if (e == hipSuccess) e = hipMemsetAsync(RS_ + sh.r0, 0xff, sizeof(int32_t) * sh.rec_cap, stream_);
+        if (int rc = check(e, "remove scene")) return rc;
+    }
     release_region(used_mats_, sh.m0);
     sh = SceneHost();
     update_high_water();
@@ -423,6 +441,7 @@ DevArrays Engine::arrays(int set) const {
     a.BOFF = BOFF_; a.BR = BR_; a.BF = BF_; a.BS = BS_; a.BL = BL_; a.BSTR = BSTR_; a.MAT = MAT_;
     a.KEY = KEY_; a.RANK = RANK_; a.CNT = CNT_; a.START = START_; a.SIDX = SIDX_; a.INV = INV_;
     a.NL = s.NL; a.XR = s.XR; a.RB = RB_;
+    a.RO = RO_; a.RF = RF_; a.RD = RD_; a.RS = RS_; a.RP = RP_; a.BRS = s.BRS;
     a.STAT = STAT_; a.TSUM = TSUM_; a.ntiles = ntiles_;
     return a;
 }
@@ -564,6 +583,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     sh.has_strength = v.bond_strengths != nullptr;
     // contact table -> contact lists (plain contacts) and bond records (everything else)
     std::vector<int32_t> BOFF(np + 1, 0);
+    std::vector<int32_t> RO;   // owner (global particle id) of each record
     std::vector<int4> BR;
     std::vector<float4> BF;
     if (v.contacts) {
@@ -596,6 +616,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
                 CF[(size_t)q * np + i] = s.ft;
             }
             BOFF[i] = (int32_t)BR.size();
+            for (size_t k = 0; k < rec.size(); k++) RO.push_back((int32_t)sh.p0 + i);
             for (auto& rr : rec) {
                 const Slot& s = tab[(size_t)i * K + rr.second];
                 int32_t bid = s.bonded >= 0 ? s.bonded : -1;
@@ -641,10 +662,14 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     h2d(INV_ + p0, INV, sizeof(int32_t) * np);
     h2d(SIDX_ + p0, SIDX, sizeof(int32_t) * np);
     h2d(BOFF_ + p0, BRANGE, sizeof(int2) * np);
+    h2d(S.BRS + p0, BRANGE, sizeof(int2) * np);   // storage order is id order after an upload
     if (sh.nrec) {
         h2d(BR_ + sh.r0, BR.data(), sizeof(int4) * sh.nrec);
         h2d(BF_ + sh.r0, BF.data(), sizeof(float4) * sh.nrec);
+        h2d(RO_ + sh.r0, RO.data(), sizeof(int32_t) * sh.nrec);
     }
+    if (sh.rec_cap > sh.nrec && e == hipSuccess)   // the rest of the region holds no records
+        e = hipMemsetAsync(RO_ + sh.r0 + sh.nrec, 0xff, sizeof(int32_t) * (sh.rec_cap - sh.nrec), stream_);
     if (sh.nb) {
         h2d(BS_ + sh.b0, BS.data(), sizeof(int2) * sh.nb);
         h2d(BL_ + sh.b0, BL.data(), sizeof(float) * sh.nb);
@@ -656,6 +681,12 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     sh.since_reorder = 1 << 30;  // re-sort storage at the first rebuild
     rb_force_ = true;            // neighbour lists for the new state
     table_dirty_ = true;
+    if (sh.nrec) {
+        // owner and partner slots of the records (the scene table maps slots to scenes)
+        if ((rc = write_scene_table())) return rc;
+        if ((rc = check(launch_record_slots(make_args(d_lf_), (int)hw_recs_, block_, stream_), "record slots")))
+            return rc;
+    }
     return DEM_OK;
 }
 
@@ -688,6 +719,14 @@ void Engine::plan_reorder(const int2* lf_dev, std::vector<LaunchOp>& ops) {
     b.ra = r;
     b.nlb = active_blocks();
     b.block = block_;
+    if (hw_recs_ > 0) {
+        // the records' owner and partner slots after the move
+        LaunchOp& c = new_op(ops, OP_RSLOTS, 3);
+        c.sa = make_args(lf_dev);
+        c.sa.a = r.dst;
+        c.ncells = (int)hw_recs_;
+        c.block = block_;
+    }
     set_ ^= 1;
     lcur_ ^= 1;
     for (SceneHost& sh : scenes_) {
@@ -716,6 +755,8 @@ hipError_t Engine::run_op(const LaunchOp& op) {
         case OP_BUILD_NL: return launch_build_nl(op.sa, op.nlb, op.grid, op.block, stream_);
         case OP_PARTITION: return launch_partition(op.ra, op.nlb, op.block, stream_);
         case OP_MOVE: return launch_reorder_move(op.ra, op.nlb, op.block, stream_);
+        case OP_BONDS: return launch_bond_forces(op.sa, op.ncells, op.block, op.f64 != 0, stream_);
+        case OP_RSLOTS: return launch_record_slots(op.sa, op.ncells, op.block, stream_);
         default: return hipErrorInvalidValue;
     }
 }
@@ -737,7 +778,8 @@ void Engine::clear_graphs() {
 int Engine::run_ops(const std::vector<LaunchOp>& ops) {
     const bool graph = use_graph_ && !timing_ && ops.size() <= kGraphMaxOps;
     if (!graph) {
-        std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
+        struct Ev { int timed, kind; hipEvent_t a, b; };
+        std::vector<Ev> evs;
         size_t ev_used = 0;
         auto ev = [&]() -> hipEvent_t {
             if (ev_used >= ev_pool_.size()) {
@@ -753,7 +795,7 @@ int Engine::run_ops(const std::vector<LaunchOp>& ops) {
             hipEvent_t a = nullptr, b = nullptr;
             if (timing_) { a = ev(); b = ev(); hipEventRecord(a, stream_); }
             const hipError_t e = run_op(op);
-            if (timing_) { hipEventRecord(b, stream_); evs.push_back({op.timed, {a, b}}); }
+            if (timing_) { hipEventRecord(b, stream_); evs.push_back({op.timed, op.kind, a, b}); }
             if (int rc = check(e, kOpNames[op.kind])) return rc;
         }
         if (timing_) {
@@ -766,9 +808,10 @@ int Engine::run_ops(const std::vector<LaunchOp>& ops) {
             n_step = n_scan = n_scatter = n_reorder = 0;
             for (auto& x : evs) {
                 float m = 0;
-                hipEventElapsedTime(&m, x.second.first, x.second.second);
-                switch (x.first) {
-                    case 0: t_step += m; n_step++; break;
+                hipEventElapsedTime(&m, x.a, x.b);
+                switch (x.timed) {
+                    // "step": the step kernel with the bond pass before it, per step kernel
+                    case 0: t_step += m; n_step += x.kind == OP_STEP ? 1 : 0; break;
                     case 1: t_scan += m; n_scan++; break;
                     case 2: t_scatter += m; n_scatter++; break;
                     default: t_reorder += m; n_reorder++; break;
@@ -891,6 +934,17 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
         bool lom = false;
         for (size_t i = 0; i < sids.size(); i++)
             if (lf[(size_t)k * scene_cap_ + sids[i]].x & SF_LOM) lom = true;
+        bool sim = false;
+        for (size_t i = 0; i < sids.size(); i++)
+            if (lf[(size_t)k * scene_cap_ + sids[i]].x & SF_SIM) sim = true;
+        if (any_bonds && sim) {
+            // bond records first: one thread per record, results read by the step kernel
+            LaunchOp& bp = new_op(ops, OP_BONDS, 0);
+            bp.sa = make_args(lfk);
+            bp.ncells = (int)hw_recs_;
+            bp.block = block_;
+            bp.f64 = any_f64 ? 1 : 0;
+        }
         LaunchOp& st = new_op(ops, OP_STEP, 0);
         st.sa = make_args(lfk);
         st.sa.flip = lom ? 1 : 0;

@@ -78,7 +78,37 @@ struct Self {
 
 struct Data3 {
     float d0, d1, d2;  // data[0..2] of the calling particle
+    __device__ void add(float a, float b, float c) {
+        d0 += a;
+        d1 += b;
+        d2 += c;
+    }
+    __device__ void assign(float a, float b, float c) {
+        d0 = a;
+        d1 = b;
+        d2 = c;
+    }
 };
+
+// The data[0..2] updates of one bond record, logged in order (at most two) so that the
+// owner replays them onto its running sums with the same roundings.
+constexpr float DD_ADD = 1.0f, DD_SET = 2.0f;
+struct DataLog {
+    float4 op[2];   // {a, b, c, DD_ADD | DD_SET}; w = 0: no update
+    int n;
+    __device__ void add(float a, float b, float c) {
+        if (n < 2) op[n] = make_float4(a, b, c, DD_ADD);
+        n++;
+    }
+    __device__ void assign(float a, float b, float c) {
+        if (n < 2) op[n] = make_float4(a, b, c, DD_SET);
+        n++;
+    }
+};
+__device__ __forceinline__ void replay(Data3& dd, const float4& o) {
+    if (o.w == DD_ADD) dd.add(o.x, o.y, o.z);
+    else if (o.w == DD_SET) dd.assign(o.x, o.y, o.z);
+}
 
 // ---- force laws (2D_Simulation.wgsl:264-432); `a` is the calling particle ----
 
@@ -112,9 +142,9 @@ __device__ __forceinline__ float rel_tangent(const Self& a, const float4& pb, co
 }
 
 // linear_model, :338-373.  L = r_a + r_b (contact) or the bond length.
-template <bool F64>
+template <bool F64, class DD>
 __device__ __forceinline__ float3 linear_model(float mu, const Self& a, const float4& pb, const float2& db, float kn_b,
-                                               float ks_b, float L, float& ft, Data3& dd) {
+                                               float ks_b, float L, float& ft, DD& dd) {
     float nd = fminf(0.0f, dist2<F64>(a, pb.x, pb.y, L));
     if (nd == 0.0f) return make_float3(0.0f, 0.0f, 0.0f);
     float kn = 1.0f / (1.0f / a.kn + 1.0f / kn_b);
@@ -128,9 +158,7 @@ __device__ __forceinline__ float3 linear_model(float mu, const Self& a, const fl
     float lim = fabsf(nf) * mu;
     ft = fminf(fmaxf(ft + rt * ks, -lim), lim);
     float m = -a.r * ft;
-    dd.d0 += nf;
-    dd.d1 += ft;
-    dd.d2 += m;
+    dd.add(nf, ft, m);
     return make_float3(nx * nf + tx * ft, ny * nf + ty * ft, m);
 }
 
@@ -141,9 +169,9 @@ struct BondCtx {
 // Intact-bond laws: linear_parallel_bonds (mode 3, :286-336), linear_contact_bonds
 // (mode 2, :375-417), normal_bonds (mode 1, :419-432).  bf = {ft, fbt, theta_b}.
 // Sets `code` to the break code when the bond tears.
-template <bool F64>
+template <bool F64, class DD>
 __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, const float4& pb, const float2& db,
-                                           float kn_b, float ks_b, const BondCtx& bc, float4& bf, Data3& dd,
+                                           float kn_b, float ks_b, const BondCtx& bc, float4& bf, DD& dd,
                                            int& code) {
     code = 0;
     if (u.bonds == 3) {
@@ -174,9 +202,7 @@ __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, 
             fy = 0.0f;
             m = 0.0f;
         }
-        dd.d0 += -nf;
-        dd.d1 += bf.y;
-        dd.d2 += m;
+        dd.add(-nf, bf.y, m);
         float3 lm = linear_model<F64>(u.friction_coefficient, a, pb, db, kn_b, ks_b, bc.L, bf.x, dd);
         return make_float3(fx + lm.x, fy + lm.y, m + lm.z);
     } else if (u.bonds == 2) {
@@ -201,9 +227,7 @@ __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, 
             fy = 0.0f;
             m = 0.0f;
         }
-        dd.d0 = -nf;
-        dd.d1 = bf.x;
-        dd.d2 = m;
+        dd.assign(-nf, bf.x, m);
         return make_float3(fx, fy, m);
     } else if (u.bonds == 1) {
         float disp = dist1<F64>(a, pb);
@@ -349,6 +373,71 @@ __device__ __forceinline__ bool bonded_partner(const DevArrays& a, int b_lo, int
     return hit;
 }
 
+// One bond endpoint record of the particle `me` (position p, scene S; partner in slot j) in a launch
+// whose force half runs: the narrow-phase check of the bonded partner, the break
+// state seen from the partner, the bond or contact law; writes the record (and the
+// bond's break state) back.  Returns {fx, fy, moment, 1}, or all 0 when the record
+// is dead or dies here (2D_Simulation.wgsl:185-204, 230-262).
+template <bool F64, class DD>
+__device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const SceneDev& S, const Self& me,
+                                                   const float4& p, bool narrow, int step, int cur, int q, int j,
+                                                   DD& dd) {
+    const dem_uniform& u = S.u;
+    int4 br = a.BR[q];
+    if (br.w == 0) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 bf = a.BF[q];
+    const float4 pb = a.P[cur][j];
+    const float2 db = a.D[cur][j];
+    int bonded = br.z;
+    if (narrow) {
+        const bool found = lenf(pb.x - p.x, pb.y - p.y) < (pb.z + p.z);
+        if (!found && (bonded < 0 || u.bonds == 0)) {
+            br.w = 0;
+            a.BR[q] = br;
+            return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else if (!found && bonded > 0 && u.bonds <= 1) {
+            bf.x = 0.0f;
+        }
+    }
+    const int bid = br.y;
+    const bool bid_ok = bid >= 0 && bid < S.nb;
+    if (bonded >= 0) {
+        // a break recorded by the partner in an earlier step is seen here
+        const int2 bs = bid_ok ? a.BS[S.b0 + bid] : make_int2(0, 0);
+        if (bs.x < 0 && bs.y < step) bonded = bs.x;
+    }
+    const int mb = a.M[j].x & PF_MAT_MASK;
+    const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
+    float3 f;
+    if (bonded < 0 || u.bonds == 0) {
+        f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness, MB.shear_stiffness,
+                              me.r + pb.z, bf.x, dd);
+    } else if (u.bonds >= 1 && u.bonds <= 3) {
+        BondCtx bc;
+        bc.L = bid_ok ? a.BL[S.b0 + bid] : 0.0f;
+        if (S.has_strength && bid_ok) {
+            const float2 st = a.BSTR[S.b0 + bid];
+            bc.sn = st.x;
+            bc.ss = st.y;
+        } else {
+            bc.sn = u.bond_normal_strength;
+            bc.ss = u.bond_shear_strength;
+        }
+        int code = 0;
+        f = bond_law<F64>(u, me, pb, db, MB.normal_stiffness, MB.shear_stiffness, bc, bf, dd, code);
+        if (code < 0) {
+            bonded = code;
+            if (bid_ok) a.BS[S.b0 + bid] = make_int2(code, step);
+        }
+    } else {
+        f = make_float3(0.0f, 0.0f, 0.0f);
+    }
+    br.z = bonded;
+    a.BR[q] = br;
+    a.BF[q] = bf;
+    return make_float4(f.x, f.y, f.z, 1.0f);
+}
+
 template <bool F64, bool BONDS>
 __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const DevArrays& a = args.a;
@@ -377,10 +466,9 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     float4 p = a.P[cur][s];
     float2 d = a.D[cur][s];
     const int cap = (int)a.cap;  // KMAX * cap < 2^31 (checked at pool creation)
-    const int oid = mw.y;
     int b_lo = 0, b_hi = 0;
     if (BONDS && act) {
-        const int2 br = a.BOFF[S.p0 + oid];
+        const int2 br = a.BRS[s];
         b_lo = br.x;
         b_hi = br.y;
     }
@@ -529,7 +617,6 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         me.kn = MA.normal_stiffness; me.ks = MA.shear_stiffness;
         Data3 dd{0.0f, 0.0f, 0.0f};
         float nfx = 0.0f, nfy = 0.0f, nm = 0.0f;
-        const int step = lfw.y;
         const int oc = min(cc_old, KMAX);
         if (!narrow) {
             // no narrow phase this step: the existing contacts carry over
@@ -540,63 +627,18 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         // 2. forces over bond records and contacts merged in ascending partner id;
         //    the new list is written as it is evaluated
         int bq = b_lo;
+        // bond records: the bond pass of this launch evaluated each one (force, moment,
+        // record and break state); the owner adds them in partner order
         auto eval_record = [&](int q) {
-            int4 br = a.BR[q];
-            if (br.w == 0) return;
-            float4 bf = a.BF[q];
-            const int j = a.INV[S.p0 + br.x];
-            const float4 pb = a.P[cur][j];
-            const float2 db = a.D[cur][j];
-            int bonded = br.z;
-            if (narrow) {
-                const bool found = lenf(pb.x - p.x, pb.y - p.y) < (pb.z + p.z);
-                if (!found && (bonded < 0 || u.bonds == 0)) {
-                    br.w = 0;
-                    a.BR[q] = br;
-                    return;
-                } else if (!found && bonded > 0 && u.bonds <= 1) {
-                    bf.x = 0.0f;
-                }
+            const float4 rf = a.RF[q];
+            if (rf.w == 0.0f) return;
+            nfx += rf.x;
+            nfy += rf.y;
+            nm += rf.z;
+            if (S.record) {
+                replay(dd, a.RD[2 * q]);
+                replay(dd, a.RD[2 * q + 1]);
             }
-            const int bid = br.y;
-            const bool bid_ok = bid >= 0 && bid < S.nb;
-            if (bonded >= 0) {
-                // a break recorded by the partner in an earlier step is seen here
-                const int2 bs = bid_ok ? a.BS[S.b0 + bid] : make_int2(0, 0);
-                if (bs.x < 0 && bs.y < step) bonded = bs.x;
-            }
-            const int mb = a.M[j].x & PF_MAT_MASK;
-            const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
-            float3 f;
-            if (bonded < 0 || u.bonds == 0) {
-                f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness, MB.shear_stiffness,
-                                      me.r + pb.z, bf.x, dd);
-            } else if (u.bonds >= 1 && u.bonds <= 3) {
-                BondCtx bc;
-                bc.L = bid_ok ? a.BL[S.b0 + bid] : 0.0f;
-                if (S.has_strength && bid_ok) {
-                    const float2 st = a.BSTR[S.b0 + bid];
-                    bc.sn = st.x;
-                    bc.ss = st.y;
-                } else {
-                    bc.sn = u.bond_normal_strength;
-                    bc.ss = u.bond_shear_strength;
-                }
-                int code = 0;
-                f = bond_law<F64>(u, me, pb, db, MB.normal_stiffness, MB.shear_stiffness, bc, bf, dd, code);
-                if (code < 0) {
-                    bonded = code;
-                    if (bid_ok) a.BS[S.b0 + bid] = make_int2(code, step);
-                }
-            } else {
-                f = make_float3(0.0f, 0.0f, 0.0f);
-            }
-            nfx += f.x;
-            nfy += f.y;
-            nm += f.z;
-            br.z = bonded;
-            a.BR[q] = br;
-            a.BF[q] = bf;
         };
 
         if (fast) {
@@ -660,7 +702,26 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             a.CFn[q * cap + s] = ft;
         }
         if (BONDS) {
-            while (bq < b_hi) eval_record(bq++);
+            // the records after the last contact, four results in flight at a time
+            while (bq < b_hi) {
+                const int m = min(b_hi - bq, 4);
+                const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                const float4 rf[4] = {a.RF[bq], m > 1 ? a.RF[bq + 1] : z, m > 2 ? a.RF[bq + 2] : z,
+                                      m > 3 ? a.RF[bq + 3] : z};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (k < m && rf[k].w != 0.0f) {
+                        nfx += rf[k].x;
+                        nfy += rf[k].y;
+                        nm += rf[k].z;
+                        if (S.record) {
+                            replay(dd, a.RD[2 * (bq + k)]);
+                            replay(dd, a.RD[2 * (bq + k) + 1]);
+                        }
+                    }
+                }
+                bq += m;
+            }
         }
         if (n != cc_old) a.CC[s] = n;   // single-buffered: written only when it changes
 
@@ -790,6 +851,73 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const float mxx = mx * mx;
     const float myy = my * my;
     if (mxx + myy > S.trig2) a.RB[nxt] = 1;
+}
+
+// ---- bond pass: one thread per bond endpoint record ----
+// Runs before the step kernel of every launch with a force half.  The owner of record
+// q is the particle with global id RO[q] (-1: no record); lbs is the scene table's
+// block of slots.  Results: RF[q] = {fx, fy, moment, alive}, RD[2q..2q+1] = the
+// record's data[0..2] updates (with data recording).
+template <bool F64>
+__device__ __forceinline__ void bond_pass_body(const StepArgs& args, const SceneDev& S, int q, int i, int j, int2 lfw) {
+    const DevArrays& a = args.a;
+    const int cur = args.cur;
+    const float4 p = a.P[cur][i];
+    const float2 d = a.D[cur][i];
+    const int mat = (int)((uint32_t)a.M[i].x & PF_MAT_MASK);
+    const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
+    Self me;
+    me.x = p.x; me.y = p.y; me.r = p.z; me.drot = p.w; me.dx = d.x; me.dy = d.y;
+    me.kn = MA.normal_stiffness; me.ks = MA.shear_stiffness;
+    const uint32_t fl = (uint32_t)lfw.x;
+    const bool narrow = (fl & SF_SIM) && (fl & SF_NARROW);
+    DataLog dl;
+    dl.n = 0;
+    dl.op[0] = dl.op[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    a.RF[q] = eval_bond_record<F64>(a, S, me, p, narrow, lfw.y, cur, q, j, dl);
+    if (S.record) {
+        a.RD[2 * q] = dl.op[0];
+        a.RD[2 * q + 1] = dl.op[1];
+    }
+}
+
+template <bool F64>
+__global__ void __launch_bounds__(256) k_bond_forces(StepArgs args, int nrec, int lbs) {
+    const DevArrays& a = args.a;
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= nrec) return;
+    const int i = a.RS[q];   // owner slot (-1: no record)
+    if (i < 0) return;
+    const int j = a.RP[q];   // partner slot
+    const int sid = args.blk_scene[i / lbs];
+    if (sid < 0) return;
+    const SceneDev& S = args.scenes[sid];
+    const int2 lfw = args.lf[sid];
+    const uint32_t fl = (uint32_t)lfw.x;
+    if (!(fl & SF_ACTIVE) || !(fl & SF_SIM)) return;
+    if ((uint32_t)a.M[i].x & PF_INACTIVE) return;   // an inactive owner evaluates nothing
+    if (F64 && S.use_f64) bond_pass_body<true>(args, S, q, i, j, lfw);
+    else bond_pass_body<false>(args, S, q, i, j, lfw);
+}
+
+// Owner and partner slots of every record (after an upload and after each re-sort):
+// RS[q] = slot of the owner RO[q] (-1: no record), RP[q] = slot of the record's partner.
+__global__ void k_record_slots(StepArgs args, int nrec, int lbs) {
+    const DevArrays& a = args.a;
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= nrec) return;
+    const int g = a.RO[q];
+    int i = -1, j = 0;
+    if (g >= 0) {
+        const int o = a.INV[g];
+        const int sid = args.blk_scene[o / lbs];
+        if (sid >= 0) {
+            i = o;
+            j = a.INV[args.scenes[sid].p0 + a.BR[q].x];
+        }
+    }
+    a.RS[q] = i;
+    a.RP[q] = j;
 }
 
 // Whether this block's scene measures distances in f64.  A launch that holds any
@@ -1140,6 +1268,7 @@ __global__ void k_reorder_move(ReorderArgs r) {
     const int nc = nl_count(w0);
     r.dst.NL[j] = nl_word(nc, nc > 0 ? r.newslot[nl_first(w0)] : 0);
     r.dst.XR[j] = r.src.XR[s];
+    r.dst.BRS[j] = r.src.BRS[s];
     for (int q = 1; q < nc; q++) r.dst.NL[q * cap + j] = r.newslot[r.src.NL[q * cap + s]];
     r.dst.INV[S.p0 + m.y] = j;
 }
@@ -1270,6 +1399,20 @@ hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, b
         if (bonds) hipLaunchKernelGGL((k_step_bonds<false>), g, b, 0, st, args);
         else hipLaunchKernelGGL(k_step_plain, g, b, 0, st, args);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_bond_forces(const StepArgs& args, int nrec, int lbs, bool f64, hipStream_t st) {
+    const int b = 256;
+    const dim3 g((unsigned)((nrec + b - 1) / b));
+    if (f64) hipLaunchKernelGGL(k_bond_forces<true>, g, dim3(b), 0, st, args, nrec, lbs);
+    else hipLaunchKernelGGL(k_bond_forces<false>, g, dim3(b), 0, st, args, nrec, lbs);
+    return hipGetLastError();
+}
+
+hipError_t launch_record_slots(const StepArgs& args, int nrec, int lbs, hipStream_t st) {
+    const int b = 256;
+    hipLaunchKernelGGL(k_record_slots, dim3((unsigned)((nrec + b - 1) / b)), dim3(b), 0, st, args, nrec, lbs);
     return hipGetLastError();
 }
 
