@@ -467,11 +467,33 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     float2 d = a.D[cur][s];
     const int cap = (int)a.cap;  // KMAX * cap < 2^31 (checked at pool creation)
     int b_lo = 0, b_hi = 0;
+    // bond records: at most four are held in registers (partner ids, alive bits) for
+    // the bonded-partner test of the contact search and the merge with the contacts
+    int rx[4] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
+    int ralive = 0;
     if (BONDS && act) {
         const int2 br = a.BRS[s];
         b_lo = br.x;
         b_hi = br.y;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (b_lo + k < b_hi) {
+                const int4 r = a.BR[b_lo + k];
+                rx[k] = r.x;
+                if (r.w != 0) ralive |= 1 << k;
+            }
+        }
     }
+    const bool rec_regs = !BONDS || b_hi - b_lo <= 4;
+    auto is_bonded = [&](int oj) {
+        if (!BONDS) return false;
+        if (!rec_regs) return bonded_partner<BONDS>(a, b_lo, b_hi, oj);
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (((ralive >> k) & 1) && rx[k] == oj) hit = true;
+        return hit;
+    };
 
     // own-state loads issued before the contact search so that their latency
     // overlaps it (the compiler keeps them behind the list stores otherwise)
@@ -486,7 +508,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     auto test = [&](int j, const float4& q) {
         if (j != s && overlaps(q, p)) {
             const int oj = a.M[j].y;
-            if (!bonded_partner<BONDS>(a, b_lo, b_hi, oj)) list_insert(a, s, oj, n);
+            if (!is_bonded(oj)) list_insert(a, s, oj, n);
         }
     };
     const int w0 = (narrow && act) ? a.NL[s] : 0;
@@ -495,7 +517,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     // The hits stay in registers as {partner id, slot, material word}, sorted by id,
     // and the force loop reads each partner by slot: no id -> slot lookup and no
     // re-read of the new list.  Same partners, same order, same arithmetic.
-    const bool fast = !BONDS && narrow && act && nc > 0 && nc <= 4 && cc_old <= 4;
+    const bool fast = rec_regs && narrow && act && nc > 0 && nc <= 4 && cc_old <= 4;
     int fo[4] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
     int fj[4] = {0, 0, 0, 0};
     int fm[4] = {0, 0, 0, 0};
@@ -519,10 +541,12 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             for (int k = 0; k < 4; k++) {
                 if (jj[k] >= 0 && jj[k] != s && overlaps(qq[k], p)) {
                     const int2 mj = a.M[jj[k]];
-                    fo[k] = mj.y;
-                    fj[k] = jj[k];
-                    fm[k] = mj.x;
-                    n++;
+                    if (!is_bonded(mj.y)) {
+                        fo[k] = mj.y;
+                        fj[k] = jj[k];
+                        fm[k] = mj.x;
+                        n++;
+                    }
                 }
             }
             // ascending partner id (misses carry INT32_MAX and sort last)
@@ -653,6 +677,10 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             for (int k = 0; k < n; k++) {
                 // the hit of rank k sits in element 0 (the arrays shift after each)
                 const int ob = fo[0], j = fj[0], mw = fm[0];
+                if (BONDS) {
+                    // records with a smaller partner id come first (at most four: registers)
+                    while (bq < b_hi && rx[(bq - b_lo) & 3] < ob) eval_record(bq++);
+                }
                 float ft = 0.0f;
 #pragma unroll
                 for (int m = 0; m < 4; m++)
