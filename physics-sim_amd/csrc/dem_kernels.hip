@@ -394,6 +394,9 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
         if (!found && (bonded < 0 || u.bonds == 0)) {
             br.w = 0;
             a.BR[q] = br;
+            // the partner may come back into contact: it returns to the neighbour
+            // lists, which leave out live bond partners, at the rebuild after this drift
+            a.RB[cur ^ 1] = 1;
             return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         } else if (!found && bonded > 0 && u.bonds <= 1) {
             bf.x = 0.0f;
@@ -1165,6 +1168,18 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
     const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
     const int pos0 = S.p0 - a.START[S.c0];
     const int rlo = max(cy - 1, 0), rhi = min(cy + 1, S.h - 1);
+    // live bond partners stay out of the list: their records carry the pair until the
+    // record dies, which raises the rebuild flag
+    const int2 brs = a.BRS[s];
+    auto bonded = [&](int j) {
+        if (brs.y <= brs.x) return false;
+        const int oj = a.M[j].y;
+        for (int q = brs.x; q < brs.y; q++) {
+            const int4 br = a.BR[q];
+            if (br.w != 0 && br.x == oj) return true;
+        }
+        return false;
+    };
     // the three stencil rows advance together, two candidates each per round, so
     // that six slot-id loads and then six position loads are in flight at once
     // (the list's order is free: the step kernel inserts partners in id order)
@@ -1199,7 +1214,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
             const float xx = dx * dx;
             const float yy = dy * dy;
             const float L = (lr + q[k].z) * 1.0001f;
-            if (xx + yy < L * L) {
+            if (xx + yy < L * L && !bonded(j[k])) {
                 if (cnt == 0) first = j[k];
                 else if (cnt < NLMAX) a.NL[cnt * cap + s] = j[k];
                 cnt++;
