@@ -435,8 +435,10 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
     } else {
         f = make_float3(0.0f, 0.0f, 0.0f);
     }
-    br.z = bonded;
-    a.BR[q] = br;
+    if (bonded != br.z) {   // the record word changes only when the bond breaks
+        br.z = bonded;
+        a.BR[q] = br;
+    }
     a.BF[q] = bf;
     return make_float4(f.x, f.y, f.z, 1.0f);
 }
