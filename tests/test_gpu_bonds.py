@@ -124,3 +124,30 @@ def test_tensile_assembly_matches_oracle():
     got.selections[top] = 1
     np.testing.assert_allclose(dev, got.get_datum(), rtol=1e-12, atol=1e-12)
     prog.close()
+
+
+def test_struck_chains_r8_striker_match_oracle():
+    """Config 1's striker radius: an r = 8 dense disc among r = 0.5 chain discs (grid cell
+    2 r_max + skin, so the striker's neighbour list overflows and it searches the
+    stencil of the cell list every step).  Through the impact the state, break codes and
+    contact table equal the oracle, with no contact dropped on either side."""
+    s, st, strengths = scenes.bonded_chains(chains=8, per_chain=48, seed=9, striker_r=8.0, striker_v=40.0,
+                                            striker_density=10.0, strength_scale=50.0)
+    scenes.chain_bonds(st, 8, 48, strengths)
+    n, nb = st.p_count, st.n_bonds
+    prog = ComputeProg(s, st.copy(), record_data=True, reorder_interval=32)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    for k in (200, 300, 300):
+        prog.step(k)
+        eng.step(k)
+    got = prog.update_state()
+    for name in FIELDS:
+        assert_bits(getattr(got, name)[: _n(name, n)], getattr(eng.st, name)[: _n(name, n)], name)
+    assert_bits(got.bonds[: 3 * nb], eng.st.bonds[: 3 * nb], "bond words (break codes)")
+    want = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n]
+    assert_bits(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want, "contact table")
+    c = prog.counters()
+    assert c["contact_overflow"] == 0 and c["list_overflow"] > 0
+    striker = got.contacts.view(np.int32).reshape(-1, 14, 6)[n - 1]
+    assert (striker[:, 1] >= 0).sum() > 0          # the striker reached the chains
+    prog.close()
