@@ -79,8 +79,11 @@ typedef struct dem_material {
  * Positions outside the grid clamp to the edge cells.  cell_size must be at
  * least twice the largest radius plus a few ulp of the grid extent; what it exceeds
  * that by is the skin of the neighbour lists (lists are rebuilt when a particle has
- * moved 0.45 skin).  cell_cap is kept for layout compatibility; the counting sort
- * has no per-cell capacity. */
+ * moved 0.45 skin).  Large bodies do not count: with the radii in decreasing order
+ * r_0 >= r_1 >= ..., the first k <= 64 with r_{k-1} > 2 r_k makes the top k discs
+ * large bodies (a striker among small discs), which pair with every disc through a
+ * per-scene list, and "the largest radius" is r_k.  cell_cap is kept for layout
+ * compatibility; the counting sort has no per-cell capacity. */
 typedef struct dem_grid {
     float   cell_size;
     int32_t cell_cap;

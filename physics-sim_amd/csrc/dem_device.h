@@ -40,6 +40,7 @@ enum : uint32_t {
     PF_WALL_R = 1u << 21,   // fixity.rot_vel != 1 (only a dead write in the reference)
     PF_EXT = 1u << 22,      // nonzero external force / moment
     PF_INACTIVE = 1u << 23, // r == 0 or beyond the active count
+    PF_LARGE = 1u << 24,    // a large body: on the scene's large-body list, not found through cells
     PF_MAT_MASK = 0xffffu,
 };
 
@@ -67,9 +68,16 @@ struct SceneDev {
     int32_t record;        // maintain data[7]
     int32_t has_strength;  // per-bond strengths present
     int32_t n_real;        // particles of the scene; slots [p0+n_real, p0+np) hold inactive padding
-    float skin;            // neighbour-list skin: cell side minus (2 r_max + rounding margin)
+    float skin;            // neighbour-list skin: cell side minus (2 r_small + rounding margin)
     float trig2;           // squared displacement since the list build that forces a rebuild
+    float rsmax;           // largest radius of the discs that are not large bodies
+    int32_t lb0, nlb;      // large bodies: particle ids LB[lb0, lb0 + nlb)
 };
+
+// Large bodies: at most LB_MAX per scene, the discs above the first gap of a factor 2
+// in the radii sorted in decreasing order (a striker among small discs).  The grid is
+// sized for the others; large bodies pair with everything through the list.
+constexpr int LB_MAX = 64;
 
 // A device array addressed by 32-bit element index: the byte offset is formed in
 // 32 bits, so loads and stores take the SGPR-base + VGPR-offset form and need no
@@ -116,6 +124,7 @@ struct DevArrays {
     Pool<float2> BSTR;      // per bond {normal, shear} strength
     Pool<const dem_material> MAT;
     // bond pass: owner of each record (global particle id, -1: none) and its results
+    Pool<int32_t> LB;       // large-body particle ids, LB_MAX per scene
     Pool<int32_t> RO;
     Pool<int32_t> RS, RP;   // slots of each record's owner (-1: none) and partner
     Pool<int2> BRS;         // [cap] per slot: its record range (moves with the particle)

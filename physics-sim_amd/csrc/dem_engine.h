@@ -64,6 +64,8 @@ struct SceneHost {
     bool del_override = false;
     bool r0_zero = false;    // particle 0 has radius 0: invocation 0 never updates coll_cont
     float rmax = 0.0f;       // largest radius (set at upload; radii never change in a run)
+    float rsmax = 0.0f;      // largest radius below the large bodies (rmax without them)
+    std::vector<int32_t> large;   // particle ids of the large bodies
     int64_t step = 0;        // Sim halves done since creation (bond-break epochs)
     // counters
     int64_t steps = 0, broad_phases = 0, reorders = 0;
@@ -162,7 +164,7 @@ private:
     void update_high_water();
     int alloc_region(std::vector<std::pair<int64_t, int64_t>>& used, int64_t cap, int64_t need, int64_t align,
                      int64_t* out);
-    SceneDev scene_dev(const SceneHost& sh) const;
+    SceneDev scene_dev(const SceneHost& sh, int32_t sid) const;
 
     int device_ = 0;
     hipStream_t stream_ = nullptr;
@@ -191,6 +193,7 @@ private:
     dem_material* MAT_ = nullptr;
     int32_t *KEY_ = nullptr, *RANK_ = nullptr, *CNT_ = nullptr, *START_ = nullptr, *SIDX_ = nullptr, *INV_ = nullptr;
     int32_t* RB_ = nullptr;
+    int32_t* LB_ = nullptr;   // large-body ids, LB_MAX per scene
     int32_t* RO_ = nullptr;   // bond pass: record owners, their slots and results
     int32_t *RS_ = nullptr, *RP_ = nullptr;
     float4 *RF_ = nullptr, *RD_ = nullptr;

@@ -4,11 +4,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <numeric>
 
 namespace psim {
 
-hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, hipStream_t st);
+hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, bool large,
+                       hipStream_t st);
 hipError_t launch_bin(const StepArgs& args, int nblocks, int block, hipStream_t st);
 int scan_tiles(int n);
 hipError_t launch_scan(const DevArrays& a, int ncells, const int32_t* gate, hipStream_t st);
@@ -32,6 +34,35 @@ namespace {
 inline int32_t fbits(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
 inline float bitsf(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
 inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+// Large bodies of a radius set: with the radii in decreasing order r_0 >= r_1 >= ...,
+// the first k <= LB_MAX with r_{k-1} > 2 r_k makes the top k discs large bodies and
+// r_k the largest small radius; without such a gap there are none (k = 0, rsmax = r_0).
+int32_t split_large(const float* radii, int32_t n, float* rsmax) {
+    std::vector<float> top;
+    top.reserve(LB_MAX + 2);
+    for (int32_t i = 0; i < n; i++) {
+        if (!(radii[i] > 0.0f)) continue;
+        top.push_back(radii[i]);
+        std::push_heap(top.begin(), top.end(), std::greater<float>());   // min-heap of the largest
+        if ((int32_t)top.size() > LB_MAX + 1) {
+            std::pop_heap(top.begin(), top.end(), std::greater<float>());
+            top.pop_back();
+        }
+    }
+    std::sort(top.begin(), top.end(), std::greater<float>());
+    *rsmax = top.empty() ? 0.0f : top[0];
+    for (size_t k = 1; k < top.size(); k++)
+        if (top[k - 1] > 2.0f * top[k]) {
+            *rsmax = top[k];
+            return (int32_t)k;
+        }
+    return 0;
+}
+
+namespace {
 
 // Carves typed, 256-byte aligned arrays out of one buffer; with a null base it only
 // adds up the size.
@@ -72,7 +103,7 @@ Engine::~Engine() {
     hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BL_); hipFree(BSTR_); hipFree(MAT_);
     hipFree(KEY_); hipFree(RANK_); hipFree(CNT_); hipFree(START_); hipFree(SIDX_); hipFree(INV_);
     hipFree(RB_); hipFree(ORDER_); hipFree(NEWSLOT_); hipFree(RO_); hipFree(RF_); hipFree(RD_);
-    hipFree(RS_); hipFree(RP_);
+    hipFree(RS_); hipFree(RP_); hipFree(LB_);
     hipFree(STAT_); hipFree(TSUM_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
     hipFree(d_ke_); hipFree(d_cnt_);
     if (stage_) hipHostFree(stage_);
@@ -146,6 +177,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     ntiles_ = scan_tiles((int)cell_cap_);
     e = e ? e : dalloc(&TSUM_, ntiles_ + 1);
     e = e ? e : dalloc(&d_scenes_, scene_cap_);
+    e = e ? e : dalloc(&LB_, (int64_t)scene_cap_ * LB_MAX);
     const int64_t nblk = launch_grid_blocks();
     e = e ? e : dalloc(&d_blk_scene_, nblk);
     lf_cap_ = (int64_t)scene_cap_ * 4096;
@@ -353,8 +385,8 @@ int Engine::set_params(int32_t sid, const dem_params& p) {
     if (sh.n > 0) {
         const float cs = p.grid.cell_size;
         const float ext = cs * (float)std::max(p.grid.w, p.grid.h);
-        if (cs < 2.0f * sh.rmax + 4.0f * (std::nextafter(ext, INFINITY) - ext)) {
-            set_error("grid cell_size is below 2*max radius + rounding margin");
+        if (cs < 2.0f * sh.rsmax + 4.0f * (std::nextafter(ext, INFINITY) - ext)) {
+            set_error("grid cell_size is below 2*max radius (large bodies aside) + rounding margin");
             return DEM_EINVAL;
         }
     }
@@ -372,17 +404,18 @@ int Engine::set_params(int32_t sid, const dem_params& p) {
     return DEM_OK;
 }
 
-// Neighbour-list skin of a scene: what the cell side leaves above 2 r_max plus the
+// Neighbour-list skin of a scene: what the cell side leaves above 2 r_small (the
+// largest radius but the large bodies') plus the
 // rounding margin of the cell index (4 ulp of the grid extent).  Zero skin is correct
 // (the lists are rebuilt after every drift), only slower.
 float Engine::skin_of(const SceneHost& sh) const {
     const float cs = sh.params.grid.cell_size;
     const float ext = cs * (float)std::max(sh.params.grid.w, sh.params.grid.h);
-    const float need = 2.0f * sh.rmax + 4.0f * (std::nextafter(ext, INFINITY) - ext);
+    const float need = 2.0f * sh.rsmax + 4.0f * (std::nextafter(ext, INFINITY) - ext);
     return std::max(cs - need, 0.0f);
 }
 
-SceneDev Engine::scene_dev(const SceneHost& sh) const {
+SceneDev Engine::scene_dev(const SceneHost& sh, int32_t sid) const {
     SceneDev d{};
     d.u = sh.params.u;
     d.u.particles = sh.n;
@@ -407,6 +440,9 @@ SceneDev Engine::scene_dev(const SceneHost& sh) const {
     d.n_real = sh.n;
     d.skin = skin_of(sh);
     d.trig2 = (0.45f * d.skin) * (0.45f * d.skin);
+    d.rsmax = sh.rsmax;
+    d.lb0 = sid * LB_MAX;
+    d.nlb = (int32_t)sh.large.size();
     return d;
 }
 
@@ -417,7 +453,7 @@ int Engine::write_scene_table() {
     for (size_t i = 0; i < scenes_.size(); i++) {
         const SceneHost& sh = scenes_[i];
         if (!sh.live) continue;
-        tab[i] = scene_dev(sh);
+        tab[i] = scene_dev(sh, (int32_t)i);
         for (int64_t b = sh.p0 / block_; b < (sh.p0 + sh.np) / block_; b++) blk[b] = (int32_t)i;
     }
     hipError_t e = hipMemcpyAsync(d_scenes_, tab.data(), sizeof(SceneDev) * scene_cap_, hipMemcpyHostToDevice, stream_);
@@ -441,7 +477,7 @@ DevArrays Engine::arrays(int set) const {
     a.BOFF = BOFF_; a.BR = BR_; a.BF = BF_; a.BS = BS_; a.BL = BL_; a.BSTR = BSTR_; a.MAT = MAT_;
     a.KEY = KEY_; a.RANK = RANK_; a.CNT = CNT_; a.START = START_; a.SIDX = SIDX_; a.INV = INV_;
     a.NL = s.NL; a.XR = s.XR; a.RB = RB_;
-    a.RO = RO_; a.RF = RF_; a.RD = RD_; a.RS = RS_; a.RP = RP_; a.BRS = s.BRS;
+    a.RO = RO_; a.RF = RF_; a.RD = RD_; a.RS = RS_; a.RP = RP_; a.BRS = s.BRS; a.LB = LB_;
     a.STAT = STAT_; a.TSUM = TSUM_; a.ntiles = ntiles_;
     return a;
 }
@@ -508,6 +544,12 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     sh.forces.assign(v.forces, v.forces + 6 * (size_t)n);
     sh.matptr.assign(v.material_pointers, v.material_pointers + n);
     sh.has_ext = false;
+    // large bodies (a striker among small discs) pair through the scene's list
+    float rsmax = 0.0f;
+    const int32_t nlarge = n > 0 ? split_large(v.radii, n, &rsmax) : 0;
+    sh.large.clear();
+    for (int32_t i = 0; i < n && nlarge > 0; i++)
+        if (v.radii[i] > rsmax) sh.large.push_back(i);
     for (int32_t i = 0; i < np; i++) {
         INV[i] = (int32_t)sh.p0 + i;
         SIDX[i] = (int32_t)sh.p0 + i;
@@ -539,6 +581,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         if (fx[2] != 1) flags |= PF_WALL_R;
         if (fbits(f[0]) || fbits(f[1]) || fbits(f[2])) { flags |= PF_EXT; sh.has_ext = true; }
         if (r == 0.0f) flags |= PF_INACTIVE;
+        if (nlarge > 0 && r > rsmax) flags |= PF_LARGE;
         M[i] = make_int2((int)flags, i);
         if (v.data) {
             const float* d = v.data + 7 * (size_t)i;
@@ -555,9 +598,10 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         float rmax = 0.0f;
         for (int32_t i = 0; i < n; i++) rmax = std::max(rmax, v.radii[i]);
         sh.rmax = rmax;
+        sh.rsmax = nlarge > 0 ? rsmax : rmax;
         const float cs = sh.params.grid.cell_size;
         const float ext = cs * (float)std::max(sh.params.grid.w, sh.params.grid.h);
-        const float need = 2.0f * rmax + 4.0f * (std::nextafter(ext, INFINITY) - ext);
+        const float need = 2.0f * sh.rsmax + 4.0f * (std::nextafter(ext, INFINITY) - ext);
         if (cs < need) {
             set_error("grid cell_size " + std::to_string(cs) + " is below 2*max radius + rounding margin " +
                       std::to_string(need));
@@ -663,6 +707,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     h2d(SIDX_ + p0, SIDX, sizeof(int32_t) * np);
     h2d(BOFF_ + p0, BRANGE, sizeof(int2) * np);
     h2d(S.BRS + p0, BRANGE, sizeof(int2) * np);   // storage order is id order after an upload
+    if (!sh.large.empty()) h2d(LB_ + (int64_t)sid * LB_MAX, sh.large.data(), sizeof(int32_t) * sh.large.size());
     if (sh.nrec) {
         h2d(BR_ + sh.r0, BR.data(), sizeof(int4) * sh.nrec);
         h2d(BF_ + sh.r0, BF.data(), sizeof(float4) * sh.nrec);
@@ -747,7 +792,7 @@ LaunchOp& Engine::new_op(std::vector<LaunchOp>& ops, int kind, int timed) {
 
 hipError_t Engine::run_op(const LaunchOp& op) {
     switch (op.kind) {
-        case OP_STEP: return launch_step(op.sa, op.nlb, op.block, op.f64 != 0, op.bonds != 0, stream_);
+        case OP_STEP: return launch_step(op.sa, op.nlb, op.block, op.f64 != 0, op.bonds != 0, op.value != 0, stream_);
         case OP_RB_STORE: return launch_rb_store(op.rb, op.value, stream_);
         case OP_RB_BIN: return launch_rb_bin(op.sa, op.nlb, op.grid, op.block, stream_);
         case OP_SCAN: return launch_scan(op.da, op.ncells, op.gate, stream_);
@@ -918,10 +963,11 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
     hipError_t e = hipMemcpyAsync(d_lf_, lf.data(), sizeof(int2) * lf.size(), hipMemcpyHostToDevice, stream_);
     if ((rc = check(e, "upload launch flags"))) return rc;
 
-    bool any_f64 = false, any_bonds = false;
+    bool any_f64 = false, any_bonds = false, any_large = false;
     for (int32_t sid : sids) {
         any_f64 |= scenes_[sid].params.use_f64 != 0;
         any_bonds |= scenes_[sid].nrec > 0;
+        any_large |= !scenes_[sid].large.empty();
     }
     const int nblk = active_blocks();
     // the neighbour-list rebuild kernels stride over the slot blocks with one round of
@@ -952,6 +998,7 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
         st.block = block_;
         st.f64 = any_f64 ? 1 : 0;
         st.bonds = any_bonds ? 1 : 0;
+        st.value = any_large ? 1 : 0;   // a scene with large bodies
         lcur_ ^= 1;  // the step kernel wrote every live slot's next contact list
         // some LOM half wrote the next positions; every other slot carried its own over
         if (!lom) continue;

@@ -443,7 +443,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
     return make_float4(f.x, f.y, f.z, 1.0f);
 }
 
-template <bool F64, bool BONDS>
+template <bool F64, bool BONDS, bool LARGE>
 __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const DevArrays& a = args.a;
     const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
@@ -582,45 +582,74 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             if (jb >= 0) test(jb, qb);
         }
     } else if (narrow && act && nc < 0) {
-        // more candidates than a list holds: the 3x3 stencil around the particle's cell
-        // at the last build, over the last cell list (cells are wider than
-        // 2 r_max + skin, so it holds every particle that can touch s until the rebuild)
+        // more candidates than a list holds, or a large body: the stencil around the
+        // particle's cell at the last build, over the last cell list (cells are wider
+        // than 2 r_small + skin, so the 3x3 stencil holds every small disc that can touch
+        // a small disc until the rebuild; a large body reaches r + r_small + skin), then
+        // the scene's large bodies from their list
         const float2 xr0 = a.XR[s];
         int cx, cy;
         cell_of(S, xr0.x, xr0.y, cx, cy);
-        const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
         const int pos0 = S.p0 - a.START[S.c0];  // cell-list position -> slot-list index
-        const int rlo = max(cy - 1, 0), rhi = min(cy + 1, S.h - 1);
-        int e0[3], e1[3];
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            const int ry = rlo + r;
-            if (ry <= rhi) {
-                e0[r] = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
-                e1[r] = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
-            } else {
-                e0[r] = 0;
-                e1[r] = 0;
-            }
-        }
-        // the three stencil rows advance together, two candidates each per round: the
-        // slot-id loads of every row are in flight before the first position load
-        int e[3] = {e0[0], e0[1], e0[2]};
-        while (e[0] < e1[0] || e[1] < e1[1] || e[2] < e1[2]) {
-            int j[6];
+        const bool large = LARGE && S.nlb > 0 && p.z > S.rsmax;
+        auto small_test = [&](int j, const float4& q) {
+            if (!LARGE || S.nlb == 0 || q.z <= S.rsmax) test(j, q);   // large bodies come from the list
+        };
+        if (!large) {
+            const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
+            const int rlo = max(cy - 1, 0), rhi = min(cy + 1, S.h - 1);
+            int e0[3], e1[3];
 #pragma unroll
             for (int r = 0; r < 3; r++) {
-                j[2 * r] = e[r] < e1[r] ? a.SIDX[e[r]] : -1;
-                j[2 * r + 1] = e[r] + 1 < e1[r] ? a.SIDX[e[r] + 1] : -1;
+                const int ry = rlo + r;
+                if (ry <= rhi) {
+                    e0[r] = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
+                    e1[r] = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
+                } else {
+                    e0[r] = 0;
+                    e1[r] = 0;
+                }
             }
-            float4 q[6];
+            // the three stencil rows advance together, two candidates each per round: the
+            // slot-id loads of every row are in flight before the first position load
+            int e[3] = {e0[0], e0[1], e0[2]};
+            while (e[0] < e1[0] || e[1] < e1[1] || e[2] < e1[2]) {
+                int j[6];
 #pragma unroll
-            for (int k = 0; k < 6; k++) q[k] = j[k] >= 0 ? a.P[cur][j[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                for (int r = 0; r < 3; r++) {
+                    j[2 * r] = e[r] < e1[r] ? a.SIDX[e[r]] : -1;
+                    j[2 * r + 1] = e[r] + 1 < e1[r] ? a.SIDX[e[r] + 1] : -1;
+                }
+                float4 q[6];
 #pragma unroll
-            for (int k = 0; k < 6; k++)
-                if (j[k] >= 0) test(j[k], q[k]);
+                for (int k = 0; k < 6; k++) q[k] = j[k] >= 0 ? a.P[cur][j[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-            for (int r = 0; r < 3; r++) e[r] += 2;
+                for (int k = 0; k < 6; k++)
+                    if (j[k] >= 0) small_test(j[k], q[k]);
+#pragma unroll
+                for (int r = 0; r < 3; r++) e[r] += 2;
+            }
+        } else {
+            const int reach = max(1, (int)ceilf((p.z + S.rsmax + S.skin) * S.inv_cs) + 1);
+            const int xa = max(cx - reach, 0), xb = min(cx + reach, S.w - 1);
+            for (int ry = max(cy - reach, 0); ry <= min(cy + reach, S.h - 1); ry++) {
+                const int e0 = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
+                const int e1 = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
+                for (int e = e0; e < e1; e += 2) {
+                    const int ja = a.SIDX[e];
+                    const int jb = e + 1 < e1 ? a.SIDX[e + 1] : -1;
+                    const float4 qa = a.P[cur][ja];
+                    const float4 qb = jb >= 0 ? a.P[cur][jb] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    small_test(ja, qa);
+                    if (jb >= 0) small_test(jb, qb);
+                }
+            }
+        }
+        if (LARGE) {
+            for (int k = 0; k < S.nlb; k++) {
+                const int jl = a.INV[S.p0 + a.LB[S.lb0 + k]];
+                test(jl, a.P[cur][jl]);   // test skips s itself
+            }
         }
     }
 
@@ -962,19 +991,24 @@ __device__ __forceinline__ bool block_scene_f64(const StepArgs& args) {
 }
 
 // Plain f32 scenes: 8 waves per SIMD (at most 64 VGPRs) to hide the latency of the
-// candidate loop.  Launches holding an f64 scene (k_step_plain_f64) and bonded scenes
-// carry more live state and keep the default register budget (8 waves would spill).
+// candidate loop; launches with a large body take k_step_plain_large, whose stencil
+// fallback also covers them.  Launches holding an f64 scene (k_step_plain_f64) and
+// bonded scenes carry more live state and keep the default register budget (8 waves
+// would spill); both handle large bodies.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_plain(StepArgs args) {
-    step_slot<false, false>(args);
+    step_slot<false, false, false>(args);
+}
+__global__ void __launch_bounds__(256) k_step_plain_large(StepArgs args) {
+    step_slot<false, false, true>(args);
 }
 __global__ void __launch_bounds__(256) k_step_plain_f64(StepArgs args) {
-    if (block_scene_f64(args)) step_slot<true, false>(args);
-    else step_slot<false, false>(args);
+    if (block_scene_f64(args)) step_slot<true, false, true>(args);
+    else step_slot<false, false, true>(args);
 }
 template <bool F64>
 __global__ void __launch_bounds__(256) k_step_bonds(StepArgs args) {
-    if (F64 && block_scene_f64(args)) step_slot<true, true>(args);
-    else step_slot<false, true>(args);
+    if (F64 && block_scene_f64(args)) step_slot<true, true, true>(args);
+    else step_slot<false, true, true>(args);
 }
 
 // Binning of the current positions only (parity views and the bond builder).
@@ -1165,6 +1199,12 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
         a.NL[s] = nl_word(0, 0);
         return;
     }
+    if (m.x & PF_LARGE) {
+        // a large body searches the stencil of the cell list every step
+        a.NL[s] = nl_word(-1, 0);
+        atomicAdd(&a.STAT[5], 1ull);
+        return;
+    }
     int cx, cy;
     cell_of(S, p.x, p.y, cx, cy);
     const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
@@ -1216,7 +1256,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
             const float xx = dx * dx;
             const float yy = dy * dy;
             const float L = (lr + q[k].z) * 1.0001f;
-            if (xx + yy < L * L && !bonded(j[k])) {
+            if (xx + yy < L * L && (S.nlb == 0 || q[k].z <= S.rsmax) && !bonded(j[k])) {
                 if (cnt == 0) first = j[k];
                 else if (cnt < NLMAX) a.NL[cnt * cap + s] = j[k];
                 cnt++;
@@ -1224,6 +1264,20 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
         }
 #pragma unroll
         for (int r = 0; r < 3; r++) e[r] += 2;
+    }
+    // the scene's large bodies, from their list (never through the cells)
+    for (int k = 0; k < S.nlb; k++) {
+        const int jl = a.INV[S.p0 + a.LB[S.lb0 + k]];
+        const float4 q = a.P[cur][jl];
+        const float dx = q.x - p.x, dy = q.y - p.y;
+        const float xx = dx * dx;
+        const float yy = dy * dy;
+        const float L = (lr + q.z) * 1.0001f;
+        if (xx + yy < L * L && !bonded(jl)) {
+            if (cnt == 0) first = jl;
+            else if (cnt < NLMAX) a.NL[cnt * cap + s] = jl;
+            cnt++;
+        }
     }
     a.NL[s] = nl_word(cnt <= NLMAX ? cnt : -1, first);
     if (cnt > NLMAX) atomicAdd(&a.STAT[5], 1ull);   // particles on the stencil fallback
@@ -1435,13 +1489,15 @@ __global__ void k_bond_pairs(StepArgs args, int reach, int maxb, float factor, i
 
 // ---- launchers (host context) ----
 
-hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, hipStream_t st) {
+hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, bool large,
+                       hipStream_t st) {
     dim3 g(nblocks), b(block);
     if (f64) {
         if (bonds) hipLaunchKernelGGL((k_step_bonds<true>), g, b, 0, st, args);
         else hipLaunchKernelGGL(k_step_plain_f64, g, b, 0, st, args);
     } else {
         if (bonds) hipLaunchKernelGGL((k_step_bonds<false>), g, b, 0, st, args);
+        else if (large) hipLaunchKernelGGL(k_step_plain_large, g, b, 0, st, args);
         else hipLaunchKernelGGL(k_step_plain, g, b, 0, st, args);
     }
     return hipGetLastError();

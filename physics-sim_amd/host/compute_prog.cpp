@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
 
 #include "psim_host.h"
@@ -20,6 +21,21 @@ void check(int rc) {
 }  // namespace
 
 constexpr double kSkinFraction = 0.1;
+constexpr int kLargeMax = 64;
+
+float small_radius(const std::vector<float>& radii, int32_t n) {
+    // the library's large-body rule: radii in decreasing order, the first k <= 64 with
+    // r_{k-1} > 2 r_k sets the top k apart and r_k is the answer (else r_0)
+    std::vector<float> r;
+    for (int32_t i = 0; i < n; i++)
+        if (radii[i] > 0.0f) r.push_back(radii[i]);
+    if (r.empty()) return 0.0f;
+    const size_t k = std::min<size_t>(r.size(), kLargeMax + 1);
+    std::partial_sort(r.begin(), r.begin() + (long)k, r.end(), std::greater<float>());
+    for (size_t i = 1; i < k; i++)
+        if (r[i - 1] > 2.0f * r[i]) return r[i];
+    return r[0];
+}
 
 float min_cell_size(float rmax, float extent) {
     // 2 r_max plus room for the rounding of (x - base) / cell_size, a few ulp of the extent
@@ -33,10 +49,12 @@ dem_grid ComputeProg::default_grid(const Settings& settings, const State& state,
     double rmax = n ? (double)*std::max_element(state.radii.begin(), state.radii.begin() + n)
                     : (double)settings.setup.max_radius;
     rmax = std::max(rmax, 1e-6);
+    // large bodies pair through a list: the cells follow the other discs
+    const double rs = n ? std::max((double)small_radius(state.radii, n), 1e-6) : rmax;
     const double ext = std::max((double)settings.simulation.hor_bound, (double)settings.simulation.vert_bound) + 8.0 * rmax;
-    const float lo = min_cell_size((float)rmax, (float)ext);
-    // default: room for a neighbour-list skin of kSkinFraction * 2 r_max
-    const float cs = cell_size > 0.0f ? std::max(lo, cell_size) : (float)((double)lo + kSkinFraction * 2.0 * rmax);
+    const float lo = min_cell_size((float)rs, (float)ext);
+    // default: room for a neighbour-list skin of kSkinFraction * 2 r_small
+    const float cs = cell_size > 0.0f ? std::max(lo, cell_size) : (float)((double)lo + kSkinFraction * 2.0 * rs);
     return settings.box_grid(cs);
 }
 

@@ -14,7 +14,7 @@ from __future__ import annotations
 import math
 from typing import Dict, Iterable, Optional, Tuple
 
-from .compute import ComputeProg, default_cell_size
+from .compute import ComputeProg, default_cell_size, small_radius
 from .settings import Settings
 from .state import State
 
@@ -26,8 +26,9 @@ def cell_size_for(settings: Settings, state: State, factor: float) -> float:
     """`factor` times the default cell side (2 r_max + margin + neighbour-list skin)."""
     n = state.p_count
     rmax = float(max(state.radii[:n].max(), 1e-6)) if n else float(settings.setup.max_radius)
+    rs = max(small_radius(state.radii[:n]), 1e-6) if n else rmax
     ext = max(settings.simulation.hor_bound, settings.simulation.vert_bound) + 8.0 * rmax
-    return default_cell_size(rmax, ext) * factor
+    return default_cell_size(rs, ext) * factor
 
 
 def time_config(settings: Settings, state: State, block: int, factor: float, steps: int = 20, warmup: int = 5) -> float:

@@ -64,11 +64,12 @@ class ComputeProg:
         n = state.p_count
         rmax = float(np.max(state.radii[:n])) if n else float(settings.setup.max_radius)
         rmax = max(rmax, 1e-6)
+        rs = max(small_radius(state.radii[:n]), 1e-6) if n else rmax   # large bodies aside
         ext = max(settings.simulation.hor_bound, settings.simulation.vert_bound) + 8.0 * rmax
         if cell_size is not None:
-            cs = max(min_cell_size(rmax, ext), float(cell_size))
+            cs = max(min_cell_size(rs, ext), float(cell_size))
         else:
-            cs = default_cell_size(rmax, ext)
+            cs = default_cell_size(rs, ext)
         return settings.box_grid(cell_size=cs)
 
     def _params(self, settings: Settings) -> abi.DemParams:
@@ -237,6 +238,26 @@ class ComputeProg:
 # Neighbour-list skin of the default grid, as a fraction of 2 r_max: the lists are
 # rebuilt when a particle has moved 0.45 skin since the last build.
 SKIN_FRACTION = 0.1
+
+
+LB_MAX = 64
+
+
+def small_radius(radii) -> float:
+    """Largest radius once the large bodies are set aside: with the radii in decreasing
+    order r_0 >= r_1 >= ..., the first k <= 64 with r_{k-1} > 2 r_k makes the top k
+    discs large bodies (they pair through a list, not through the cells) and r_k the
+    answer; without such a gap it is r_0.  The library applies the same rule."""
+    r = np.asarray(radii, dtype=np.float32)
+    r = r[r > 0]
+    if len(r) == 0:
+        return 0.0
+    k = min(len(r), LB_MAX + 1)
+    top = -np.sort(-np.partition(r, len(r) - k)[len(r) - k:])
+    for i in range(1, len(top)):
+        if top[i - 1] > np.float32(2.0) * top[i]:
+            return float(top[i])
+    return float(top[0])
 
 
 def default_cell_size(rmax: float, extent: float) -> float:

@@ -106,3 +106,15 @@ def test_cpp_client_equals_python_client(scene, calls, regen):
     got = ph.run(b, calls, regen)
     assert got[0] == want[0]
     assert got[1] == want[1] and got[2] == want[2]
+
+
+def test_grids_set_large_bodies_aside_like_python():
+    """A striker of r = 8 among r = 0.5 chain discs: both clients size the cells for the
+    chain discs (the striker pairs through the large-body list)."""
+    s, st, strengths = scenes.bonded_chains(chains=4, per_chain=16, seed=2, striker_r=8.0)
+    b = st.save(s)
+    st2 = State.load(b, Settings())
+    (g_ref, g_def) = ph.grids(b)
+    d = ComputeProg.default_grid(s, st2)
+    assert g_def == (d.cell_size, d.cell_cap, d.w, d.h)
+    assert d.cell_size < 1.5          # 2 x 0.5 + margin + skin, not 2 x 8
