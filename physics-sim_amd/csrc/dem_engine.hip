@@ -764,7 +764,10 @@ void Engine::plan_reorder(const int2* lf_dev, std::vector<LaunchOp>& ops) {
     b.ra = r;
     b.nlb = active_blocks();
     b.block = block_;
-    if (hw_recs_ > 0) {
+    bool records = false;
+    for (const SceneHost& sh : scenes_)
+        if (sh.live && sh.nrec > 0) records = true;
+    if (records) {
         // the records' owner and partner slots after the move
         LaunchOp& c = new_op(ops, OP_RSLOTS, 3);
         c.sa = make_args(lf_dev);
