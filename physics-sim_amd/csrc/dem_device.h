@@ -49,7 +49,7 @@ enum : uint32_t {
     SF_SIM = 1u,        // run the Sim half (forces, first half-kick)
     SF_LOM = 2u,        // run the LOM half of the next step (second half-kick, drift, walls)
     SF_NARROW = 4u,     // the Sim half runs the narrow phase (collisions && coll_cont[0])
-    SF_BIN = 8u,        // the LOM half bins particles for the next narrow phase
+    SF_BIN = 8u,        // the next launch runs a narrow phase (the debug binning bins such scenes)
     SF_DELOVR = 16u,    // the LOM half keeps del_pos/del_rot (coll_cont[3] != 0)
     SF_ACTIVE = 32u,    // scene takes part in this launch
 };
