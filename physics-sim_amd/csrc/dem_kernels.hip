@@ -474,12 +474,20 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     int b_lo = 0, b_hi = 0;
     // bond records: at most four are held in registers (partner ids, alive bits) for
     // the bonded-partner test of the contact search and the merge with the contacts
+    // (loaded on first use: list candidates are never live bond partners, since the
+    // lists leave those out and a record's death forces a rebuild, so only the stencil
+    // fallback and the merge with contacts need the records)
     int rx[4] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
     int ralive = 0;
+    bool rloaded = false;
     if (BONDS && act) {
         const int2 br = a.BRS[s];
         b_lo = br.x;
         b_hi = br.y;
+    }
+    auto load_records = [&]() {
+        if (!BONDS || rloaded) return;
+        rloaded = true;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (b_lo + k < b_hi) {
@@ -488,11 +496,12 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 if (r.w != 0) ralive |= 1 << k;
             }
         }
-    }
+    };
     const bool rec_regs = !BONDS || b_hi - b_lo <= 4;
     auto is_bonded = [&](int oj) {
         if (!BONDS) return false;
         if (!rec_regs) return bonded_partner<BONDS>(a, b_lo, b_hi, oj);
+        load_records();
         bool hit = false;
 #pragma unroll
         for (int k = 0; k < 4; k++)
@@ -510,10 +519,11 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     // ---- narrow phase: every overlapping partner ----
     int n = 0;
     const bool narrow = (fl & SF_SIM) && (fl & SF_NARROW);
-    auto test = [&](int j, const float4& q) {
+    // listed: a neighbour-list candidate (never a live bond partner)
+    auto test = [&](int j, const float4& q, bool listed = false) {
         if (j != s && overlaps(q, p)) {
             const int oj = a.M[j].y;
-            if (!is_bonded(oj)) list_insert(a, s, oj, n);
+            if (listed || !is_bonded(oj)) list_insert(a, s, oj, n);
         }
     };
     const int w0 = (narrow && act) ? a.NL[s] : 0;
@@ -546,12 +556,10 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             for (int k = 0; k < 4; k++) {
                 if (jj[k] >= 0 && jj[k] != s && overlaps(qq[k], p)) {
                     const int2 mj = a.M[jj[k]];
-                    if (!is_bonded(mj.y)) {
-                        fo[k] = mj.y;
-                        fj[k] = jj[k];
-                        fm[k] = mj.x;
-                        n++;
-                    }
+                    fo[k] = mj.y;
+                    fj[k] = jj[k];
+                    fm[k] = mj.x;
+                    n++;
                 }
             }
             // ascending partner id (misses carry INT32_MAX and sort last)
@@ -568,18 +576,18 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             cswap(1, 3);
             cswap(1, 2);
         } else {
-            test(j0, q0);
-            if (j1 >= 0) test(j1, q1);
-            if (j2 >= 0) test(j2, q2);
-            if (j3 >= 0) test(j3, q3);
+            test(j0, q0, true);
+            if (j1 >= 0) test(j1, q1, true);
+            if (j2 >= 0) test(j2, q2, true);
+            if (j3 >= 0) test(j3, q3, true);
         }
         for (int k = 4; k < nc; k += 2) {
             const int ja = a.NL[k * cap + s];
             const int jb = k + 1 < nc ? a.NL[(k + 1) * cap + s] : -1;
             const float4 qa = a.P[cur][ja];
             const float4 qb = jb >= 0 ? a.P[cur][jb] : z;
-            test(ja, qa);
-            if (jb >= 0) test(jb, qb);
+            test(ja, qa, true);
+            if (jb >= 0) test(jb, qb, true);
         }
     } else if (narrow && act && nc < 0) {
         // more candidates than a list holds, or a large body: the stencil around the
@@ -713,6 +721,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 const int ob = fo[0], j = fj[0], mw = fm[0];
                 if (BONDS) {
                     // records with a smaller partner id come first (at most four: registers)
+                    load_records();
                     while (bq < b_hi && rx[(bq - b_lo) & 3] < ob) eval_record(bq++);
                 }
                 float ft = 0.0f;
