@@ -36,8 +36,9 @@ def assert_same(got: State, want: State, tag):
 
 
 def mixed_scenes():
-    """Granular boxes of several sizes, frictions and packings (f32), plus the
-    reference's parallel-bond experiment state (f64 distances, bond mode 3)."""
+    """Granular boxes of several sizes, frictions and packings (f32), the reference's
+    parallel-bond experiment state (f64 distances, bond mode 3) and bonded chains struck
+    by an r = 8 disc (a large body), through the impact."""
     out = []
     for k, (n_side, mu, spacing) in enumerate([(16, 0.5, 1.8), (24, 0.2, 1.9), (8, 0.8, 1.7), (32, 0.5, 2.0),
                                                (12, 0.3, 1.75)]):
@@ -46,6 +47,11 @@ def mixed_scenes():
     s = Settings()
     st = State.load_file(os.path.join(GOLDEN, "saved_states", "PBEXPS.bin"), s)
     out.append(("PBEXPS", s, st, 45))
+    # bonded chains struck by config 1's r = 8 disc (a large body on its scene's list)
+    s, st, strengths = scenes.bonded_chains(chains=4, per_chain=24, seed=31, striker_r=8.0, striker_v=40.0,
+                                            striker_density=10.0, strength_scale=40.0)
+    scenes.chain_bonds(st, 4, 24, strengths)
+    out.append(("striker", s, st, 300))
     return out
 
 
@@ -63,7 +69,7 @@ def test_batched_equals_solo_with_continuous_arrivals(graph):
     ticks' launches run as HIP graphs (captured once per launch pattern, replayed after)."""
     sc = mixed_scenes()
     want = [solo(s, st, n) for _, s, st, n in sc]
-    srv = Server(slot_capacity=8192, cell_capacity=1 << 16, bond_capacity=64, scene_capacity=16, substeps=7,
+    srv = Server(slot_capacity=8192, cell_capacity=1 << 16, bond_capacity=256, scene_capacity=16, substeps=7,
                  graph=graph)
     tickets, outs = [], []
     # two scenes start; the others arrive one per tick while those run
