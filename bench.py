@@ -31,18 +31,22 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_step_plain_column_collapse_4M_traffic.md")
 
 
-def measured_traffic(args, min_cell: bool):
+def measured_traffic(args, cell_size: float):
     """Bytes per launch of the step kernel from the committed counter summary, for the
-    configuration that summary covers (4,194,304 discs, block 256, the smallest cell);
-    None otherwise."""
-    default = (args.particles == 4194304 and args.per_row == 2048 and min_cell and args.block == 256
-               and args.collision_interval == 1)
-    if not default or not os.path.exists(TRAFFIC_PROFILE):
+    configuration that summary covers (4,194,304 discs, the block and cell side it
+    records); None otherwise."""
+    if not (args.particles == 4194304 and args.per_row == 2048 and args.collision_interval == 1):
         return None
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    meta = {}
     for line in open(TRAFFIC_PROFILE):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
-    return None
+        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "block", "cell_size"):
+            k, v = line.split(":", 1)
+            meta[k] = float(v)
+    if meta.get("block") != args.block or abs(meta.get("cell_size", -1.0) - cell_size) > 1e-6:
+        return None
+    return meta.get("traffic_bytes_per_launch")
 
 
 def parse():
@@ -290,7 +294,6 @@ def main():
         print("bench: autotuned block %d, cell factor %.2f" % (args.block, tuned["cell_factor"]), file=sys.stderr,
               flush=True)
     grid = ComputeProg.default_grid(s, st, cell_size=args.cell if args.cell > 0 else None)
-    min_cell = grid.cell_size == ComputeProg.default_grid(s, st).cell_size
     t_up = time.perf_counter()
     prog = ComputeProg(s, st, grid=grid, record_data=False, reorder_interval=args.reorder, block_size=args.block)
     prog.sync()
@@ -358,7 +361,7 @@ def main():
                          for k in c2},
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args, min_cell)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args, grid.cell_size)},
     }
     prog.close()
     if not args.no_bonded:

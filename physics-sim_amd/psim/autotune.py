@@ -19,7 +19,9 @@ from .settings import Settings
 from .state import State
 
 BLOCKS = (64, 128, 256)
-CELL_FACTORS = (1.0, 1.25, 1.5)
+# relative to the default cell (skin 0.1 x 2 r_max): 0.92 leaves a skin of about
+# 0.02 x 2 r_max (more rebuilds, fewer list candidates), 1.25 a wider one
+CELL_FACTORS = (0.92, 1.0, 1.25)
 
 
 def cell_size_for(settings: Settings, state: State, factor: float) -> float:
@@ -31,7 +33,7 @@ def cell_size_for(settings: Settings, state: State, factor: float) -> float:
     return default_cell_size(rs, ext) * factor
 
 
-def time_config(settings: Settings, state: State, block: int, factor: float, steps: int = 20, warmup: int = 5) -> float:
+def time_config(settings: Settings, state: State, block: int, factor: float, steps: int = 60, warmup: int = 5) -> float:
     """Mean milliseconds per step of `steps` steps (after `warmup`) at this block
     size and cell factor, from HIP events on the context's stream."""
     grid = ComputeProg.default_grid(settings, state, cell_size=cell_size_for(settings, state, factor))
@@ -49,7 +51,7 @@ def time_config(settings: Settings, state: State, block: int, factor: float, ste
 
 
 def tune(settings: Settings, state: State, blocks: Iterable[int] = BLOCKS,
-         factors: Iterable[float] = CELL_FACTORS, steps: int = 20, warmup: int = 5) -> Dict:
+         factors: Iterable[float] = CELL_FACTORS, steps: int = 60, warmup: int = 5) -> Dict:
     table = {}
     for b in blocks:
         for f in factors:
@@ -65,7 +67,7 @@ class Tuner:
     count)."""
 
     def __init__(self, blocks: Iterable[int] = BLOCKS, factors: Iterable[float] = CELL_FACTORS,
-                 steps: int = 20, warmup: int = 5):
+                 steps: int = 60, warmup: int = 5):
         self.blocks, self.factors = tuple(blocks), tuple(factors)
         self.steps, self.warmup = steps, warmup
         self.cache: Dict[Tuple[str, int], Dict] = {}
