@@ -150,3 +150,26 @@ def test_assemble_bonds_slots_sorted_by_partner():
     t = contacts.view(np.int32).reshape(4, 14, 6)
     assert t[1, :3, 1].tolist() == [0, 2, 3] and t[1, :3, 5].tolist() == [0, 1, 2]
     assert t[0, 0, :2].tolist() == [0, 1] and t[3, 0, 1] == 1 and t[3, 1, 1] == -1
+
+
+def test_large_body_split():
+    """The large-body rule (radii in decreasing order, the first k <= 64 with
+    r_{k-1} > 2 r_k sets the top k apart): brute force over small radius sets."""
+    from psim.compute import small_radius
+
+    def brute(r):
+        r = sorted([x for x in r if x > 0], reverse=True)
+        for k in range(1, min(len(r), 65)):
+            if r[k - 1] > 2 * r[k]:
+                return r[k]
+        return r[0] if r else 0.0
+
+    rng = np.random.default_rng(7)
+    cases = [[8.0] + [0.5] * 100, [1.0] + [0.5] * 10,            # a striker; exactly 2x is no gap
+             [8.0, 6.0] + [0.5] * 50, [3.0] * 70 + [1.0] * 10,    # two large bodies; 70 > 64 large
+             [0.175, 0.35, 0.525] * 20, [0.0, 0.0, 4.0, 1.0],     # no gap; inactive discs
+             []]
+    cases += [list(rng.choice([0.5, 1.0, 2.5, 7.0], size=int(rng.integers(1, 90)))) for _ in range(200)]
+    for c in cases:
+        r = np.asarray(c, dtype=np.float32)
+        assert small_radius(r) == np.float32(brute([float(x) for x in r])), c[:8]
