@@ -133,7 +133,7 @@ def serving_bench(args):
     from psim.serve import Server
     stream = scenes.serving_stream(n_scenes=args.serve_scenes, rate=args.serve_rate)
     # cell sizes per scene class (kind, size bucket) at the server's block size
-    tuner = Tuner(blocks=(256,), steps=10, warmup=3)
+    tuner = Tuner(blocks=(256,), steps=60, warmup=5)   # long enough to include list rebuilds
     grids = [tuner.grid_for("box" if i % 2 == 0 else "chain", s, st, block=256)
              for i, (_, s, st, _) in enumerate(stream)]
     slots = sum((st.p_count + 255) // 256 * 256 for _, _, st, _ in stream)
