@@ -30,6 +30,21 @@ __host__ __device__ inline int nl_count(int w) {
 __host__ __device__ inline int nl_first(int w) { return w & ((1 << NL_SHIFT) - 1); }
 constexpr int DATA_WORDS = 7;
 
+// Word V.w of the velocity record (bits, not a float):
+//  VW_EXPLICIT  the displacement of the last drift, {del_x, del_y, del_rot}, is the
+//               explicit record DX of the same buffer; otherwise it is the stored
+//               velocity times dT (2D_LOM.wgsl:97-99), bit for bit, because no wall
+//               changed the velocity after it was formed.  Set after a wall bounce,
+//               when the drift kept the loaded displacement (coll_cont[3] != 0), after
+//               a force half without a drift (the velocity is kicked again) and at upload.
+//  VW_PATH      distance travelled since the neighbour lists were built, an upper bound
+//               in units of 0.45 skin / 2^24 (sum of |dx| + |dy| per drift, rounded up,
+//               saturating): the position at the build is read only once it passes 2^24.
+constexpr uint32_t VW_EXPLICIT = 1u << 31;
+constexpr uint32_t VW_PATH = (1u << 31) - 1u;
+constexpr uint32_t VW_PATH_TRIG = 1u << 24;
+constexpr uint32_t VW_PATH_SAT = 1u << 30;
+
 // per-particle flag bits (word M.x bits 16..31; bits 0..15 hold the material id)
 enum : uint32_t {
     PF_KICK_X = 1u << 16,   // fixity.x_vel == 0: velocity integrates (2D_LOM.wgsl:88)
@@ -70,8 +85,10 @@ struct SceneDev {
     int32_t n_real;        // particles of the scene; slots [p0+n_real, p0+np) hold inactive padding
     float skin;            // neighbour-list skin: cell side minus (2 r_small + rounding margin)
     float trig2;           // squared displacement since the list build that forces a rebuild
+    float pscale;          // 2^24 / (0.45 skin): path units per unit length (1e30 for no skin)
     float rsmax;           // largest radius of the discs that are not large bodies
     int32_t lb0, nlb;      // large bodies: particle ids LB[lb0, lb0 + nlb)
+    float rho1, kn1, ks1;  // density and stiffnesses of material 0 (the whole scene when nm == 1)
 };
 
 // Large bodies: at most LB_MAX per scene, the discs above the first gap of a factor 2
@@ -96,10 +113,14 @@ struct Pool {
 };
 
 struct DevArrays {
-    // double-buffered neighbour-visible state (cur = read, nxt = written)
-    Pool<float4> P[2];      // {x, y, r, del_rot}
-    Pool<float2> D[2];      // {del_x, del_y}
-    Pool<float4> V;         // {vx, vy, w, rot}  (after a LOM half: the mid-step velocity)
+    // double-buffered neighbour-visible state.  Positions: P[cur] read, P[cur ^ 1]
+    // written by a drift.  Velocities and explicit displacements flip after every launch
+    // with the contact lists: V / DX read, Vn / DXn written.
+    Pool<float4> P[2];      // {x, y, r, rot}
+    Pool<float4> V;         // {vx, vy, w, word: VW_EXPLICIT | path}  (after a LOM half: the mid-step velocity)
+    Pool<float4> Vn;
+    Pool<float4> DX;        // {del_x, del_y, del_rot, 0}, valid where V.w has VW_EXPLICIT
+    Pool<float4> DXn;
     Pool<float4> A;         // {ax, ay, alpha, 0}
     Pool<int2> M;           // {material | flags, particle id within the scene}
     Pool<float4> EXT;       // external {fx, fy, moment, 0}
@@ -156,9 +177,11 @@ struct StepArgs {
     const int32_t* blk_scene;  // scene of each block (-1: none)
     const int2* lf;            // per scene {step flags, global step index of the Sim half}
     DevArrays a;
-    int cur;                   // which P/D buffer holds the current positions
+    int cur;                   // which P buffer holds the current positions
     int flip;                  // some scene drifts in this launch, so cur flips after it: every slot
-                               // that does not drift carries its P/D over to the other buffer
+                               // that does not drift carries its P over to the other buffer
+    int lbs;                   // block size of the step kernel's launch (a kernel argument: the
+                               // dispatch packet's copy costs a vector load per wave)
 };
 
 struct ReorderArgs {

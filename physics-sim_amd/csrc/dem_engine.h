@@ -17,9 +17,9 @@ namespace psim {
 void set_error(const std::string& msg);
 
 struct DevSet {
-    float4* P[2] = {nullptr, nullptr};
-    float2* D[2] = {nullptr, nullptr};
-    float4* V = nullptr;
+    float4* P[2] = {nullptr, nullptr};    // by position parity (cur_)
+    float4* V[2] = {nullptr, nullptr};    // by list parity (lcur_), as DX
+    float4* DX[2] = {nullptr, nullptr};
     float4* A = nullptr;
     int2* M = nullptr;
     float4* EXT = nullptr;

@@ -95,8 +95,8 @@ struct Slot {  // 2D_Simulation.wgsl:19-26
 Engine::~Engine() {
     if (stream_) hipStreamSynchronize(stream_);
     for (auto& s : sets_) {
-        hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.D[0]); hipFree(s.D[1]);
-        hipFree(s.V); hipFree(s.A); hipFree(s.M); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
+        hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.V[0]); hipFree(s.V[1]); hipFree(s.DX[0]); hipFree(s.DX[1]);
+        hipFree(s.A); hipFree(s.M); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
     }
     for (auto& s : sets_) { hipFree(s.CC); hipFree(s.NL); hipFree(s.XR); hipFree(s.BRS); }
     for (auto& l : lists_) { hipFree(l.CP); hipFree(l.CF); }
@@ -150,8 +150,9 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     for (auto& s : sets_) {
         hipError_t e = hipSuccess;
         e = e ? e : dalloc(&s.P[0], S); e = e ? e : dalloc(&s.P[1], S);
-        e = e ? e : dalloc(&s.D[0], S); e = e ? e : dalloc(&s.D[1], S);
-        e = e ? e : dalloc(&s.V, S); e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S);
+        e = e ? e : dalloc(&s.V[0], S); e = e ? e : dalloc(&s.V[1], S);
+        e = e ? e : dalloc(&s.DX[0], S); e = e ? e : dalloc(&s.DX[1], S);
+        e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S);
         e = e ? e : dalloc(&s.EXT, S); e = e ? e : dalloc(&s.CC, S);
         e = e ? e : dalloc(&s.NL, S * NLMAX); e = e ? e : dalloc(&s.XR, S); e = e ? e : dalloc(&s.BRS, S);
         if (with_data) { e = e ? e : dalloc(&s.DATA0, S); e = e ? e : dalloc(&s.DATA1, S); }
@@ -203,6 +204,11 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = e ? e : hipMemsetAsync(BOFF_, 0, sizeof(int2) * S, stream_);
     for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int2) * S, stream_);
     for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.CC, 0, sizeof(int32_t) * S, stream_);
+    for (auto& s : sets_)
+        for (int b = 0; b < 2; b++) {
+            e = e ? e : hipMemsetAsync(s.V[b], 0, sizeof(float4) * S, stream_);
+            e = e ? e : hipMemsetAsync(s.DX[b], 0, sizeof(float4) * S, stream_);
+        }
     std::vector<int32_t> blk(nblk, -1);
     e = e ? e : hipMemcpyAsync(d_blk_scene_, blk.data(), sizeof(int32_t) * nblk, hipMemcpyHostToDevice, stream_);
     e = e ? e : hipStreamSynchronize(stream_);
@@ -440,9 +446,15 @@ SceneDev Engine::scene_dev(const SceneHost& sh, int32_t sid) const {
     d.n_real = sh.n;
     d.skin = skin_of(sh);
     d.trig2 = (0.45f * d.skin) * (0.45f * d.skin);
+    d.pscale = d.skin > 0.0f ? (float)VW_PATH_TRIG / (0.45f * d.skin) : 1.0e30f;
     d.rsmax = sh.rsmax;
     d.lb0 = sid * LB_MAX;
     d.nlb = (int32_t)sh.large.size();
+    if (!sh.mats.empty()) {
+        d.rho1 = sh.mats[0].density;
+        d.kn1 = sh.mats[0].normal_stiffness;
+        d.ks1 = sh.mats[0].shear_stiffness;
+    }
     return d;
 }
 
@@ -467,8 +479,9 @@ int Engine::write_scene_table() {
 DevArrays Engine::arrays(int set) const {
     const DevSet& s = sets_[set];
     DevArrays a{};
-    a.P[0] = s.P[0]; a.P[1] = s.P[1]; a.D[0] = s.D[0]; a.D[1] = s.D[1];
-    a.V = s.V; a.A = s.A; a.M = s.M; a.EXT = s.EXT; a.DATA0 = s.DATA0; a.DATA1 = s.DATA1;
+    a.P[0] = s.P[0]; a.P[1] = s.P[1];
+    a.V = s.V[lcur_]; a.Vn = s.V[lcur_ ^ 1]; a.DX = s.DX[lcur_]; a.DXn = s.DX[lcur_ ^ 1];
+    a.A = s.A; a.M = s.M; a.EXT = s.EXT; a.DATA0 = s.DATA0; a.DATA1 = s.DATA1;
     const ContactLists& lc = lists_[lcur_];
     const ContactLists& ln = lists_[lcur_ ^ 1];
     a.CC = s.CC; a.CP = lc.CP; a.CF = lc.CF;
@@ -515,7 +528,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     // the device layout is built in pinned staging memory and copied from there
     Carve cv;
     auto carve = [&](Carve& c) {
-        c.take<float4>(np); c.take<float2>(np); c.take<float4>(np); c.take<float4>(np); c.take<int2>(np);
+        c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<int2>(np);
         c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<int32_t>(np);
         c.take<int32_t>((size_t)np * KMAX); c.take<float>((size_t)np * KMAX); c.take<int32_t>(np);
         c.take<int32_t>(np); c.take<int2>(np);
@@ -525,7 +538,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     if (!cv.base) { set_error("upload: pinned staging allocation failed"); return DEM_ENOMEM; }
     cv.off = 0;
     float4* P = cv.take<float4>(np);
-    float2* D = cv.take<float2>(np);
+    float4* DX = cv.take<float4>(np);
     float4* V = cv.take<float4>(np);
     float4* A = cv.take<float4>(np);
     int2* M = cv.take<int2>(np);
@@ -554,17 +567,18 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         INV[i] = (int32_t)sh.p0 + i;
         SIDX[i] = (int32_t)sh.p0 + i;
         if (i >= n) {
-            P[i] = make_float4(0, 0, 0, 0);
-            D[i] = make_float2(0, 0);
-            V[i] = A[i] = EXT[i] = D0[i] = D1[i] = make_float4(0, 0, 0, 0);
+            P[i] = DX[i] = V[i] = A[i] = EXT[i] = D0[i] = D1[i] = make_float4(0, 0, 0, 0);
             M[i] = make_int2((int)PF_INACTIVE, i);
             continue;
         }
         const float r = v.radii[i];
-        // WGPUComputeProg::restore uploads vel into del_pos and rot_vel into del_rot (wgpu_prog.rs:1247-1248)
-        P[i] = make_float4(v.pos[2 * i], v.pos[2 * i + 1], r, v.rot_vel[i]);
-        D[i] = make_float2(v.vel[2 * i], v.vel[2 * i + 1]);
-        V[i] = make_float4(v.vel[2 * i], v.vel[2 * i + 1], v.rot_vel[i], v.rot[i]);
+        // WGPUComputeProg::restore uploads vel into del_pos and rot_vel into del_rot
+        // (wgpu_prog.rs:1247-1248): an explicit displacement record
+        P[i] = make_float4(v.pos[2 * i], v.pos[2 * i + 1], r, v.rot[i]);
+        DX[i] = make_float4(v.vel[2 * i], v.vel[2 * i + 1], v.rot_vel[i], 0.0f);
+        float vw;
+        std::memcpy(&vw, &VW_EXPLICIT, sizeof(vw));
+        V[i] = make_float4(v.vel[2 * i], v.vel[2 * i + 1], v.rot_vel[i], vw);
         A[i] = make_float4(v.acc[2 * i], v.acc[2 * i + 1], v.rot_acc[i], 0.0f);
         const float* f = v.forces + 6 * (size_t)i;
         EXT[i] = make_float4(f[0], f[1], f[2], 0.0f);
@@ -684,8 +698,8 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_);
     };
     h2d(S.P[cur_] + p0, P, sizeof(float4) * np);
-    h2d(S.D[cur_] + p0, D, sizeof(float2) * np);
-    h2d(S.V + p0, V, sizeof(float4) * np);
+    h2d(S.DX[lcur_] + p0, DX, sizeof(float4) * np);
+    h2d(S.V[lcur_] + p0, V, sizeof(float4) * np);
     h2d(S.A + p0, A, sizeof(float4) * np);
     h2d(S.M + p0, M, sizeof(int2) * np);
     h2d(S.EXT + p0, EXT, sizeof(float4) * np);
@@ -747,8 +761,12 @@ void Engine::plan_reorder(const int2* lf_dev, std::vector<LaunchOp>& ops) {
     r.lf = lf_dev;
     r.src = arrays(set_);
     r.dst = arrays(set_ ^ 1);
+    // the contact lists and velocity records land in the other list buffer (lcur_
+    // flips with set_ below)
     r.dst.CP = r.src.CPn;
     r.dst.CF = r.src.CFn;
+    r.dst.V = r.dst.Vn;
+    r.dst.DX = r.dst.DXn;
     r.order = ORDER_;
     r.newslot = NEWSLOT_;
     r.cur = cur_;
@@ -1094,7 +1112,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_);
     };
     d2h(P, S.P[cur_] + p0, sizeof(float4) * np);
-    d2h(V, S.V + p0, sizeof(float4) * np);
+    d2h(V, S.V[lcur_] + p0, sizeof(float4) * np);
     d2h(A, S.A + p0, sizeof(float4) * np);
     d2h(M, S.M + p0, sizeof(int2) * np);
     d2h(INV, INV_ + p0, sizeof(int32_t) * np);
@@ -1129,7 +1147,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
         const float4 p = P[s], vv = V[s], aa = A[s];
         if (v.pos) { v.pos[2 * i] = p.x; v.pos[2 * i + 1] = p.y; }
         if (v.vel) { v.vel[2 * i] = vv.x; v.vel[2 * i + 1] = vv.y; }
-        if (v.rot) v.rot[i] = vv.w;
+        if (v.rot) v.rot[i] = p.w;
         if (v.rot_vel) v.rot_vel[i] = vv.z;
         if (v.acc) { v.acc[2 * i] = aa.x; v.acc[2 * i + 1] = aa.y; }
         if (v.rot_acc) v.rot_acc[i] = aa.z;

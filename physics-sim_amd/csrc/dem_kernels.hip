@@ -76,6 +76,26 @@ struct Self {
     float kn, ks;
 };
 
+__device__ __forceinline__ uint32_t vword(const float4& v) { return __float_as_uint(v.w); }
+
+// Displacement of the last drift, {del_x, del_y, del_rot, 0}, of the particle in slot j
+// whose current velocity record is v: the explicit record where the velocity word says
+// so, else the velocity times dT (the drift formed it so and no wall changed the
+// velocity afterwards).
+__device__ __forceinline__ float4 del_from(const DevArrays& a, int j, const float4& v, float dt) {
+    if (vword(v) & VW_EXPLICIT) return a.DX[j];
+    return make_float4(v.x * dt, v.y * dt, v.z * dt, 0.0f);
+}
+__device__ __forceinline__ float4 del_of(const DevArrays& a, int j, float dt) { return del_from(a, j, a.V[j], dt); }
+
+// Carries the velocity record (and an explicit displacement) of slot s over to the
+// buffers of the next launch.
+__device__ __forceinline__ void carry_velocity(const DevArrays& a, int s) {
+    const float4 v = a.V[s];
+    a.Vn[s] = v;
+    if (vword(v) & VW_EXPLICIT) a.DXn[s] = a.DX[s];
+}
+
 struct Data3 {
     float d0, d1, d2;  // data[0..2] of the calling particle
     __device__ void add(float a, float b, float c) {
@@ -132,10 +152,10 @@ __device__ __forceinline__ float dist1(const Self& a, const float4& pb) {
     return lenf(a.x - pb.x, a.y - pb.y) - (a.r + pb.z);
 }
 
-// relative tangential displacement, :352-360
-__device__ __forceinline__ float rel_tangent(const Self& a, const float4& pb, const float2& db, float tx, float ty) {
+// relative tangential displacement, :352-360; db = the partner's {del_x, del_y, del_rot}
+__device__ __forceinline__ float rel_tangent(const Self& a, const float4& pb, const float4& db, float tx, float ty) {
     float dra = a.drot * a.r;
-    float drb = pb.w * pb.z;
+    float drb = db.z * pb.z;
     float rtx = db.x - a.dx, rty = db.y - a.dy;
     float rr = drb + dra;
     return (rtx * tx + rty * ty) + rr;
@@ -143,7 +163,7 @@ __device__ __forceinline__ float rel_tangent(const Self& a, const float4& pb, co
 
 // linear_model, :338-373.  L = r_a + r_b (contact) or the bond length.
 template <bool F64, class DD>
-__device__ __forceinline__ float3 linear_model(float mu, const Self& a, const float4& pb, const float2& db, float kn_b,
+__device__ __forceinline__ float3 linear_model(float mu, const Self& a, const float4& pb, const float4& db, float kn_b,
                                                float ks_b, float L, float& ft, DD& dd) {
     float nd = fminf(0.0f, dist2<F64>(a, pb.x, pb.y, L));
     if (nd == 0.0f) return make_float3(0.0f, 0.0f, 0.0f);
@@ -170,7 +190,7 @@ struct BondCtx {
 // (mode 2, :375-417), normal_bonds (mode 1, :419-432).  bf = {ft, fbt, theta_b}.
 // Sets `code` to the break code when the bond tears.
 template <bool F64, class DD>
-__device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, const float4& pb, const float2& db,
+__device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, const float4& pb, const float4& db,
                                            float kn_b, float ks_b, const BondCtx& bc, float4& bf, DD& dd,
                                            int& code) {
     code = 0;
@@ -188,7 +208,7 @@ __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, 
         float tx = -ny, ty = nx;
         float rt = rel_tangent(a, pb, db, tx, ty);
         bf.y = bf.y + rt * ks * A;
-        float dtb = a.drot - pb.w;
+        float dtb = a.drot - db.z;
         bf.z = bf.z + dtb;
         float m = -kn * I * bf.z - bf.y * a.r;
         float fx = nx * nf + tx * bf.y, fy = ny * nf + ty * bf.y;
@@ -387,7 +407,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
     if (br.w == 0) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float4 bf = a.BF[q];
     const float4 pb = a.P[cur][j];
-    const float2 db = a.D[cur][j];
+    const float4 db = del_of(a, j, u.dT);
     int bonded = br.z;
     if (narrow) {
         const bool found = lenf(pb.x - p.x, pb.y - p.y) < (pb.z + p.z);
@@ -447,7 +467,7 @@ template <bool F64, bool BONDS, bool LARGE>
 __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const DevArrays& a = args.a;
     const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
-    const int s = (int)(blk * blockDim.x + threadIdx.x);
+    const int s = blk * args.lbs + (int)threadIdx.x;
     const int sid = args.blk_scene[blk];
     if (sid < 0) return;
     const SceneDev& S = args.scenes[sid];
@@ -456,11 +476,9 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const uint32_t fl = (uint32_t)lfw.x;
     const int cur = args.cur, nxt = cur ^ 1;
     if (!(fl & SF_ACTIVE)) {
-        copy_list(a, s, a.CC[s]);  // the lists flip after every launch
-        if (args.flip) {
-            a.P[nxt][s] = a.P[cur][s];
-            a.D[nxt][s] = a.D[cur][s];
-        }
+        copy_list(a, s, a.CC[s]);  // the lists and velocities flip after every launch
+        carry_velocity(a, s);
+        if (args.flip) a.P[nxt][s] = a.P[cur][s];
         return;
     }
     const dem_uniform& u = S.u;
@@ -469,7 +487,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const uint32_t pf = (uint32_t)mw.x;
     const bool act = !(pf & PF_INACTIVE);
     float4 p = a.P[cur][s];
-    float2 d = a.D[cur][s];
+    const int nlw = a.NL[s];     // issued with the other own-state loads, used if the narrow phase runs
     const int cap = (int)a.cap;  // KMAX * cap < 2^31 (checked at pool creation)
     int b_lo = 0, b_hi = 0;
     // bond records: at most four are held in registers (partner ids, alive bits) for
@@ -514,7 +532,17 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     float4 v = a.V[s];
     const int cc_old = a.CC[s];
     const int mat = (int)(pf & PF_MAT_MASK);
-    const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
+    // a scene of one material (the common case) reads its record without waiting for
+    // the particle's material word
+    // a scene of one material (the common case) takes the record from the scene table,
+    // without waiting for the particle's material word
+    float ma_rho = S.rho1, ma_kn = S.kn1, ma_ks = S.ks1;
+    if (S.nm > 1) {
+        const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
+        ma_rho = MA.density;
+        ma_kn = MA.normal_stiffness;
+        ma_ks = MA.shear_stiffness;
+    }
 
     // ---- narrow phase: every overlapping partner ----
     int n = 0;
@@ -526,7 +554,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             if (listed || !is_bonded(oj)) list_insert(a, s, oj, n);
         }
     };
-    const int w0 = (narrow && act) ? a.NL[s] : 0;
+    const int w0 = (narrow && act) ? nlw : 0;
     const int nc = nl_count(w0);
     // Register path (plain kernels): at most four candidates and four old contacts.
     // The hits stay in registers as {partner id, slot, material word}, sorted by id,
@@ -668,19 +696,19 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             const float4 d0 = a.DATA0[s];
             a.DATA0[s] = make_float4(0.0f, 0.0f, 0.0f, d0.w);
         }
-        if (args.flip) {
-            a.P[nxt][s] = p;
-            a.D[nxt][s] = d;
-        }
+        carry_velocity(a, s);
+        if (args.flip) a.P[nxt][s] = p;
         return;
     }
 
     float ax, ay, al;
 
     if (fl & SF_SIM) {
+        // this particle's displacement in the last drift (before the kick below)
+        const float4 dme = del_from(a, s, v, dt);
         Self me;
-        me.x = p.x; me.y = p.y; me.r = p.z; me.drot = p.w; me.dx = d.x; me.dy = d.y;
-        me.kn = MA.normal_stiffness; me.ks = MA.shear_stiffness;
+        me.x = p.x; me.y = p.y; me.r = p.z; me.drot = dme.z; me.dx = dme.x; me.dy = dme.y;
+        me.kn = ma_kn; me.ks = ma_ks;
         Data3 dd{0.0f, 0.0f, 0.0f};
         float nfx = 0.0f, nfy = 0.0f, nm = 0.0f;
         const int oc = min(cc_old, KMAX);
@@ -729,7 +757,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 for (int m = 0; m < 4; m++)
                     if (m < oc && co[m] == ob) ft = cf[m];
                 const float4 pb = a.P[cur][j];
-                const float2 db = a.D[cur][j];
+                const float4 db = del_of(a, j, dt);
                 const int mb = mw & PF_MAT_MASK;
                 const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
                 const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness,
@@ -762,7 +790,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             }
             const int j = a.INV[S.p0 + ob];
             const float4 pb = a.P[cur][j];
-            const float2 db = a.D[cur][j];
+            const float4 db = del_of(a, j, dt);
             const int mb = a.M[j].x & PF_MAT_MASK;
             const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
             const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness,
@@ -797,7 +825,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         if (n != cc_old) a.CC[s] = n;   // single-buffered: written only when it changes
 
         // 3. store_forces, 2D_Simulation.wgsl:434-478
-        const float density = MA.density;
+        const float density = ma_rho;
         const float rr = p.z;
         const float mass = density * PI_F * rr * rr;
         const float inertia = 0.5f * mass * rr * rr;
@@ -840,13 +868,13 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             a.DATA0[s] = make_float4(dd.d0, dd.d1, dd.d2, d0.w);
         }
         if (!(fl & SF_LOM)) {
-            a.V[s] = v;
+            // no drift follows: the kicked velocity no longer gives the displacement,
+            // which is kept explicitly
+            a.Vn[s] = make_float4(v.x, v.y, v.z, __uint_as_float(VW_EXPLICIT | (vword(v) & VW_PATH)));
+            a.DXn[s] = dme;
             a.A[s] = make_float4(ax, ay, al, 0.0f);
-            if (args.flip) {
-                // force pass only, while other scenes drift: positions carry over
-                a.P[nxt][s] = p;
-                a.D[nxt][s] = d;
-            }
+            // force pass only, while other scenes drift: positions carry over
+            if (args.flip) a.P[nxt][s] = p;
             return;
         }
     } else {
@@ -863,8 +891,15 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     if (pf & PF_KICK_Y) ivy += ay * dt * 0.5f;
     if (pf & PF_KICK_R) iw += al * dt * 0.5f;
     if (S.record) a.DATA1[s] = make_float4(ivx, ivy, iw, 0.0f);
-    float dpx = d.x, dpy = d.y, dr = p.w;
-    if (!(fl & SF_DELOVR)) {
+    float dpx, dpy, dr;
+    // a displacement that the next force half cannot re-form as velocity x dT
+    bool expl = (fl & SF_DELOVR) != 0;
+    if (expl) {
+        const float4 e = a.DX[s];   // explicit: written by the upload or the last force half
+        dpx = e.x;
+        dpy = e.y;
+        dr = e.z;
+    } else {
         dpx = ivx * dt;
         dpy = ivy * dt;
         dr = iw * dt;
@@ -879,21 +914,26 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             if (npy - rr < -yH) {
                 ivy = -ivy * 0.5f;
                 py += -yH - (npy - rr);
+                expl = true;
             } else if (npy + rr > yH) {
                 ivy = -ivy * 0.5f;
                 py -= (npy + rr) - yH;
+                expl = true;
             }
         }
         if (pf & PF_WALL_X) {
             if (npx - rr < -xW) {
                 ivx = -ivx * 0.5f;
                 px += -xW - (npx - rr);
+                expl = true;
             } else if (npx + rr > xW) {
                 ivx = -ivx * 0.5f;
                 px -= (npx + rr) - xW;
+                expl = true;
             }
         }
     } else if (lenf(npx, npy) + rr > u.bound_radius) {
+        expl = true;
         const float l = lenf(npx, npy);
         const float nx = npx / l, ny = npy / l;
         const float ddd = dpx * nx + dpy * ny;
@@ -911,17 +951,25 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     }
     px += dpx;
     py += dpy;
-    const float rot = v.w + dr;
-    a.P[nxt][s] = make_float4(px, py, rr, dr);
-    a.D[nxt][s] = make_float2(dpx, dpy);
-    a.V[s] = make_float4(ivx, ivy, iw, rot);
-    // moved more than 0.45 skin since the neighbour lists were built: rebuild them after
-    // this launch (every writer stores 1)
-    const float2 xr = a.XR[s];
-    const float mx = px - xr.x, my = py - xr.y;
-    const float mxx = mx * mx;
-    const float myy = my * my;
-    if (mxx + myy > S.trig2) a.RB[nxt] = 1;
+    const float rot = p.w + dr;
+    a.P[nxt][s] = make_float4(px, py, rr, rot);
+    if (expl) a.DXn[s] = make_float4(dpx, dpy, dr, 0.0f);
+    // Moved more than 0.45 skin since the neighbour lists were built: rebuild them after
+    // this launch (every writer stores 1).  The path word bounds the distance from above
+    // (|dx| + |dy| per drift, rounded up), so the build position is read only once the
+    // path passes 0.45 skin; the word then restarts from the exact distance.
+    uint32_t q = vword(v) & VW_PATH;
+    const float inc = fminf((fabsf(px - p.x) + fabsf(py - p.y)) * S.pscale * 1.0001f, 1.0e9f);
+    q = min(q + (uint32_t)ceilf(inc), VW_PATH_SAT);
+    if (q > VW_PATH_TRIG) {
+        const float2 xr = a.XR[s];
+        const float mx = px - xr.x, my = py - xr.y;
+        const float mxx = mx * mx;
+        const float myy = my * my;
+        if (mxx + myy > S.trig2) a.RB[nxt] = 1;
+        else q = min((uint32_t)ceilf(fminf((fabsf(mx) + fabsf(my)) * S.pscale * 1.0001f, 1.0e9f)), VW_PATH_SAT);
+    }
+    a.Vn[s] = make_float4(ivx, ivy, iw, __uint_as_float((expl ? VW_EXPLICIT : 0u) | q));
 }
 
 // ---- bond pass: one thread per bond endpoint record ----
@@ -934,11 +982,11 @@ __device__ __forceinline__ void bond_pass_body(const StepArgs& args, const Scene
     const DevArrays& a = args.a;
     const int cur = args.cur;
     const float4 p = a.P[cur][i];
-    const float2 d = a.D[cur][i];
+    const float4 d = del_of(a, i, S.u.dT);
     const int mat = (int)((uint32_t)a.M[i].x & PF_MAT_MASK);
     const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
     Self me;
-    me.x = p.x; me.y = p.y; me.r = p.z; me.drot = p.w; me.dx = d.x; me.dy = d.y;
+    me.x = p.x; me.y = p.y; me.r = p.z; me.drot = d.z; me.dx = d.x; me.dy = d.y;
     me.kn = MA.normal_stiffness; me.ks = MA.shear_stiffness;
     const uint32_t fl = (uint32_t)lfw.x;
     const bool narrow = (fl & SF_SIM) && (fl & SF_NARROW);
@@ -1204,6 +1252,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
     const int2 m = a.M[s];
     const float4 p = a.P[cur][s];
     a.XR[s] = make_float2(p.x, p.y);
+    a.V[s].w = __uint_as_float(__float_as_uint(a.V[s].w) & VW_EXPLICIT);   // path since the build: 0
     if (m.x & PF_INACTIVE) {
         a.NL[s] = nl_word(0, 0);
         return;
@@ -1354,8 +1403,9 @@ __global__ void k_reorder_move(ReorderArgs r) {
     const int s = r.order[j];
     const int cur = r.cur;
     r.dst.P[cur][j] = r.src.P[cur][s];
-    r.dst.D[cur][j] = r.src.D[cur][s];
-    r.dst.V[j] = r.src.V[s];
+    const float4 v = r.src.V[s];
+    r.dst.V[j] = v;
+    if (vword(v) & VW_EXPLICIT) r.dst.DX[j] = r.src.DX[s];
     const int2 m = r.src.M[s];
     r.dst.M[j] = m;
     r.dst.A[j] = r.src.A[s];
@@ -1501,13 +1551,15 @@ __global__ void k_bond_pairs(StepArgs args, int reach, int maxb, float factor, i
 hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, bool large,
                        hipStream_t st) {
     dim3 g(nblocks), b(block);
+    StepArgs a = args;
+    a.lbs = block;
     if (f64) {
-        if (bonds) hipLaunchKernelGGL((k_step_bonds<true>), g, b, 0, st, args);
-        else hipLaunchKernelGGL(k_step_plain_f64, g, b, 0, st, args);
+        if (bonds) hipLaunchKernelGGL((k_step_bonds<true>), g, b, 0, st, a);
+        else hipLaunchKernelGGL(k_step_plain_f64, g, b, 0, st, a);
     } else {
-        if (bonds) hipLaunchKernelGGL((k_step_bonds<false>), g, b, 0, st, args);
-        else if (large) hipLaunchKernelGGL(k_step_plain_large, g, b, 0, st, args);
-        else hipLaunchKernelGGL(k_step_plain, g, b, 0, st, args);
+        if (bonds) hipLaunchKernelGGL((k_step_bonds<false>), g, b, 0, st, a);
+        else if (large) hipLaunchKernelGGL(k_step_plain_large, g, b, 0, st, a);
+        else hipLaunchKernelGGL(k_step_plain, g, b, 0, st, a);
     }
     return hipGetLastError();
 }
