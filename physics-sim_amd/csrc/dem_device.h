@@ -58,6 +58,12 @@ enum : uint32_t {
     PF_LARGE = 1u << 24,    // a large body: on the scene's large-body list, not found through cells
     PF_MAT_MASK = 0xffffu,
 };
+// Bits 25..29 of M.x: the particle's contact count (entries of its contact list), kept
+// beside the flags so that the step kernel reads it with them; rewritten only when it
+// changes (neighbours read the word for its material bits, which never change).
+constexpr int PF_CC_SHIFT = 25;
+constexpr uint32_t PF_CC_MASK = 0x1fu << PF_CC_SHIFT;
+__host__ __device__ inline int cc_of(uint32_t mx) { return (int)((mx & PF_CC_MASK) >> PF_CC_SHIFT); }
 
 // per-step flags (one word per scene and step)
 enum : uint32_t {
@@ -122,14 +128,13 @@ struct DevArrays {
     Pool<float4> DX;        // {del_x, del_y, del_rot, 0}, valid where V.w has VW_EXPLICIT
     Pool<float4> DXn;
     Pool<float4> A;         // {ax, ay, alpha, 0}
-    Pool<int2> M;           // {material | flags, particle id within the scene}
+    Pool<int2> M;           // {material | flags | contact count, particle id within the scene}
     Pool<float4> EXT;       // external {fx, fy, moment, 0}
     Pool<float4> DATA0;     // {Σfn, Σft, ΣM, data3}
     Pool<float4> DATA1;     // {v*x, v*y, w*, 0}
     // contact lists, slot-major [KMAX][cap]: the partner and force columns are
     // double-buffered (every step kernel reads CP/CF and writes CPn/CFn); the count is
-    // part of the particle set and is rewritten only when it changes
-    Pool<int32_t> CC;       // count
+    // part of M.x (cc_of)
     Pool<int32_t> CP;       // partner id (scene-local), ascending
     Pool<float> CF;         // tangential force
     Pool<int32_t> CPn;

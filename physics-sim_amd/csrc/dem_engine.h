@@ -25,7 +25,6 @@ struct DevSet {
     float4* EXT = nullptr;
     float4* DATA0 = nullptr;
     float4* DATA1 = nullptr;
-    int32_t* CC = nullptr;   // contact count (written only when it changes)
     int32_t* NL = nullptr;   // neighbour lists [NLMAX][cap], row 0 packs the count
     float2* XR = nullptr;    // positions at the list build
     int2* BRS = nullptr;     // bond record range of each slot

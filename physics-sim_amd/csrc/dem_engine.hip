@@ -26,7 +26,7 @@ hipError_t launch_partition(const ReorderArgs& r, int nblocks, int block, hipStr
 hipError_t launch_bond_forces(const StepArgs& args, int nrec, int lbs, bool f64, hipStream_t st);
 hipError_t launch_record_slots(const StepArgs& args, int nrec, int lbs, hipStream_t st);
 hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, int np, double* out, hipStream_t st);
-hipError_t launch_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out, hipStream_t st);
+hipError_t launch_count_contacts(const int2* m, int p0, int np, unsigned long long* out, hipStream_t st);
 hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
                              int32_t* out_cnt, int32_t* out_j, hipStream_t st);
 
@@ -98,7 +98,7 @@ Engine::~Engine() {
         hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.V[0]); hipFree(s.V[1]); hipFree(s.DX[0]); hipFree(s.DX[1]);
         hipFree(s.A); hipFree(s.M); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
     }
-    for (auto& s : sets_) { hipFree(s.CC); hipFree(s.NL); hipFree(s.XR); hipFree(s.BRS); }
+    for (auto& s : sets_) { hipFree(s.NL); hipFree(s.XR); hipFree(s.BRS); }
     for (auto& l : lists_) { hipFree(l.CP); hipFree(l.CF); }
     hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BL_); hipFree(BSTR_); hipFree(MAT_);
     hipFree(KEY_); hipFree(RANK_); hipFree(CNT_); hipFree(START_); hipFree(SIDX_); hipFree(INV_);
@@ -153,7 +153,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
         e = e ? e : dalloc(&s.V[0], S); e = e ? e : dalloc(&s.V[1], S);
         e = e ? e : dalloc(&s.DX[0], S); e = e ? e : dalloc(&s.DX[1], S);
         e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S);
-        e = e ? e : dalloc(&s.EXT, S); e = e ? e : dalloc(&s.CC, S);
+        e = e ? e : dalloc(&s.EXT, S);
         e = e ? e : dalloc(&s.NL, S * NLMAX); e = e ? e : dalloc(&s.XR, S); e = e ? e : dalloc(&s.BRS, S);
         if (with_data) { e = e ? e : dalloc(&s.DATA0, S); e = e ? e : dalloc(&s.DATA1, S); }
         if ((rc = check(e, "hipMalloc particle pools"))) return rc;
@@ -203,7 +203,6 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     for (auto& st : sets_) e = e ? e : hipMemsetAsync(st.NL, 0, sizeof(int32_t) * S, stream_);   // empty lists
     e = e ? e : hipMemsetAsync(BOFF_, 0, sizeof(int2) * S, stream_);
     for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int2) * S, stream_);
-    for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.CC, 0, sizeof(int32_t) * S, stream_);
     for (auto& s : sets_)
         for (int b = 0; b < 2; b++) {
             e = e ? e : hipMemsetAsync(s.V[b], 0, sizeof(float4) * S, stream_);
@@ -484,7 +483,7 @@ DevArrays Engine::arrays(int set) const {
     a.A = s.A; a.M = s.M; a.EXT = s.EXT; a.DATA0 = s.DATA0; a.DATA1 = s.DATA1;
     const ContactLists& lc = lists_[lcur_];
     const ContactLists& ln = lists_[lcur_ ^ 1];
-    a.CC = s.CC; a.CP = lc.CP; a.CF = lc.CF;
+    a.CP = lc.CP; a.CF = lc.CF;
     a.CPn = ln.CP; a.CFn = ln.CF;
     a.cap = slot_cap_;
     a.BOFF = BOFF_; a.BR = BR_; a.BF = BF_; a.BS = BS_; a.BL = BL_; a.BSTR = BSTR_; a.MAT = MAT_;
@@ -668,6 +667,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
             }
             CC[i] = (int32_t)con.size();
             rows = std::max(rows, CC[i]);
+            M[i].x = (int)(((uint32_t)M[i].x & ~PF_CC_MASK) | ((uint32_t)CC[i] << PF_CC_SHIFT));
             for (size_t q = 0; q < con.size(); q++) {
                 const Slot& s = tab[(size_t)i * K + con[q].second];
                 CP[(size_t)q * np + i] = s.b;
@@ -708,7 +708,6 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         h2d(S.DATA1 + p0, D1, sizeof(float4) * np);
     }
     const ContactLists& L = lists_[lcur_];
-    h2d(S.CC + p0, CC, sizeof(int32_t) * np);
     if (rows > 0 && e == hipSuccess) {
         // rows [0, rows) of the slot-major lists in one strided copy each
         e = hipMemcpy2DAsync(L.CP + p0, sizeof(int32_t) * slot_cap_, CP, sizeof(int32_t) * np, sizeof(int32_t) * np,
@@ -1085,7 +1084,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     Carve cv;
     auto carve = [&](Carve& c) {
         c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np);
-        c.take<int2>(np); c.take<int32_t>(np); c.take<int32_t>(np); c.take<int32_t>((size_t)np * KMAX);
+        c.take<int2>(np); c.take<int32_t>(np); c.take<int32_t>((size_t)np * KMAX);
         c.take<float>((size_t)np * KMAX); c.take<int2>(np); c.take<int4>(nrec); c.take<float4>(nrec);
         c.take<int2>(nbs);
     };
@@ -1100,7 +1099,6 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     float4* D1 = cv.take<float4>(np);
     int2* M = cv.take<int2>(np);
     int32_t* INV = cv.take<int32_t>(np);
-    int32_t* CC = cv.take<int32_t>(np);
     int32_t* CP = cv.take<int32_t>((size_t)np * KMAX);
     float* CF = cv.take<float>((size_t)np * KMAX);
     int2* BRANGE = cv.take<int2>(np);
@@ -1123,7 +1121,6 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     }
     if (v.contacts) {
         const ContactLists& L = lists_[lcur_];
-        d2h(CC, sets_[set_].CC + p0, sizeof(int32_t) * np);
         // the slot-major lists, KMAX rows in one strided copy each
         if (e == hipSuccess)
             e = hipMemcpy2DAsync(CP, sizeof(int32_t) * np, L.CP + p0, sizeof(int32_t) * slot_cap_,
@@ -1177,7 +1174,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
         for (int32_t i = 0; i < n; i++) {
             row.clear();
             const int64_t s = (int64_t)INV[i] - p0;
-            for (int q = 0; q < CC[s] && q < KMAX; q++) {
+            for (int q = 0; q < cc_of((uint32_t)M[s].x) && q < KMAX; q++) {
                 Slot x;
                 x.a = i;
                 x.b = CP[(size_t)q * np + s];
@@ -1250,7 +1247,7 @@ int Engine::counters(int32_t sid, int64_t* out, int32_t n_out) {
     unsigned long long st[8] = {0}, nc = 0;
     hipError_t e = hipMemcpyAsync(st, STAT_, sizeof(st), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipMemsetAsync(d_cnt_, 0, sizeof(unsigned long long), stream_);
-    if (e == hipSuccess) e = launch_count_contacts(sets_[set_].CC, (int)sh.p0, sh.np, d_cnt_, stream_);
+    if (e == hipSuccess) e = launch_count_contacts(sets_[set_].M, (int)sh.p0, sh.np, d_cnt_, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(&nc, d_cnt_, sizeof(nc), hipMemcpyDeviceToHost, stream_);
     int64_t nrec_alive = 0;
     std::vector<int4> BR(std::max(sh.nrec, 1));
@@ -1336,8 +1333,9 @@ int Engine::debug_contacts(int32_t sid, int32_t max_per, int32_t* counts, int32_
     if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
     const SceneHost& sh = scenes_[sid];
     const int np = sh.np;
-    std::vector<int32_t> CC(np), CP((size_t)np * KMAX), INV(np);
-    hipError_t e = hipMemcpyAsync(CC.data(), sets_[set_].CC + sh.p0, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
+    std::vector<int2> M(np);
+    std::vector<int32_t> CP((size_t)np * KMAX), INV(np);
+    hipError_t e = hipMemcpyAsync(M.data(), sets_[set_].M + sh.p0, sizeof(int2) * np, hipMemcpyDeviceToHost, stream_);
     for (int q = 0; q < KMAX && e == hipSuccess; q++)
         e = hipMemcpyAsync(CP.data() + (size_t)q * np, lists_[lcur_].CP + (int64_t)q * slot_cap_ + sh.p0,
                            sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
@@ -1348,7 +1346,7 @@ int Engine::debug_contacts(int32_t sid, int32_t max_per, int32_t* counts, int32_
     bool over = false;
     for (int32_t i = 0; i < sh.n; i++) {
         const int64_t s = (int64_t)INV[i] - sh.p0;
-        const int c = CC[s];
+        const int c = cc_of((uint32_t)M[s].x);
         counts[i] = c;
         for (int k = 0; k < max_per; k++)
             partners[(size_t)i * max_per + k] = (k < c && k < KMAX) ? CP[(size_t)k * np + s] : -1;

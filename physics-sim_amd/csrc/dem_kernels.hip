@@ -476,7 +476,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const uint32_t fl = (uint32_t)lfw.x;
     const int cur = args.cur, nxt = cur ^ 1;
     if (!(fl & SF_ACTIVE)) {
-        copy_list(a, s, a.CC[s]);  // the lists and velocities flip after every launch
+        copy_list(a, s, cc_of((uint32_t)a.M[s].x));  // the lists and velocities flip after every launch
         carry_velocity(a, s);
         if (args.flip) a.P[nxt][s] = a.P[cur][s];
         return;
@@ -530,7 +530,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     // own-state loads issued before the contact search so that their latency
     // overlaps it (the compiler keeps them behind the list stores otherwise)
     float4 v = a.V[s];
-    const int cc_old = a.CC[s];
+    const int cc_old = cc_of(pf);
     const int mat = (int)(pf & PF_MAT_MASK);
     // a scene of one material (the common case) reads its record without waiting for
     // the particle's material word
@@ -822,7 +822,8 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 bq += m;
             }
         }
-        if (n != cc_old) a.CC[s] = n;   // single-buffered: written only when it changes
+        // single-buffered: written only when it changes
+        if (n != cc_old) a.M[s].x = (int)((pf & ~PF_CC_MASK) | ((uint32_t)n << PF_CC_SHIFT));
 
         // 3. store_forces, 2D_Simulation.wgsl:434-478
         const float density = ma_rho;
@@ -1047,12 +1048,13 @@ __device__ __forceinline__ bool block_scene_f64(const StepArgs& args) {
     return sid >= 0 && args.scenes[sid].use_f64 != 0;
 }
 
-// Plain f32 scenes: 8 waves per SIMD (at most 64 VGPRs) to hide the latency of the
-// candidate loop; launches with a large body take k_step_plain_large, whose stencil
-// fallback also covers them.  Launches holding an f64 scene (k_step_plain_f64) and
-// bonded scenes carry more live state and keep the default register budget (8 waves
-// would spill); both handle large bodies.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_plain(StepArgs args) {
+// Plain f32 scenes: at least 7 waves per SIMD (at most 72 VGPRs) to hide the latency
+// of the candidate loop (at 64 VGPRs the contact loop spills, and a kernel with
+// scratch costs more per launch); launches with a large body take
+// k_step_plain_large, whose stencil fallback also covers them.  Launches holding an
+// f64 scene (k_step_plain_f64) and bonded scenes carry more live state and keep the
+// default register budget; both handle large bodies.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) k_step_plain(StepArgs args) {
     step_slot<false, false, false>(args);
 }
 __global__ void __launch_bounds__(256) k_step_plain_large(StepArgs args) {
@@ -1414,8 +1416,7 @@ __global__ void k_reorder_move(ReorderArgs r) {
         r.dst.DATA0[j] = r.src.DATA0[s];
         r.dst.DATA1[j] = r.src.DATA1[s];
     }
-    const int c = r.src.CC[s];
-    r.dst.CC[j] = c;
+    const int c = cc_of((uint32_t)m.x);
     const int cap = (int)r.src.cap;
     for (int q = 0; q < c && q < KMAX; q++) {
         r.dst.CP[q * cap + j] = r.src.CP[q * cap + s];
@@ -1486,11 +1487,11 @@ __global__ void k_kinetic(DevArrays a, int cur, int m0, int nm, int p0, int np, 
 // order independent.  A grid of COUNT_BLOCKS strides over the scene, so only that
 // many same-address atomics meet in L2.
 constexpr int COUNT_BLOCKS = 256;
-__global__ void __launch_bounds__(256) k_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out) {
+__global__ void __launch_bounds__(256) k_count_contacts(const int2* m, int p0, int np, unsigned long long* out) {
     __shared__ int s_w[4];
     int c = 0;
     for (int s = (int)(blockIdx.x * blockDim.x + threadIdx.x); s < np; s += (int)(gridDim.x * blockDim.x))
-        c += cc[p0 + s];
+        c += cc_of((uint32_t)m[p0 + s].x);
     c = wave_sum(c);
     if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
     __syncthreads();
@@ -1643,10 +1644,10 @@ hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, i
     return hipGetLastError();
 }
 
-hipError_t launch_count_contacts(const int32_t* cc, int p0, int np, unsigned long long* out, hipStream_t st) {
+hipError_t launch_count_contacts(const int2* m, int p0, int np, unsigned long long* out, hipStream_t st) {
     const int b = 256;
     const int nb = (np + b - 1) / b;
-    hipLaunchKernelGGL(k_count_contacts, dim3(nb < COUNT_BLOCKS ? nb : COUNT_BLOCKS), dim3(b), 0, st, cc, p0, np, out);
+    hipLaunchKernelGGL(k_count_contacts, dim3(nb < COUNT_BLOCKS ? nb : COUNT_BLOCKS), dim3(b), 0, st, m, p0, np, out);
     return hipGetLastError();
 }
 
