@@ -22,8 +22,8 @@
 //              (slot -> position START[key] + RANK in the cell-ordered slot list) and
 //              k_build_nl (the lists) - stages 1, 2b, 3 of the reference's broad phase.
 //   k_partition, k_reorder_move   every few rebuilds: move every per-particle
-//              array, the neighbour lists included, into cell-list order, each block
-//              partitioned into particles with list candidates first (k_cell_sort
+//              array, the neighbour lists included, into cell-list order, each unit of
+//              32 blocks partitioned into particles with list candidates first (k_cell_sort
 //              orders each cell by id for the debug view of the broad phase only).
 //
 // Arithmetic follows the WGSL expression by expression (operand order, no fused
@@ -1432,36 +1432,83 @@ __global__ void k_reorder_move(ReorderArgs r) {
     r.dst.INV[S.p0 + m.y] = j;
 }
 
-// 0. the destination of every slot: each block of list positions (the scene table's
-//    blocks, so no block straddles scenes) keeps its positions, partitioned stably
-//    into particles with neighbour-list candidates first.  The candidate lanes of a
-//    block then fill whole waves and the other waves skip the narrow phase.  The cell
-//    list is renamed to the destination slots in place.
-__global__ void k_partition(ReorderArgs r) {
-    const int lb = (int)blockIdx.x;
-    const int sid = r.blk_scene[lb];
-    if (sid < 0) return;
-    const SceneDev& S = r.scenes[sid];
-    const int j = lb * (int)blockDim.x + (int)threadIdx.x;
-    if (j < S.p0 || j >= S.p0 + S.np) return;   // whole blocks (np is a multiple of the block)
-    int s = r.src.SIDX[j];
+// 0. the destination of every slot.  Units of PART_BLOCKS consecutive blocks of list
+//    positions (cut at scene boundaries: the scene table's blocks never straddle
+//    scenes) keep their positions, partitioned stably into particles with neighbour-
+//    list candidates first.  The few candidates of a unit then fill whole waves at its
+//    start and every other wave skips the narrow phase, while particles stay within
+//    the unit (8192 slots at blocks of 256) and so near their cell-order neighbours.
+//    The cell list is renamed to the destination slots in place.  One workgroup per
+//    unit, blockDim.x = the logical block size.
+constexpr int PART_BLOCKS = 32;
+
+__device__ __forceinline__ bool part_flag(const ReorderArgs& r, const SceneDev& S, int j, int& s) {
+    s = r.src.SIDX[j];
     if (s < S.p0 || s >= S.p0 + S.np) s = j;     // a damaged list keeps the slot in place
-    const bool f = nl_count(r.src.NL[s]) != 0;
-    __shared__ int s_cnt[16];                    // flagged lanes per wave (blockDim <= 1024)
-    const uint64_t bf = __ballot(f);
-    const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
-    if (lane == 0) s_cnt[wv] = __popcll(bf);
-    __syncthreads();
-    int pre = 0, tot = 0;
-    for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
-        if (q < wv) pre += s_cnt[q];
-        tot += s_cnt[q];
+    return nl_count(r.src.NL[s]) != 0;
+}
+
+__global__ void __launch_bounds__(256) k_partition(ReorderArgs r, int nlb) {
+    const int lbs = (int)blockDim.x;
+    const int b0 = (int)blockIdx.x * PART_BLOCKS;
+    const int b1 = min(b0 + PART_BLOCKS, nlb);
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6, nw = lbs >> 6;
+    __shared__ int s_wc[4];                // flagged lanes per wave of the current block
+    __shared__ int s_cnt[PART_BLOCKS];     // flagged slots per block, then their exclusive prefix
+    __shared__ int s_tot;
+    int bs = b0;
+    while (bs < b1) {                      // segments: runs of blocks of one scene
+        const int sid = r.blk_scene[bs];
+        int be = bs + 1;
+        while (be < b1 && r.blk_scene[be] == sid) be++;
+        if (sid >= 0) {
+            const SceneDev& S = r.scenes[sid];
+            for (int b = bs; b < be; b++) {
+                int s;
+                const bool f = part_flag(r, S, b * lbs + tid, s);
+                const uint64_t bf = __ballot(f);
+                if (lane == 0) s_wc[wv] = __popcll(bf);
+                __syncthreads();
+                if (tid == 0) {
+                    int t = 0;
+                    for (int q = 0; q < nw; q++) t += s_wc[q];
+                    s_cnt[b - bs] = t;
+                }
+                __syncthreads();
+            }
+            if (tid == 0) {
+                int run = 0;
+                for (int q = 0; q < be - bs; q++) {
+                    const int c = s_cnt[q];
+                    s_cnt[q] = run;
+                    run += c;
+                }
+                s_tot = run;
+            }
+            __syncthreads();
+            const int seg0 = bs * lbs;
+            for (int b = bs; b < be; b++) {
+                const int j = b * lbs + tid;
+                int s;
+                const bool f = part_flag(r, S, j, s);
+                const uint64_t bf = __ballot(f);
+                if (lane == 0) s_wc[wv] = __popcll(bf);
+                __syncthreads();
+                int pre = 0;
+                for (int q = 0; q < wv; q++) pre += s_wc[q];
+                const int rf = pre + __popcll(bf & ((1ull << lane) - 1ull));   // flagged lanes before this one
+                const int cb = s_cnt[b - bs];                                  // flagged slots before block b
+                const int dst = seg0 + (f ? cb + rf : s_tot + ((b - bs) * lbs - cb) + (tid - rf));
+                r.order[dst] = s;
+                r.newslot[s] = dst;
+                r.src.SIDX[j] = dst;
+                __syncthreads();   // s_wc is rewritten by the next block
+            }
+        }
+        bs = be;
+        __syncthreads();       // s_cnt / s_tot are rewritten by the next segment
     }
-    const int rf = pre + __popcll(bf & ((1ull << lane) - 1ull));   // flagged lanes before this one
-    const int dst = lb * (int)blockDim.x + (f ? rf : tot + ((int)threadIdx.x - rf));
-    r.order[dst] = s;
-    r.newslot[s] = dst;
-    r.src.SIDX[j] = dst;
 }
 
 // ---- observables ----
@@ -1634,7 +1681,7 @@ hipError_t launch_reorder_move(const ReorderArgs& r, int nblocks, int block, hip
 }
 
 hipError_t launch_partition(const ReorderArgs& r, int nblocks, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_partition, dim3(nblocks), dim3(block), 0, st, r);
+    hipLaunchKernelGGL(k_partition, dim3((nblocks + PART_BLOCKS - 1) / PART_BLOCKS), dim3(block), 0, st, r, nblocks);
     return hipGetLastError();
 }
 
