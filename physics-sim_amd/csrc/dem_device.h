@@ -128,7 +128,8 @@ struct DevArrays {
     Pool<float4> DX;        // {del_x, del_y, del_rot, 0}, valid where V.w has VW_EXPLICIT
     Pool<float4> DXn;
     Pool<float4> A;         // {ax, ay, alpha, 0}
-    Pool<int2> M;           // {material | flags | contact count, particle id within the scene}
+    Pool<int32_t> M;        // material | flags | contact count (the word the step kernel reads)
+    Pool<int32_t> ID;       // particle id within the scene (read for partners only)
     Pool<float4> EXT;       // external {fx, fy, moment, 0}
     Pool<float4> DATA0;     // {Σfn, Σft, ΣM, data3}
     Pool<float4> DATA1;     // {v*x, v*y, w*, 0}

@@ -21,7 +21,8 @@ struct DevSet {
     float4* V[2] = {nullptr, nullptr};    // by list parity (lcur_), as DX
     float4* DX[2] = {nullptr, nullptr};
     float4* A = nullptr;
-    int2* M = nullptr;
+    int32_t* M = nullptr;    // material | flags | contact count
+    int32_t* ID = nullptr;   // particle id within the scene
     float4* EXT = nullptr;
     float4* DATA0 = nullptr;
     float4* DATA1 = nullptr;

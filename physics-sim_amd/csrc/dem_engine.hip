@@ -26,7 +26,7 @@ hipError_t launch_partition(const ReorderArgs& r, int nblocks, int block, hipStr
 hipError_t launch_bond_forces(const StepArgs& args, int nrec, int lbs, bool f64, hipStream_t st);
 hipError_t launch_record_slots(const StepArgs& args, int nrec, int lbs, hipStream_t st);
 hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, int np, double* out, hipStream_t st);
-hipError_t launch_count_contacts(const int2* m, int p0, int np, unsigned long long* out, hipStream_t st);
+hipError_t launch_count_contacts(const int32_t* m, int p0, int np, unsigned long long* out, hipStream_t st);
 hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
                              int32_t* out_cnt, int32_t* out_j, hipStream_t st);
 
@@ -96,7 +96,7 @@ Engine::~Engine() {
     if (stream_) hipStreamSynchronize(stream_);
     for (auto& s : sets_) {
         hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.V[0]); hipFree(s.V[1]); hipFree(s.DX[0]); hipFree(s.DX[1]);
-        hipFree(s.A); hipFree(s.M); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
+        hipFree(s.A); hipFree(s.M); hipFree(s.ID); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
     }
     for (auto& s : sets_) { hipFree(s.NL); hipFree(s.XR); hipFree(s.BRS); }
     for (auto& l : lists_) { hipFree(l.CP); hipFree(l.CF); }
@@ -152,7 +152,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
         e = e ? e : dalloc(&s.P[0], S); e = e ? e : dalloc(&s.P[1], S);
         e = e ? e : dalloc(&s.V[0], S); e = e ? e : dalloc(&s.V[1], S);
         e = e ? e : dalloc(&s.DX[0], S); e = e ? e : dalloc(&s.DX[1], S);
-        e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S);
+        e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S); e = e ? e : dalloc(&s.ID, S);
         e = e ? e : dalloc(&s.EXT, S);
         e = e ? e : dalloc(&s.NL, S * NLMAX); e = e ? e : dalloc(&s.XR, S); e = e ? e : dalloc(&s.BRS, S);
         if (with_data) { e = e ? e : dalloc(&s.DATA0, S); e = e ? e : dalloc(&s.DATA1, S); }
@@ -202,7 +202,8 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     for (auto& st : sets_) e = e ? e : hipMemsetAsync(st.BRS, 0, sizeof(int2) * S, stream_);
     for (auto& st : sets_) e = e ? e : hipMemsetAsync(st.NL, 0, sizeof(int32_t) * S, stream_);   // empty lists
     e = e ? e : hipMemsetAsync(BOFF_, 0, sizeof(int2) * S, stream_);
-    for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int2) * S, stream_);
+    for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.M, 0xff, sizeof(int32_t) * S, stream_);
+    for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.ID, 0xff, sizeof(int32_t) * S, stream_);
     for (auto& s : sets_)
         for (int b = 0; b < 2; b++) {
             e = e ? e : hipMemsetAsync(s.V[b], 0, sizeof(float4) * S, stream_);
@@ -480,7 +481,7 @@ DevArrays Engine::arrays(int set) const {
     DevArrays a{};
     a.P[0] = s.P[0]; a.P[1] = s.P[1];
     a.V = s.V[lcur_]; a.Vn = s.V[lcur_ ^ 1]; a.DX = s.DX[lcur_]; a.DXn = s.DX[lcur_ ^ 1];
-    a.A = s.A; a.M = s.M; a.EXT = s.EXT; a.DATA0 = s.DATA0; a.DATA1 = s.DATA1;
+    a.A = s.A; a.M = s.M; a.ID = s.ID; a.EXT = s.EXT; a.DATA0 = s.DATA0; a.DATA1 = s.DATA1;
     const ContactLists& lc = lists_[lcur_];
     const ContactLists& ln = lists_[lcur_ ^ 1];
     a.CP = lc.CP; a.CF = lc.CF;
@@ -527,8 +528,8 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     // the device layout is built in pinned staging memory and copied from there
     Carve cv;
     auto carve = [&](Carve& c) {
-        c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<int2>(np);
-        c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<int32_t>(np);
+        c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<int32_t>(np);
+        c.take<int32_t>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<int32_t>(np);
         c.take<int32_t>((size_t)np * KMAX); c.take<float>((size_t)np * KMAX); c.take<int32_t>(np);
         c.take<int32_t>(np); c.take<int2>(np);
     };
@@ -540,7 +541,8 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     float4* DX = cv.take<float4>(np);
     float4* V = cv.take<float4>(np);
     float4* A = cv.take<float4>(np);
-    int2* M = cv.take<int2>(np);
+    int32_t* M = cv.take<int32_t>(np);    // flag words
+    int32_t* ID = cv.take<int32_t>(np);
     float4* EXT = cv.take<float4>(np);
     float4* D0 = cv.take<float4>(np);
     float4* D1 = cv.take<float4>(np);
@@ -567,7 +569,8 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         SIDX[i] = (int32_t)sh.p0 + i;
         if (i >= n) {
             P[i] = DX[i] = V[i] = A[i] = EXT[i] = D0[i] = D1[i] = make_float4(0, 0, 0, 0);
-            M[i] = make_int2((int)PF_INACTIVE, i);
+            M[i] = (int32_t)PF_INACTIVE;
+            ID[i] = i;
             continue;
         }
         const float r = v.radii[i];
@@ -595,7 +598,8 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         if (fbits(f[0]) || fbits(f[1]) || fbits(f[2])) { flags |= PF_EXT; sh.has_ext = true; }
         if (r == 0.0f) flags |= PF_INACTIVE;
         if (nlarge > 0 && r > rsmax) flags |= PF_LARGE;
-        M[i] = make_int2((int)flags, i);
+        M[i] = (int32_t)flags;
+        ID[i] = i;
         if (v.data) {
             const float* d = v.data + 7 * (size_t)i;
             D0[i] = make_float4(d[0], d[1], d[2], d[3]);
@@ -667,7 +671,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
             }
             CC[i] = (int32_t)con.size();
             rows = std::max(rows, CC[i]);
-            M[i].x = (int)(((uint32_t)M[i].x & ~PF_CC_MASK) | ((uint32_t)CC[i] << PF_CC_SHIFT));
+            M[i] = (int32_t)(((uint32_t)M[i] & ~PF_CC_MASK) | ((uint32_t)CC[i] << PF_CC_SHIFT));
             for (size_t q = 0; q < con.size(); q++) {
                 const Slot& s = tab[(size_t)i * K + con[q].second];
                 CP[(size_t)q * np + i] = s.b;
@@ -701,7 +705,8 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     h2d(S.DX[lcur_] + p0, DX, sizeof(float4) * np);
     h2d(S.V[lcur_] + p0, V, sizeof(float4) * np);
     h2d(S.A + p0, A, sizeof(float4) * np);
-    h2d(S.M + p0, M, sizeof(int2) * np);
+    h2d(S.M + p0, M, sizeof(int32_t) * np);
+    h2d(S.ID + p0, ID, sizeof(int32_t) * np);
     h2d(S.EXT + p0, EXT, sizeof(float4) * np);
     if (with_data_) {
         h2d(S.DATA0 + p0, D0, sizeof(float4) * np);
@@ -1084,7 +1089,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     Carve cv;
     auto carve = [&](Carve& c) {
         c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np);
-        c.take<int2>(np); c.take<int32_t>(np); c.take<int32_t>((size_t)np * KMAX);
+        c.take<int32_t>(np); c.take<int32_t>(np); c.take<int32_t>((size_t)np * KMAX);
         c.take<float>((size_t)np * KMAX); c.take<int2>(np); c.take<int4>(nrec); c.take<float4>(nrec);
         c.take<int2>(nbs);
     };
@@ -1097,7 +1102,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     float4* A = cv.take<float4>(np);
     float4* D0 = cv.take<float4>(np);
     float4* D1 = cv.take<float4>(np);
-    int2* M = cv.take<int2>(np);
+    int32_t* M = cv.take<int32_t>(np);
     int32_t* INV = cv.take<int32_t>(np);
     int32_t* CP = cv.take<int32_t>((size_t)np * KMAX);
     float* CF = cv.take<float>((size_t)np * KMAX);
@@ -1112,7 +1117,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     d2h(P, S.P[cur_] + p0, sizeof(float4) * np);
     d2h(V, S.V[lcur_] + p0, sizeof(float4) * np);
     d2h(A, S.A + p0, sizeof(float4) * np);
-    d2h(M, S.M + p0, sizeof(int2) * np);
+    d2h(M, S.M + p0, sizeof(int32_t) * np);
     d2h(INV, INV_ + p0, sizeof(int32_t) * np);
     const bool rec = with_data_ && sh.params.record_data && v.data;
     if (rec) {
@@ -1174,7 +1179,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
         for (int32_t i = 0; i < n; i++) {
             row.clear();
             const int64_t s = (int64_t)INV[i] - p0;
-            for (int q = 0; q < cc_of((uint32_t)M[s].x) && q < KMAX; q++) {
+            for (int q = 0; q < cc_of((uint32_t)M[s]) && q < KMAX; q++) {
                 Slot x;
                 x.a = i;
                 x.b = CP[(size_t)q * np + s];
@@ -1310,20 +1315,20 @@ int Engine::debug_broad_phase(int32_t sid, int32_t* keys, int32_t* sorted_ids, i
     if (e == hipSuccess) e = launch_scatter(args, active_blocks(), block_, stream_);
     if (e == hipSuccess) e = launch_cell_sort(args, (int)sh.c0, (int)(sh.c0 + ncl + 1), (int)sh.p0, stream_);
     std::vector<int32_t> KEY(sh.np), SIDX(sh.np), START(ncl + 2);
-    std::vector<int2> M(sh.np);
+    std::vector<int32_t> ID(sh.np);
     if (e == hipSuccess) e = hipMemcpyAsync(KEY.data(), KEY_ + sh.p0, sizeof(int32_t) * sh.np, hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(SIDX.data(), SIDX_ + sh.p0, sizeof(int32_t) * sh.np, hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(START.data(), START_ + sh.c0, sizeof(int32_t) * (ncl + 1), hipMemcpyDeviceToHost, stream_);
-    if (e == hipSuccess) e = hipMemcpyAsync(M.data(), sets_[set_].M + sh.p0, sizeof(int2) * sh.np, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(ID.data(), sets_[set_].ID + sh.p0, sizeof(int32_t) * sh.np, hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     if ((rc = check(e, "debug broad phase"))) return rc;
     // the cell lists now describe the current storage; the next step re-bins anyway
     for (int32_t s = 0; s < sh.np; s++) {
-        const int32_t id = M[s].y;
+        const int32_t id = ID[s];
         if (id < sh.n && keys) keys[id] = KEY[s] - (int32_t)sh.c0;
     }
     if (sorted_ids)
-        for (int32_t e2 = 0; e2 < sh.n; e2++) sorted_ids[e2] = M[SIDX[e2] - sh.p0].y;
+        for (int32_t e2 = 0; e2 < sh.n; e2++) sorted_ids[e2] = ID[SIDX[e2] - sh.p0];
     if (cell_start)
         for (int32_t c = 0; c <= ncl; c++) cell_start[c] = START[c] - START[0];
     return DEM_OK;
@@ -1333,9 +1338,9 @@ int Engine::debug_contacts(int32_t sid, int32_t max_per, int32_t* counts, int32_
     if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
     const SceneHost& sh = scenes_[sid];
     const int np = sh.np;
-    std::vector<int2> M(np);
+    std::vector<int32_t> M(np);
     std::vector<int32_t> CP((size_t)np * KMAX), INV(np);
-    hipError_t e = hipMemcpyAsync(M.data(), sets_[set_].M + sh.p0, sizeof(int2) * np, hipMemcpyDeviceToHost, stream_);
+    hipError_t e = hipMemcpyAsync(M.data(), sets_[set_].M + sh.p0, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
     for (int q = 0; q < KMAX && e == hipSuccess; q++)
         e = hipMemcpyAsync(CP.data() + (size_t)q * np, lists_[lcur_].CP + (int64_t)q * slot_cap_ + sh.p0,
                            sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
@@ -1346,7 +1351,7 @@ int Engine::debug_contacts(int32_t sid, int32_t max_per, int32_t* counts, int32_
     bool over = false;
     for (int32_t i = 0; i < sh.n; i++) {
         const int64_t s = (int64_t)INV[i] - sh.p0;
-        const int c = cc_of((uint32_t)M[s].x);
+        const int c = cc_of((uint32_t)M[s]);
         counts[i] = c;
         for (int k = 0; k < max_per; k++)
             partners[(size_t)i * max_per + k] = (k < c && k < KMAX) ? CP[(size_t)k * np + s] : -1;

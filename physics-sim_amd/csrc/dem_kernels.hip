@@ -23,7 +23,7 @@
 //              k_build_nl (the lists) - stages 1, 2b, 3 of the reference's broad phase.
 //   k_partition, k_reorder_move   every few rebuilds: move every per-particle
 //              array, the neighbour lists included, into cell-list order, each unit of
-//              32 blocks partitioned into particles with list candidates first (k_cell_sort
+//              64 blocks partitioned into particles with list candidates first (k_cell_sort
 //              orders each cell by id for the debug view of the broad phase only).
 //
 // Arithmetic follows the WGSL expression by expression (operand order, no fused
@@ -429,7 +429,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
         const int2 bs = bid_ok ? a.BS[S.b0 + bid] : make_int2(0, 0);
         if (bs.x < 0 && bs.y < step) bonded = bs.x;
     }
-    const int mb = a.M[j].x & PF_MAT_MASK;
+    const int mb = a.M[j] & PF_MAT_MASK;
     const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
     float3 f;
     if (bonded < 0 || u.bonds == 0) {
@@ -476,15 +476,14 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const uint32_t fl = (uint32_t)lfw.x;
     const int cur = args.cur, nxt = cur ^ 1;
     if (!(fl & SF_ACTIVE)) {
-        copy_list(a, s, cc_of((uint32_t)a.M[s].x));  // the lists and velocities flip after every launch
+        copy_list(a, s, cc_of((uint32_t)a.M[s]));  // the lists and velocities flip after every launch
         carry_velocity(a, s);
         if (args.flip) a.P[nxt][s] = a.P[cur][s];
         return;
     }
     const dem_uniform& u = S.u;
     const float dt = u.dT;
-    const int2 mw = a.M[s];
-    const uint32_t pf = (uint32_t)mw.x;
+    const uint32_t pf = (uint32_t)a.M[s];
     const bool act = !(pf & PF_INACTIVE);
     float4 p = a.P[cur][s];
     const int nlw = a.NL[s];     // issued with the other own-state loads, used if the narrow phase runs
@@ -550,7 +549,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     // listed: a neighbour-list candidate (never a live bond partner)
     auto test = [&](int j, const float4& q, bool listed = false) {
         if (j != s && overlaps(q, p)) {
-            const int oj = a.M[j].y;
+            const int oj = a.ID[j];
             if (listed || !is_bonded(oj)) list_insert(a, s, oj, n);
         }
     };
@@ -583,7 +582,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (jj[k] >= 0 && jj[k] != s && overlaps(qq[k], p)) {
-                    const int2 mj = a.M[jj[k]];
+                    const int2 mj = make_int2(a.M[jj[k]], a.ID[jj[k]]);
                     fo[k] = mj.y;
                     fj[k] = jj[k];
                     fm[k] = mj.x;
@@ -791,7 +790,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             const int j = a.INV[S.p0 + ob];
             const float4 pb = a.P[cur][j];
             const float4 db = del_of(a, j, dt);
-            const int mb = a.M[j].x & PF_MAT_MASK;
+            const int mb = a.M[j] & PF_MAT_MASK;
             const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
             const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness,
                                                MB.shear_stiffness, me.r + pb.z, ft, dd);
@@ -823,7 +822,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             }
         }
         // single-buffered: written only when it changes
-        if (n != cc_old) a.M[s].x = (int)((pf & ~PF_CC_MASK) | ((uint32_t)n << PF_CC_SHIFT));
+        if (n != cc_old) a.M[s] = (int)((pf & ~PF_CC_MASK) | ((uint32_t)n << PF_CC_SHIFT));
 
         // 3. store_forces, 2D_Simulation.wgsl:434-478
         const float density = ma_rho;
@@ -984,7 +983,7 @@ __device__ __forceinline__ void bond_pass_body(const StepArgs& args, const Scene
     const int cur = args.cur;
     const float4 p = a.P[cur][i];
     const float4 d = del_of(a, i, S.u.dT);
-    const int mat = (int)((uint32_t)a.M[i].x & PF_MAT_MASK);
+    const int mat = (int)((uint32_t)a.M[i] & PF_MAT_MASK);
     const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
     Self me;
     me.x = p.x; me.y = p.y; me.r = p.z; me.drot = d.z; me.dx = d.x; me.dy = d.y;
@@ -1015,7 +1014,7 @@ __global__ void __launch_bounds__(256) k_bond_forces(StepArgs args, int nrec, in
     const int2 lfw = args.lf[sid];
     const uint32_t fl = (uint32_t)lfw.x;
     if (!(fl & SF_ACTIVE) || !(fl & SF_SIM)) return;
-    if ((uint32_t)a.M[i].x & PF_INACTIVE) return;   // an inactive owner evaluates nothing
+    if ((uint32_t)a.M[i] & PF_INACTIVE) return;   // an inactive owner evaluates nothing
     if (F64 && S.use_f64) bond_pass_body<true>(args, S, q, i, j, lfw);
     else bond_pass_body<false>(args, S, q, i, j, lfw);
 }
@@ -1079,7 +1078,7 @@ __global__ void k_bin(StepArgs args) {
     const SceneDev& S = args.scenes[sid];
     if (!(args.lf[sid].x & SF_ACTIVE)) return;
     if (s < S.p0 || s >= S.p0 + S.np) return;
-    const int2 m = a.M[s];
+    const int2 m = make_int2(a.M[s], a.ID[s]);
     int key;
     if (m.x & PF_INACTIVE) {
         key = S.c0 + S.w * S.h;
@@ -1221,7 +1220,7 @@ __device__ __forceinline__ void rb_bin_slot(const StepArgs& args, int s, int lbs
     const int sid = args.blk_scene[s / lbs];
     if (sid < 0) return;   // whole waves: lbs is a multiple of 64
     const SceneDev& S = args.scenes[sid];
-    const int2 m = a.M[s];
+    const int2 m = make_int2(a.M[s], a.ID[s]);
     int key = S.c0 + S.w * S.h;
     if (!(m.x & PF_INACTIVE)) {
         const float4 p = a.P[args.cur][s];
@@ -1251,7 +1250,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
     const SceneDev& S = args.scenes[sid];
     const int cap = (int)a.cap;
     const int cur = args.cur;
-    const int2 m = a.M[s];
+    const int2 m = make_int2(a.M[s], a.ID[s]);
     const float4 p = a.P[cur][s];
     a.XR[s] = make_float2(p.x, p.y);
     a.V[s].w = __uint_as_float(__float_as_uint(a.V[s].w) & VW_EXPLICIT);   // path since the build: 0
@@ -1275,7 +1274,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
     const int2 brs = a.BRS[s];
     auto bonded = [&](int j) {
         if (brs.y <= brs.x) return false;
-        const int oj = a.M[j].y;
+        const int oj = a.ID[j];
         for (int q = brs.x; q < brs.y; q++) {
             const int4 br = a.BR[q];
             if (br.w != 0 && br.x == oj) return true;
@@ -1382,9 +1381,9 @@ __global__ void k_cell_sort(StepArgs args, int cell_lo, int cell_hi, int p0) {
     if (e0 < p0 || e1 > plim || e1 < e0) return;
     for (int e = e0 + 1; e < e1; e++) {
         const int v = a.SIDX[e];
-        const int key = a.M[v].y;
+        const int key = a.ID[v];
         int f = e - 1;
-        while (f >= e0 && a.M[a.SIDX[f]].y > key) {
+        while (f >= e0 && a.ID[a.SIDX[f]] > key) {
             a.SIDX[f + 1] = a.SIDX[f];
             f--;
         }
@@ -1408,8 +1407,9 @@ __global__ void k_reorder_move(ReorderArgs r) {
     const float4 v = r.src.V[s];
     r.dst.V[j] = v;
     if (vword(v) & VW_EXPLICIT) r.dst.DX[j] = r.src.DX[s];
-    const int2 m = r.src.M[s];
-    r.dst.M[j] = m;
+    const int2 m = make_int2(r.src.M[s], r.src.ID[s]);
+    r.dst.M[j] = m.x;
+    r.dst.ID[j] = m.y;
     r.dst.A[j] = r.src.A[s];
     if (r.with_ext) r.dst.EXT[j] = r.src.EXT[s];
     if (r.with_data) {
@@ -1437,10 +1437,11 @@ __global__ void k_reorder_move(ReorderArgs r) {
 //    scenes) keep their positions, partitioned stably into particles with neighbour-
 //    list candidates first.  The few candidates of a unit then fill whole waves at its
 //    start and every other wave skips the narrow phase, while particles stay within
-//    the unit (8192 slots at blocks of 256) and so near their cell-order neighbours.
+//    the unit (16384 slots at blocks of 256) and so near their cell-order neighbours.
 //    The cell list is renamed to the destination slots in place.  One workgroup per
 //    unit, blockDim.x = the logical block size.
-constexpr int PART_BLOCKS = 32;
+constexpr int PART_BLOCKS = 64;
+static_assert(PART_BLOCKS <= 64, "one flag bit per block in a 64-bit mask");
 
 __device__ __forceinline__ bool part_flag(const ReorderArgs& r, const SceneDev& S, int j, int& s) {
     s = r.src.SIDX[j];
@@ -1454,60 +1455,57 @@ __global__ void __launch_bounds__(256) k_partition(ReorderArgs r, int nlb) {
     const int b1 = min(b0 + PART_BLOCKS, nlb);
     const int tid = (int)threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6, nw = lbs >> 6;
-    __shared__ int s_wc[4];                // flagged lanes per wave of the current block
-    __shared__ int s_cnt[PART_BLOCKS];     // flagged slots per block, then their exclusive prefix
+    const uint64_t below = (1ull << lane) - 1ull;
+    __shared__ uint64_t s_bal[PART_BLOCKS][4];   // flagged lanes of each wave of each block
+    __shared__ int s_pre[PART_BLOCKS];           // flagged slots of the segment before each block
     __shared__ int s_tot;
     int bs = b0;
-    while (bs < b1) {                      // segments: runs of blocks of one scene
+    while (bs < b1) {                            // segments: runs of blocks of one scene
         const int sid = r.blk_scene[bs];
         int be = bs + 1;
         while (be < b1 && r.blk_scene[be] == sid) be++;
         if (sid >= 0) {
             const SceneDev& S = r.scenes[sid];
+            // this thread's slot of every block of the segment: one flag bit per block,
+            // the loads of several blocks in flight together
+            uint64_t mask = 0;
+#pragma unroll 8
             for (int b = bs; b < be; b++) {
                 int s;
-                const bool f = part_flag(r, S, b * lbs + tid, s);
-                const uint64_t bf = __ballot(f);
-                if (lane == 0) s_wc[wv] = __popcll(bf);
-                __syncthreads();
-                if (tid == 0) {
-                    int t = 0;
-                    for (int q = 0; q < nw; q++) t += s_wc[q];
-                    s_cnt[b - bs] = t;
-                }
-                __syncthreads();
+                if (part_flag(r, S, b * lbs + tid, s)) mask |= 1ull << (b - bs);
             }
+            for (int b = bs; b < be; b++) {
+                const uint64_t bf = __ballot((mask >> (b - bs)) & 1ull);
+                if (lane == 0) s_bal[b - bs][wv] = bf;
+            }
+            __syncthreads();
             if (tid == 0) {
                 int run = 0;
                 for (int q = 0; q < be - bs; q++) {
-                    const int c = s_cnt[q];
-                    s_cnt[q] = run;
-                    run += c;
+                    s_pre[q] = run;
+                    for (int w = 0; w < nw; w++) run += __popcll(s_bal[q][w]);
                 }
                 s_tot = run;
             }
             __syncthreads();
             const int seg0 = bs * lbs;
             for (int b = bs; b < be; b++) {
+                const int k = b - bs;
                 const int j = b * lbs + tid;
-                int s;
-                const bool f = part_flag(r, S, j, s);
-                const uint64_t bf = __ballot(f);
-                if (lane == 0) s_wc[wv] = __popcll(bf);
-                __syncthreads();
-                int pre = 0;
-                for (int q = 0; q < wv; q++) pre += s_wc[q];
-                const int rf = pre + __popcll(bf & ((1ull << lane) - 1ull));   // flagged lanes before this one
-                const int cb = s_cnt[b - bs];                                  // flagged slots before block b
-                const int dst = seg0 + (f ? cb + rf : s_tot + ((b - bs) * lbs - cb) + (tid - rf));
+                int s = r.src.SIDX[j];
+                if (s < S.p0 || s >= S.p0 + S.np) s = j;
+                const bool f = (mask >> k) & 1ull;
+                int rf = __popcll(s_bal[k][wv] & below);                      // flagged lanes before this one
+                for (int w = 0; w < wv; w++) rf += __popcll(s_bal[k][w]);
+                const int cb = s_pre[k];                                       // flagged slots before block b
+                const int dst = seg0 + (f ? cb + rf : s_tot + (k * lbs - cb) + (tid - rf));
                 r.order[dst] = s;
                 r.newslot[s] = dst;
                 r.src.SIDX[j] = dst;
-                __syncthreads();   // s_wc is rewritten by the next block
             }
         }
         bs = be;
-        __syncthreads();       // s_cnt / s_tot are rewritten by the next segment
+        __syncthreads();   // s_bal / s_pre / s_tot are rewritten by the next segment
     }
 }
 
@@ -1516,7 +1514,7 @@ __global__ void k_kinetic(DevArrays a, int cur, int m0, int nm, int p0, int np, 
     const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (s >= np) return;
     const int slot = p0 + s;
-    const int2 m = a.M[slot];
+    const int2 m = make_int2(a.M[slot], a.ID[slot]);
     double e = 0.0;
     if (!(m.x & PF_INACTIVE)) {
         const int mat = m.x & PF_MAT_MASK;
@@ -1534,11 +1532,11 @@ __global__ void k_kinetic(DevArrays a, int cur, int m0, int nm, int p0, int np, 
 // order independent.  A grid of COUNT_BLOCKS strides over the scene, so only that
 // many same-address atomics meet in L2.
 constexpr int COUNT_BLOCKS = 256;
-__global__ void __launch_bounds__(256) k_count_contacts(const int2* m, int p0, int np, unsigned long long* out) {
+__global__ void __launch_bounds__(256) k_count_contacts(const int32_t* m, int p0, int np, unsigned long long* out) {
     __shared__ int s_w[4];
     int c = 0;
     for (int s = (int)(blockIdx.x * blockDim.x + threadIdx.x); s < np; s += (int)(gridDim.x * blockDim.x))
-        c += cc_of((uint32_t)m[p0 + s].x);
+        c += cc_of((uint32_t)m[p0 + s]);
     c = wave_sum(c);
     if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
     __syncthreads();
@@ -1558,7 +1556,7 @@ __global__ void k_bond_pairs(StepArgs args, int reach, int maxb, float factor, i
     if (sid < 0) return;
     const SceneDev& S = args.scenes[sid];
     if (s < S.p0 || s >= S.p0 + S.np) return;
-    const int2 m = a.M[s];
+    const int2 m = make_int2(a.M[s], a.ID[s]);
     const int i = m.y;
     if (m.x & PF_INACTIVE) {
         if (i < S.n_real) out_cnt[i] = 0;
@@ -1575,7 +1573,7 @@ __global__ void k_bond_pairs(StepArgs args, int reach, int maxb, float factor, i
         const int e1 = min(a.START[S.c0 + ry * S.w + min(cx + reach, S.w - 1) + 1] + pos0, S.p0 + S.np);
         for (int e = e0; e < e1; e++) {
             const int js = a.SIDX[e];
-            const int j = a.M[js].y;
+            const int j = a.ID[js];
             if (j <= i) continue;
             const float4 q = a.P[args.cur][js];
             const float dx = q.x - p.x, dy = q.y - p.y;
@@ -1691,7 +1689,7 @@ hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, i
     return hipGetLastError();
 }
 
-hipError_t launch_count_contacts(const int2* m, int p0, int np, unsigned long long* out, hipStream_t st) {
+hipError_t launch_count_contacts(const int32_t* m, int p0, int np, unsigned long long* out, hipStream_t st) {
     const int b = 256;
     const int nb = (np + b - 1) / b;
     hipLaunchKernelGGL(k_count_contacts, dim3(nb < COUNT_BLOCKS ? nb : COUNT_BLOCKS), dim3(b), 0, st, m, p0, np, out);
