@@ -184,8 +184,9 @@ struct StepArgs {
     const int2* lf;            // per scene {step flags, global step index of the Sim half}
     DevArrays a;
     int cur;                   // which P buffer holds the current positions
-    int flip;                  // some scene drifts in this launch, so cur flips after it: every slot
-                               // that does not drift carries its P over to the other buffer
+    int flip;                  // cur flips after this launch: every slot that does not drift
+                               // carries its P over to the other buffer (every launch flips, so the
+                               // position and list parities keep one relation)
     int lbs;                   // block size of the step kernel's launch (a kernel argument: the
                                // dispatch packet's copy costs a vector load per wave)
 };

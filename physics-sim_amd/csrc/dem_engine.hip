@@ -771,6 +771,8 @@ void Engine::plan_reorder(const int2* lf_dev, std::vector<LaunchOp>& ops) {
     r.dst.CF = r.src.CFn;
     r.dst.V = r.dst.Vn;
     r.dst.DX = r.dst.DXn;
+    // and the positions in the other position buffer (cur_ flips too)
+    std::swap(r.dst.P[0], r.dst.P[1]);
     r.order = ORDER_;
     r.newslot = NEWSLOT_;
     r.cur = cur_;
@@ -799,6 +801,7 @@ void Engine::plan_reorder(const int2* lf_dev, std::vector<LaunchOp>& ops) {
     }
     set_ ^= 1;
     lcur_ ^= 1;
+    cur_ ^= 1;
     for (SceneHost& sh : scenes_) {
         if (!sh.live) continue;
         sh.since_reorder = 0;
@@ -1008,6 +1011,14 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
         bool sim = false;
         for (size_t i = 0; i < sids.size(); i++)
             if (lf[(size_t)k * scene_cap_ + sids[i]].x & SF_SIM) sim = true;
+        if (!lom) {
+            // No drift, so no rebuild follows this launch, but its position parity flips:
+            // the flag that the last rebuild chain consumed (RB[cur]) is cleared here,
+            // before any record death of this launch sets both flags.
+            LaunchOp& o = new_op(ops, OP_RB_STORE, 2);
+            o.rb = RB_ + cur_;
+            o.value = 0;
+        }
         if (any_bonds && sim) {
             // bond records first: one thread per record, results read by the step kernel
             LaunchOp& bp = new_op(ops, OP_BONDS, 0);
@@ -1018,16 +1029,19 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
         }
         LaunchOp& st = new_op(ops, OP_STEP, 0);
         st.sa = make_args(lfk);
-        st.sa.flip = lom ? 1 : 0;
+        st.sa.flip = 1;
         st.nlb = nblk;
         st.block = block_;
         st.f64 = any_f64 ? 1 : 0;
         st.bonds = any_bonds ? 1 : 0;
         st.value = any_large ? 1 : 0;   // a scene with large bodies
-        lcur_ ^= 1;  // the step kernel wrote every live slot's next contact list
-        // some LOM half wrote the next positions; every other slot carried its own over
-        if (!lom) continue;
+        // The step kernel wrote every live slot's next contact list and velocity record,
+        // and its next position (drifted or carried over): both parities flip with every
+        // launch, so which position buffer pairs with which velocity buffer never changes
+        // (one pairing of the buffers streams measurably faster than the other).
+        lcur_ ^= 1;
         cur_ ^= 1;
+        if (!lom) continue;
         // Neighbour-list rebuild after every drift.  The kernels run only when the
         // rebuild flag is set: by the step kernel (a particle moved more than 0.45
         // skin since the last build) or here (new state, new parameters, re-sort due).
@@ -1068,7 +1082,14 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
         bl.nlb = nblk;
         bl.grid = rb_grid;
         bl.block = block_;
-        if (due) plan_reorder(lfk, ops);   // the lists move with their particles
+        if (due) {
+            // the lists move with their particles; the re-sort flips the position parity,
+            // so the flag this chain consumed would otherwise be read by the next chain
+            plan_reorder(lfk, ops);
+            LaunchOp& o = new_op(ops, OP_RB_STORE, 3);
+            o.rb = rb;
+            o.value = 0;
+        }
         rb_force_ = false;
     }
     return run_ops(ops);

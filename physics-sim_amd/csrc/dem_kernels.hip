@@ -415,8 +415,10 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
             br.w = 0;
             a.BR[q] = br;
             // the partner may come back into contact: it returns to the neighbour
-            // lists, which leave out live bond partners, at the rebuild after this drift
-            a.RB[cur ^ 1] = 1;
+            // lists, which leave out live bond partners, at the rebuild after this
+            // launch's drift, or after the next launch's if this one has none (both flags)
+            a.RB[0] = 1;
+            a.RB[1] = 1;
             return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         } else if (!found && bonded > 0 && u.bonds <= 1) {
             bf.x = 0.0f;
