@@ -1362,12 +1362,20 @@ __global__ void k_rb_scatter(StepArgs args, int nlb) {
         rb_scatter_slot(args, lb * lbs + (int)threadIdx.x, lbs);
 }
 
+// The lists are built in cell-list order (thread e takes the slot at list position e),
+// so that a wave's stencils overlap whatever the storage order: the re-sort moves
+// candidates to the front of their units, out of cell order.
 __global__ void k_build_nl(StepArgs args, int nlb) {
     if (args.a.RB[args.cur] == 0) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&args.a.STAT[4], 1ull);   // rebuilds
     const int lbs = (int)blockDim.x;
-    for (int lb = (int)blockIdx.x; lb < nlb; lb += (int)gridDim.x)
-        build_nl_slot(args, lb * lbs + (int)threadIdx.x, lbs);
+    for (int lb = (int)blockIdx.x; lb < nlb; lb += (int)gridDim.x) {
+        const int sid = args.blk_scene[lb];
+        if (sid < 0) continue;
+        const SceneDev& S = args.scenes[sid];
+        const int s = args.a.SIDX[lb * lbs + (int)threadIdx.x];   // the scene's list is a permutation of its slots
+        if (s >= S.p0 && s < S.p0 + S.np) build_nl_slot(args, s, lbs);
+    }
 }
 
 // ---- (cell, id) order of the cell list (the debug view of stages 1-3) ----
