@@ -1499,6 +1499,7 @@ __global__ void __launch_bounds__(256) k_partition(ReorderArgs r, int nlb) {
             }
             __syncthreads();
             const int seg0 = bs * lbs;
+#pragma unroll 4
             for (int b = bs; b < be; b++) {
                 const int k = b - bs;
                 const int j = b * lbs + tid;
