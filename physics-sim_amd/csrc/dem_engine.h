@@ -154,7 +154,7 @@ private:
     int write_scene_table();
     void plan_reorder(const int2* lf_dev, std::vector<LaunchOp>& ops);
     static LaunchOp& new_op(std::vector<LaunchOp>& ops, int kind, int timed);
-    hipError_t run_op(const LaunchOp& op);
+    hipError_t run_op(const LaunchOp& op, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
     int run_ops(const std::vector<LaunchOp>& ops);
     void clear_graphs();
     int rebuild_chain(const int2* lf_dev, bool resort);

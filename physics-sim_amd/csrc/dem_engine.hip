@@ -10,7 +10,7 @@
 namespace psim {
 
 hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, bool large,
-                       hipStream_t st);
+                       hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 hipError_t launch_bin(const StepArgs& args, int nblocks, int block, hipStream_t st);
 int scan_tiles(int n);
 hipError_t launch_scan(const DevArrays& a, int ncells, const int32_t* gate, hipStream_t st);
@@ -818,9 +818,10 @@ LaunchOp& Engine::new_op(std::vector<LaunchOp>& ops, int kind, int timed) {
     return op;
 }
 
-hipError_t Engine::run_op(const LaunchOp& op) {
+hipError_t Engine::run_op(const LaunchOp& op, hipEvent_t ev0, hipEvent_t ev1) {
     switch (op.kind) {
-        case OP_STEP: return launch_step(op.sa, op.nlb, op.block, op.f64 != 0, op.bonds != 0, op.value != 0, stream_);
+        case OP_STEP:
+            return launch_step(op.sa, op.nlb, op.block, op.f64 != 0, op.bonds != 0, op.value != 0, stream_, ev0, ev1);
         case OP_RB_STORE: return launch_rb_store(op.rb, op.value, stream_);
         case OP_RB_BIN: return launch_rb_bin(op.sa, op.nlb, op.grid, op.block, stream_);
         case OP_SCAN: return launch_scan(op.da, op.ncells, op.gate, stream_);
@@ -866,9 +867,12 @@ int Engine::run_ops(const std::vector<LaunchOp>& ops) {
         if (timing_) { t0 = ev(); t1 = ev(); hipEventRecord(t0, stream_); }
         for (const LaunchOp& op : ops) {
             hipEvent_t a = nullptr, b = nullptr;
-            if (timing_) { a = ev(); b = ev(); hipEventRecord(a, stream_); }
-            const hipError_t e = run_op(op);
-            if (timing_) { hipEventRecord(b, stream_); evs.push_back({op.timed, op.kind, a, b}); }
+            // the step kernel's events carry its own start and end; other launches are
+            // bracketed by events recorded on the stream
+            const bool own = op.kind == OP_STEP;
+            if (timing_) { a = ev(); b = ev(); if (!own) hipEventRecord(a, stream_); }
+            const hipError_t e = own && timing_ ? run_op(op, a, b) : run_op(op);
+            if (timing_) { if (!own) hipEventRecord(b, stream_); evs.push_back({op.timed, op.kind, a, b}); }
             if (int rc = check(e, kOpNames[op.kind])) return rc;
         }
         if (timing_) {

@@ -32,6 +32,7 @@
 // of the reference bit for bit when both sum contacts in the same order (ascending
 // partner id).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 
 #include <algorithm>
@@ -1605,18 +1606,27 @@ __global__ void k_bond_pairs(StepArgs args, int reach, int maxb, float factor, i
 
 // ---- launchers (host context) ----
 
+template <class K>
+static void launch_step_kernel(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
+                               const StepArgs& a) {
+    // with events (timing only, never under graph capture): hipExtLaunchKernelGGL
+    // stamps them with the kernel's own start and end
+    if (ev0) hipExtLaunchKernelGGL(kernel, g, b, 0, st, ev0, ev1, 0, a);
+    else hipLaunchKernelGGL(kernel, g, b, 0, st, a);
+}
+
 hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, bool large,
-                       hipStream_t st) {
+                       hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     dim3 g(nblocks), b(block);
     StepArgs a = args;
     a.lbs = block;
     if (f64) {
-        if (bonds) hipLaunchKernelGGL((k_step_bonds<true>), g, b, 0, st, a);
-        else hipLaunchKernelGGL(k_step_plain_f64, g, b, 0, st, a);
+        if (bonds) launch_step_kernel(k_step_bonds<true>, g, b, st, ev0, ev1, a);
+        else launch_step_kernel(k_step_plain_f64, g, b, st, ev0, ev1, a);
     } else {
-        if (bonds) hipLaunchKernelGGL((k_step_bonds<false>), g, b, 0, st, a);
-        else if (large) hipLaunchKernelGGL(k_step_plain_large, g, b, 0, st, a);
-        else hipLaunchKernelGGL(k_step_plain, g, b, 0, st, a);
+        if (bonds) launch_step_kernel(k_step_bonds<false>, g, b, st, ev0, ev1, a);
+        else if (large) launch_step_kernel(k_step_plain_large, g, b, st, ev0, ev1, a);
+        else launch_step_kernel(k_step_plain, g, b, st, ev0, ev1, a);
     }
     return hipGetLastError();
 }
