@@ -58,6 +58,32 @@ def test_struck_chains_match_oracle(mode):
     prog.close()
 
 
+@pytest.mark.parametrize("mode", [3, 1])
+def test_struck_chains_without_data_match_oracle(mode):
+    """Without data recording the bond pass sums each contact-free owner's records in
+    record order itself (one sum per owner for the step kernel); owners in contact keep
+    per-record results.  Both paths run here (the striker's neighbours are in contact,
+    the rest of the chains are not): state, break codes and contact table equal the
+    oracle."""
+    s, st = chain_scene(mode)
+    n, nb = st.p_count, st.n_bonds
+    prog = ComputeProg(s, st.copy(), record_data=False, reorder_interval=8)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    in_contact = 0
+    for k in (120, 240, 240):
+        prog.step(k)
+        eng.step(k)
+        in_contact = max(in_contact, prog.counters()["contacts"])
+    got = prog.update_state()
+    for name in FIELDS:
+        assert_bits(getattr(got, name)[: _n(name, n)], getattr(eng.st, name)[: _n(name, n)], name)
+    assert_bits(got.bonds[: 3 * nb], eng.st.bonds[: 3 * nb], "bond words (break codes)")
+    want = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n]
+    assert_bits(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want, "contact table")
+    assert in_contact > 0          # some owners took the per-record path
+    prog.close()
+
+
 def test_bond_builder_matches_oracle():
     """dem_build_bond_pairs (pair search on the cell list) + host assembly equal the
     oracle's State::regen_bonds: the same bonds in the same order, the same rest
