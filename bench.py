@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--particles", type=int, default=4194304)
     ap.add_argument("--per-row", type=int, default=2048)
     ap.add_argument("--cell", type=float, default=0.0, help="grid cell side (0: just above 2 r_max)")
-    ap.add_argument("--reorder", type=int, default=128,
+    ap.add_argument("--reorder", type=int, default=256,
                     help="storage re-sort every k steps (with a neighbour-list rebuild)")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--collision-interval", type=int, default=1,
