@@ -89,6 +89,15 @@ __device__ __forceinline__ float4 del_from(const DevArrays& a, int j, const floa
 }
 __device__ __forceinline__ float4 del_of(const DevArrays& a, int j, float dt) { return del_from(a, j, a.V[j], dt); }
 
+// Normal and shear stiffness of the partner in slot j: a one-material scene takes them
+// from the scene table (the record of material 0), without a load.
+__device__ __forceinline__ float2 partner_k(const DevArrays& a, const SceneDev& S, int j) {
+    if (S.nm <= 1) return make_float2(S.kn1, S.ks1);
+    const int mb = a.M[j] & PF_MAT_MASK;
+    const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
+    return make_float2(MB.normal_stiffness, MB.shear_stiffness);
+}
+
 // Carries the velocity record (and an explicit displacement) of slot s over to the
 // buffers of the next launch.
 __device__ __forceinline__ void carry_velocity(const DevArrays& a, int s) {
@@ -432,12 +441,10 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
         const int2 bs = bid_ok ? a.BS[S.b0 + bid] : make_int2(0, 0);
         if (bs.x < 0 && bs.y < step) bonded = bs.x;
     }
-    const int mb = a.M[j] & PF_MAT_MASK;
-    const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
+    const float2 kb = partner_k(a, S, j);
     float3 f;
     if (bonded < 0 || u.bonds == 0) {
-        f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness, MB.shear_stiffness,
-                              me.r + pb.z, bf.x, dd);
+        f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb.x, kb.y, me.r + pb.z, bf.x, dd);
     } else if (u.bonds >= 1 && u.bonds <= 3) {
         BondCtx bc;
         bc.L = bid_ok ? a.BL[S.b0 + bid] : 0.0f;
@@ -450,7 +457,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
             bc.ss = u.bond_shear_strength;
         }
         int code = 0;
-        f = bond_law<F64>(u, me, pb, db, MB.normal_stiffness, MB.shear_stiffness, bc, bf, dd, code);
+        f = bond_law<F64>(u, me, pb, db, kb.x, kb.y, bc, bf, dd, code);
         if (code < 0) {
             bonded = code;
             if (bid_ok) a.BS[S.b0 + bid] = make_int2(code, step);
@@ -559,13 +566,12 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const int w0 = (narrow && act) ? nlw : 0;
     const int nc = nl_count(w0);
     // Register path (plain kernels): at most four candidates and four old contacts.
-    // The hits stay in registers as {partner id, slot, material word}, sorted by id,
+    // The hits stay in registers as {partner id, slot}, sorted by id,
     // and the force loop reads each partner by slot: no id -> slot lookup and no
     // re-read of the new list.  Same partners, same order, same arithmetic.
     const bool fast = rec_regs && narrow && act && nc > 0 && nc <= 4 && cc_old <= 4;
     int fo[4] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
     int fj[4] = {0, 0, 0, 0};
-    int fm[4] = {0, 0, 0, 0};
     if (narrow && act && nc > 0) {
         // neighbour list: every particle within r_a + r_b + skin at the last build; none
         // outside it can touch s while no particle has moved more than 0.45 skin since.
@@ -585,19 +591,17 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (jj[k] >= 0 && jj[k] != s && overlaps(qq[k], p)) {
-                    const int2 mj = make_int2(a.M[jj[k]], a.ID[jj[k]]);
-                    fo[k] = mj.y;
+                    fo[k] = a.ID[jj[k]];
                     fj[k] = jj[k];
-                    fm[k] = mj.x;
                     n++;
                 }
             }
             // ascending partner id (misses carry INT32_MAX and sort last)
             auto cswap = [&](int x, int y) {
                 if (fo[y] < fo[x]) {
-                    const int to = fo[x], tj = fj[x], tm = fm[x];
-                    fo[x] = fo[y]; fj[x] = fj[y]; fm[x] = fm[y];
-                    fo[y] = to; fj[y] = tj; fm[y] = tm;
+                    const int to = fo[x], tj = fj[x];
+                    fo[x] = fo[y]; fj[x] = fj[y];
+                    fo[y] = to; fj[y] = tj;
                 }
             };
             cswap(0, 1);
@@ -748,7 +752,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             }
             for (int k = 0; k < n; k++) {
                 // the hit of rank k sits in element 0 (the arrays shift after each)
-                const int ob = fo[0], j = fj[0], mw = fm[0];
+                const int ob = fo[0], j = fj[0];
                 if (BONDS) {
                     // records with a smaller partner id come first (at most four: registers)
                     load_records();
@@ -760,10 +764,8 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                     if (m < oc && co[m] == ob) ft = cf[m];
                 const float4 pb = a.P[cur][j];
                 const float4 db = del_of(a, j, dt);
-                const int mb = mw & PF_MAT_MASK;
-                const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
-                const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness,
-                                                   MB.shear_stiffness, me.r + pb.z, ft, dd);
+                const float2 kb = partner_k(a, S, j);
+                const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb.x, kb.y, me.r + pb.z, ft, dd);
                 nfx += f.x;
                 nfy += f.y;
                 nm += f.z;
@@ -773,7 +775,6 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 for (int m = 0; m < 3; m++) {
                     fo[m] = fo[m + 1];
                     fj[m] = fj[m + 1];
-                    fm[m] = fm[m + 1];
                 }
             }
         }
@@ -793,10 +794,8 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             const int j = a.INV[S.p0 + ob];
             const float4 pb = a.P[cur][j];
             const float4 db = del_of(a, j, dt);
-            const int mb = a.M[j] & PF_MAT_MASK;
-            const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
-            const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, MB.normal_stiffness,
-                                               MB.shear_stiffness, me.r + pb.z, ft, dd);
+            const float2 kb = partner_k(a, S, j);
+            const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb.x, kb.y, me.r + pb.z, ft, dd);
             nfx += f.x;
             nfy += f.y;
             nm += f.z;
@@ -1050,13 +1049,13 @@ __device__ __forceinline__ bool block_scene_f64(const StepArgs& args) {
     return sid >= 0 && args.scenes[sid].use_f64 != 0;
 }
 
-// Plain f32 scenes: at least 7 waves per SIMD (at most 72 VGPRs) to hide the latency
-// of the candidate loop (at 64 VGPRs the contact loop spills, and a kernel with
-// scratch costs more per launch); launches with a large body take
+// Plain f32 scenes: 8 waves per SIMD (at most 64 VGPRs, no spill) to hide the latency
+// of the candidate loop (a kernel with scratch costs more per launch, so the register
+// budget is kept without spills); launches with a large body take
 // k_step_plain_large, whose stencil fallback also covers them.  Launches holding an
 // f64 scene (k_step_plain_f64) and bonded scenes carry more live state and keep the
 // default register budget; both handle large bodies.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) k_step_plain(StepArgs args) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_plain(StepArgs args) {
     step_slot<false, false, false>(args);
 }
 __global__ void __launch_bounds__(256) k_step_plain_large(StepArgs args) {
