@@ -180,10 +180,10 @@ private:
     int64_t hw_slots_ = 0, hw_cells_ = 0;
     int32_t mat_cap_ = 0, scene_cap_ = 0;
     int set_ = 0;   // which DevSet is current
-    int cur_ = 0;   // which P/D buffer is current
+    int cur_ = 0;   // which position buffer is current (flips with lcur_ after every launch)
     DevSet sets_[2];
     ContactLists lists_[2];
-    int lcur_ = 0;  // which contact-list buffer is current
+    int lcur_ = 0;  // which contact-list (and velocity) buffer is current
     int2* BOFF_ = nullptr;      // per slot (scene p0 + particle id): bond record range [x, y)
     int4* BR_ = nullptr;
     float4* BF_ = nullptr;

@@ -694,7 +694,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     // no scene's ranges touch a neighbour region)
     for (int32_t i = 0; i < np; i++)
         BRANGE[i] = make_int2(BOFF[i] + (int32_t)sh.r0, BOFF[i + 1] + (int32_t)sh.r0);
-    // device copies: the current set, the current P/D buffer
+    // device copies: the current set, the current position and list buffers
     hipError_t e = hipSuccess;
     const DevSet& S = sets_[set_];
     const int64_t p0 = sh.p0;
@@ -756,9 +756,9 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
 // ---------------------------------------------------------------------------
 void Engine::plan_reorder(const int2* lf_dev, std::vector<LaunchOp>& ops) {
     // Every live scene, binned by the rebuild this re-sort belongs to and with its
-    // neighbour lists just built, moves into cell order, each block of slots then
-    // partitioned stably into particles with list candidates first (the order inside
-    // a cell is the binning's; results depend on neither).
+    // neighbour lists just built, moves into cell order, each unit of 64 blocks of slots
+    // then partitioned stably into particles with list candidates first (the order
+    // inside a cell is the binning's; results depend on neither).
     ReorderArgs r{};
     r.scenes = d_scenes_;
     r.blk_scene = d_blk_scene_;
