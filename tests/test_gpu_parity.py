@@ -54,6 +54,33 @@ def test_granular_box_bit_exact(steps):
     prog.close()
 
 
+@pytest.mark.parametrize("zero", [(17, 300), (0, 17, 300)])
+def test_zero_radius_discs_and_wall_bounces_bit_exact(zero):
+    """Discs of radius 0 are inactive (2D_LOM.wgsl:83, 2D_Simulation.wgsl:118) and keep
+    their state; with disc 0 among them the reference's invocation 0 never updates
+    coll_cont, so no narrow phase runs at all.  Fast random velocities drive the outer
+    discs into the walls: a bounce halves and reverses the velocity after the
+    displacement was formed, so those drifts keep their displacement as an explicit
+    record.  Three dem_step calls: state identical to the oracle."""
+    s, st = scenes.granular_box(n_side=24, seed=11, spacing=1.9)
+    rng = np.random.default_rng(11)
+    n = st.p_count
+    st.vel[: 2 * n] = rng.uniform(-200.0, 200.0, 2 * n).astype(np.float32)
+    st.rot_vel[:n] = rng.uniform(-5.0, 5.0, n).astype(np.float32)
+    for k in zero:
+        st.radii[k] = 0.0
+    prog = ComputeProg(s, st.copy(), reorder_interval=4)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    for k in (3, 10, 25):
+        prog.step(k)
+        eng.step(k)
+    got = prog.update_state()
+    assert_bits_equal(got, eng.st, n)
+    for k in zero:   # inactive discs never move
+        assert got.pos[2 * k] == st.pos[2 * k] and got.pos[2 * k + 1] == st.pos[2 * k + 1]
+    prog.close()
+
+
 def test_granular_contacts_dense_bit_exact():
     """A compressed box (spacing 1.8 < 2 r_max) so that most discs touch several
     neighbours; 30 steps bit-exact including the contact table."""
