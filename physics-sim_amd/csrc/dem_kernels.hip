@@ -541,8 +541,6 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     float4 v = a.V[s];
     const int cc_old = cc_of(pf);
     const int mat = (int)(pf & PF_MAT_MASK);
-    // a scene of one material (the common case) reads its record without waiting for
-    // the particle's material word
     // a scene of one material (the common case) takes the record from the scene table,
     // without waiting for the particle's material word
     float ma_rho = S.rho1, ma_kn = S.kn1, ma_ks = S.ks1;
