@@ -189,6 +189,8 @@ struct StepArgs {
                                // position and list parities keep one relation)
     int lbs;                   // block size of the step kernel's launch (a kernel argument: the
                                // dispatch packet's copy costs a vector load per wave)
+    int one_scene;             // the only live scene, or -1: the step kernel then skips the
+                               // block -> scene table load
 };
 
 struct ReorderArgs {

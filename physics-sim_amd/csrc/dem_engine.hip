@@ -502,6 +502,14 @@ StepArgs Engine::make_args(const int2* lf) const {
     s.lf = lf;
     s.a = arrays(set_);
     s.cur = cur_;
+    s.one_scene = -1;
+    int live = 0;
+    for (size_t i = 0; i < scenes_.size(); i++)
+        if (scenes_[i].live) {
+            live++;
+            s.one_scene = (int)i;
+        }
+    if (live != 1) s.one_scene = -1;
     return s;
 }
 

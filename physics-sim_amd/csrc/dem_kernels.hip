@@ -478,7 +478,8 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const DevArrays& a = args.a;
     const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int s = blk * args.lbs + (int)threadIdx.x;
-    const int sid = args.blk_scene[blk];
+    // one live scene: its slot range below decides, with no table load on the way
+    const int sid = args.one_scene >= 0 ? args.one_scene : args.blk_scene[blk];
     if (sid < 0) return;
     const SceneDev& S = args.scenes[sid];
     if (s < S.p0 || s >= S.p0 + S.np) return;   // whole blocks (np is a multiple of the block)
