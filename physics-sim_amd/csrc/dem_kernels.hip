@@ -11,7 +11,11 @@
 //              rebuild's cell list when a list overflows), force laws in ascending
 //              partner-id order, store_forces, then the next drift, which raises the
 //              rebuild flag when the particle has moved more than 0.45 skin since the
-//              rebuild (stages 2a, 4, 5).  The first launch of a dem_step call runs
+//              rebuild (a path bound in the velocity word gates the exact test)
+//              (stages 2a, 4, 5).  Per disc it streams P (x, y, r, rotation) and V
+//              (velocity, path word) in and out and reads the flag word and the list
+//              word; the drift's displacement is re-formed as velocity x dT unless an
+//              explicit record DX holds it.  The first launch of a dem_step call runs
 //              only the LOM half and the last one only the Sim half, so the state
 //              between calls is the reference's post-Sim state.
 //   neighbour-list rebuild, after every drift, each kernel a no-op unless the flag
@@ -20,7 +24,8 @@
 //              (exclusive prefix of the cell counts over tiles of 4096 cells, wave64
 //              prefix in registers + LDS across waves; zeroes the counts), k_rb_scatter
 //              (slot -> position START[key] + RANK in the cell-ordered slot list) and
-//              k_build_nl (the lists) - stages 1, 2b, 3 of the reference's broad phase.
+//              k_build_nl (the lists, in cell-list order) - stages 1, 2b, 3 of the
+//              reference's broad phase.
 //   k_partition, k_reorder_move   every few rebuilds: move every per-particle
 //              array, the neighbour lists included, into cell-list order, each unit of
 //              64 blocks partitioned into particles with list candidates first (k_cell_sort
