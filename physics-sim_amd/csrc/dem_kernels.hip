@@ -1006,8 +1006,9 @@ __device__ __forceinline__ void bond_pass_body(const StepArgs& args, const Scene
     }
 }
 
+// 8 waves per SIMD: both variants fit 64 VGPRs without spills.
 template <bool F64>
-__global__ void __launch_bounds__(256) k_bond_forces(StepArgs args, int nrec, int lbs) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_bond_forces(StepArgs args, int nrec, int lbs) {
     const DevArrays& a = args.a;
     const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= nrec) return;
@@ -1069,9 +1070,13 @@ __global__ void __launch_bounds__(256) k_step_plain_f64(StepArgs args) {
     if (block_scene_f64(args)) step_slot<true, false, true>(args);
     else step_slot<false, false, true>(args);
 }
-template <bool F64>
-__global__ void __launch_bounds__(256) k_step_bonds(StepArgs args) {
-    if (F64 && block_scene_f64(args)) step_slot<true, true, true>(args);
+// Bonded launches: the f32 variant fits 80 VGPRs without spills (6 waves per SIMD
+// instead of 5); the f64 one keeps the default budget (it would spill).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) k_step_bonds_f32(StepArgs args) {
+    step_slot<false, true, true>(args);
+}
+__global__ void __launch_bounds__(256) k_step_bonds_f64(StepArgs args) {
+    if (block_scene_f64(args)) step_slot<true, true, true>(args);
     else step_slot<false, true, true>(args);
 }
 
@@ -1624,10 +1629,10 @@ hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, b
     StepArgs a = args;
     a.lbs = block;
     if (f64) {
-        if (bonds) launch_step_kernel(k_step_bonds<true>, g, b, st, ev0, ev1, a);
+        if (bonds) launch_step_kernel(k_step_bonds_f64, g, b, st, ev0, ev1, a);
         else launch_step_kernel(k_step_plain_f64, g, b, st, ev0, ev1, a);
     } else {
-        if (bonds) launch_step_kernel(k_step_bonds<false>, g, b, st, ev0, ev1, a);
+        if (bonds) launch_step_kernel(k_step_bonds_f32, g, b, st, ev0, ev1, a);
         else if (large) launch_step_kernel(k_step_plain_large, g, b, st, ev0, ev1, a);
         else launch_step_kernel(k_step_plain, g, b, st, ev0, ev1, a);
     }
