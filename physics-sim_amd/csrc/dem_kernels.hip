@@ -1063,7 +1063,8 @@ __device__ __forceinline__ bool block_scene_f64(const StepArgs& args) {
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_plain(StepArgs args) {
     step_slot<false, false, false>(args);
 }
-__global__ void __launch_bounds__(256) k_step_plain_large(StepArgs args) {
+// 7 waves per SIMD: 71 VGPRs without spills.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) k_step_plain_large(StepArgs args) {
     step_slot<false, false, true>(args);
 }
 __global__ void __launch_bounds__(256) k_step_plain_f64(StepArgs args) {
