@@ -29,6 +29,7 @@ __host__ __device__ inline int nl_count(int w) {
 }
 __host__ __device__ inline int nl_first(int w) { return w & ((1 << NL_SHIFT) - 1); }
 constexpr int DATA_WORDS = 7;
+constexpr int REC_DEAD = INT32_MIN;   // bonded word of a dead bond record in BRB
 
 // Word V.w of the velocity record (bits, not a float):
 //  VW_EXPLICIT  the displacement of the last drift, {del_x, del_y, del_rot}, is the
@@ -145,10 +146,10 @@ struct DevArrays {
     // by partner id; records never move
     Pool<int2> BOFF;        // [cap] record range [x, y) of (scene p0 + particle id)
     Pool<int4> BR;          // {partner id, bond id, bonded, alive}
+    Pool<int2> BRB;         // the bond pass's copy: {bond id, bonded, or REC_DEAD once dead}
     Pool<float4> BF;        // {ft, fbt, theta_b, 0}
     Pool<int2> BS;          // per bond {index word, step of break}
-    Pool<float> BL;         // per bond rest length
-    Pool<float2> BSTR;      // per bond {normal, shear} strength
+    Pool<float4> BLS;       // per bond {rest length, normal strength, shear strength, 0}
     Pool<const dem_material> MAT;
     // bond pass: owner of each record (global particle id, -1: none) and its results
     Pool<int32_t> LB;       // large-body particle ids, LB_MAX per scene

@@ -188,8 +188,8 @@ private:
     int4* BR_ = nullptr;
     float4* BF_ = nullptr;
     int2* BS_ = nullptr;
-    float* BL_ = nullptr;
-    float2* BSTR_ = nullptr;
+    float4* BLS_ = nullptr;
+    int2* BRB_ = nullptr;
     dem_material* MAT_ = nullptr;
     int32_t *KEY_ = nullptr, *RANK_ = nullptr, *CNT_ = nullptr, *START_ = nullptr, *SIDX_ = nullptr, *INV_ = nullptr;
     int32_t* RB_ = nullptr;

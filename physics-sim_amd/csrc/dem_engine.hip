@@ -100,7 +100,7 @@ Engine::~Engine() {
     }
     for (auto& s : sets_) { hipFree(s.NL); hipFree(s.XR); hipFree(s.BRS); }
     for (auto& l : lists_) { hipFree(l.CP); hipFree(l.CF); }
-    hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BL_); hipFree(BSTR_); hipFree(MAT_);
+    hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BLS_); hipFree(BRB_); hipFree(MAT_);
     hipFree(KEY_); hipFree(RANK_); hipFree(CNT_); hipFree(START_); hipFree(SIDX_); hipFree(INV_);
     hipFree(RB_); hipFree(ORDER_); hipFree(NEWSLOT_); hipFree(RO_); hipFree(RF_); hipFree(RD_);
     hipFree(RS_); hipFree(RP_); hipFree(LB_);
@@ -168,7 +168,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = e ? e : dalloc(&RO_, rec_cap_); e = e ? e : dalloc(&RF_, rec_cap_);
     e = e ? e : dalloc(&RS_, rec_cap_); e = e ? e : dalloc(&RP_, rec_cap_);
     e = e ? e : dalloc(&RD_, with_data ? 2 * rec_cap_ : 2);
-    e = e ? e : dalloc(&BS_, bond_cap_); e = e ? e : dalloc(&BL_, bond_cap_); e = e ? e : dalloc(&BSTR_, bond_cap_);
+    e = e ? e : dalloc(&BS_, bond_cap_); e = e ? e : dalloc(&BLS_, bond_cap_); e = e ? e : dalloc(&BRB_, rec_cap_);
     e = e ? e : dalloc(&MAT_, mat_cap_);
     e = e ? e : dalloc(&KEY_, S); e = e ? e : dalloc(&RANK_, S); e = e ? e : dalloc(&CNT_, cell_cap_ + 1);
     e = e ? e : dalloc(&START_, cell_cap_ + 2); e = e ? e : dalloc(&SIDX_, S); e = e ? e : dalloc(&INV_, S);
@@ -487,7 +487,7 @@ DevArrays Engine::arrays(int set) const {
     a.CP = lc.CP; a.CF = lc.CF;
     a.CPn = ln.CP; a.CFn = ln.CF;
     a.cap = slot_cap_;
-    a.BOFF = BOFF_; a.BR = BR_; a.BF = BF_; a.BS = BS_; a.BL = BL_; a.BSTR = BSTR_; a.MAT = MAT_;
+    a.BOFF = BOFF_; a.BR = BR_; a.BF = BF_; a.BS = BS_; a.BLS = BLS_; a.BRB = BRB_; a.MAT = MAT_;
     a.KEY = KEY_; a.RANK = RANK_; a.CNT = CNT_; a.START = START_; a.SIDX = SIDX_; a.INV = INV_;
     a.NL = s.NL; a.XR = s.XR; a.RB = RB_;
     a.RO = RO_; a.RF = RF_; a.RD = RD_; a.RS = RS_; a.RP = RP_; a.BRS = s.BRS; a.LB = LB_;
@@ -638,16 +638,16 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     sh.bond_angle.assign(sh.nb, 0);
     sh.bond_len.assign(sh.nb, 0);
     std::vector<int2> BS(std::max(sh.nb, 1));
-    std::vector<float> BL(std::max(sh.nb, 1));
-    std::vector<float2> BSTR(std::max(sh.nb, 1));
+    std::vector<float4> BLS(std::max(sh.nb, 1));
     for (int32_t k = 0; k < sh.nb; k++) {
         const int32_t idx = v.bonds[3 * k];
         sh.bond_angle[k] = v.bonds[3 * k + 1];
         sh.bond_len[k] = v.bonds[3 * k + 2];
         BS[k] = make_int2(idx, INT32_MIN);
-        BL[k] = bitsf(v.bonds[3 * k + 2]);
-        BSTR[k] = v.bond_strengths ? make_float2(v.bond_strengths[2 * k], v.bond_strengths[2 * k + 1])
-                                   : make_float2(sh.params.u.bond_normal_strength, sh.params.u.bond_shear_strength);
+        BLS[k] = v.bond_strengths ? make_float4(bitsf(v.bonds[3 * k + 2]), v.bond_strengths[2 * k],
+                                                v.bond_strengths[2 * k + 1], 0.0f)
+                                  : make_float4(bitsf(v.bonds[3 * k + 2]), sh.params.u.bond_normal_strength,
+                                                sh.params.u.bond_shear_strength, 0.0f);
     }
     sh.has_strength = v.bond_strengths != nullptr;
     // contact table -> contact lists (plain contacts) and bond records (everything else)
@@ -655,6 +655,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     std::vector<int32_t> RO;   // owner (global particle id) of each record
     std::vector<int4> BR;
     std::vector<float4> BF;
+    std::vector<int2> BRB;
     if (v.contacts) {
         const Slot* tab = reinterpret_cast<const Slot*>(v.contacts);
         const int K = v.contact_slots;
@@ -736,6 +737,10 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     if (!sh.large.empty()) h2d(LB_ + (int64_t)sid * LB_MAX, sh.large.data(), sizeof(int32_t) * sh.large.size());
     if (sh.nrec) {
         h2d(BR_ + sh.r0, BR.data(), sizeof(int4) * sh.nrec);
+        BRB.resize(sh.nrec);
+        for (int32_t q = 0; q < sh.nrec; q++)   // the bond pass's copy of the record words
+            BRB[q] = make_int2(BR[q].y, BR[q].w != 0 ? BR[q].z : REC_DEAD);
+        h2d(BRB_ + sh.r0, BRB.data(), sizeof(int2) * sh.nrec);
         h2d(BF_ + sh.r0, BF.data(), sizeof(float4) * sh.nrec);
         h2d(RO_ + sh.r0, RO.data(), sizeof(int32_t) * sh.nrec);
     }
@@ -743,8 +748,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         e = hipMemsetAsync(RO_ + sh.r0 + sh.nrec, 0xff, sizeof(int32_t) * (sh.rec_cap - sh.nrec), stream_);
     if (sh.nb) {
         h2d(BS_ + sh.b0, BS.data(), sizeof(int2) * sh.nb);
-        h2d(BL_ + sh.b0, BL.data(), sizeof(float) * sh.nb);
-        h2d(BSTR_ + sh.b0, BSTR.data(), sizeof(float2) * sh.nb);
+        h2d(BLS_ + sh.b0, BLS.data(), sizeof(float4) * sh.nb);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     int rc = check(e, "upload");

@@ -418,17 +418,19 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
                                                    const float4& p, bool narrow, int step, int cur, int q, int j,
                                                    DD& dd) {
     const dem_uniform& u = S.u;
-    int4 br = a.BR[q];
-    if (br.w == 0) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // {bond id, bonded word}: the 8-byte copy of the record word (BR keeps the partner
+    // id for the step kernel and the lists; both are written together)
+    const int2 rb = a.BRB[q];
+    if (rb.y == REC_DEAD) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float4 bf = a.BF[q];
     const float4 pb = a.P[cur][j];
     const float4 db = del_of(a, j, u.dT);
-    int bonded = br.z;
+    int bonded = rb.y;
     if (narrow) {
         const bool found = lenf(pb.x - p.x, pb.y - p.y) < (pb.z + p.z);
         if (!found && (bonded < 0 || u.bonds == 0)) {
-            br.w = 0;
-            a.BR[q] = br;
+            a.BR[q].w = 0;
+            a.BRB[q].y = REC_DEAD;
             // the partner may come back into contact: it returns to the neighbour
             // lists, which leave out live bond partners, at the rebuild after this
             // launch's drift, or after the next launch's if this one has none (both flags)
@@ -439,7 +441,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
             bf.x = 0.0f;
         }
     }
-    const int bid = br.y;
+    const int bid = rb.x;
     const bool bid_ok = bid >= 0 && bid < S.nb;
     if (bonded >= 0) {
         // a break recorded by the partner in an earlier step is seen here
@@ -452,11 +454,11 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
         f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb.x, kb.y, me.r + pb.z, bf.x, dd);
     } else if (u.bonds >= 1 && u.bonds <= 3) {
         BondCtx bc;
-        bc.L = bid_ok ? a.BL[S.b0 + bid] : 0.0f;
+        const float4 bl = bid_ok ? a.BLS[S.b0 + bid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bc.L = bl.x;
         if (S.has_strength && bid_ok) {
-            const float2 st = a.BSTR[S.b0 + bid];
-            bc.sn = st.x;
-            bc.ss = st.y;
+            bc.sn = bl.y;
+            bc.ss = bl.z;
         } else {
             bc.sn = u.bond_normal_strength;
             bc.ss = u.bond_shear_strength;
@@ -470,9 +472,9 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
     } else {
         f = make_float3(0.0f, 0.0f, 0.0f);
     }
-    if (bonded != br.z) {   // the record word changes only when the bond breaks
-        br.z = bonded;
-        a.BR[q] = br;
+    if (bonded != rb.y) {   // the record word changes only when the bond breaks
+        a.BR[q].z = bonded;
+        a.BRB[q].y = bonded;
     }
     a.BF[q] = bf;
     return make_float4(f.x, f.y, f.z, 1.0f);
