@@ -46,7 +46,7 @@ constexpr uint32_t VW_PATH = (1u << 31) - 1u;
 constexpr uint32_t VW_PATH_TRIG = 1u << 24;
 constexpr uint32_t VW_PATH_SAT = 1u << 30;
 
-// per-particle flag bits (word M.x bits 16..31; bits 0..15 hold the material id)
+// per-particle flag bits (word M bits 16..24; bits 0..15 hold the material id)
 enum : uint32_t {
     PF_KICK_X = 1u << 16,   // fixity.x_vel == 0: velocity integrates (2D_LOM.wgsl:88)
     PF_KICK_Y = 1u << 17,
@@ -59,7 +59,7 @@ enum : uint32_t {
     PF_LARGE = 1u << 24,    // a large body: on the scene's large-body list, not found through cells
     PF_MAT_MASK = 0xffffu,
 };
-// Bits 25..29 of M.x: the particle's contact count (entries of its contact list), kept
+// Bits 25..29 of M: the particle's contact count (entries of its contact list), kept
 // beside the flags so that the step kernel reads it with them; rewritten only when it
 // changes (neighbours read the word for its material bits, which never change).
 constexpr int PF_CC_SHIFT = 25;
@@ -136,7 +136,7 @@ struct DevArrays {
     Pool<float4> DATA1;     // {v*x, v*y, w*, 0}
     // contact lists, slot-major [KMAX][cap]: the partner and force columns are
     // double-buffered (every step kernel reads CP/CF and writes CPn/CFn); the count is
-    // part of M.x (cc_of)
+    // part of M (cc_of)
     Pool<int32_t> CP;       // partner id (scene-local), ascending
     Pool<float> CF;         // tangential force
     Pool<int32_t> CPn;
