@@ -58,8 +58,8 @@ def test_struck_chains_match_oracle(mode):
     prog.close()
 
 
-@pytest.mark.parametrize("mode", [3, 1])
-def test_struck_chains_without_data_match_oracle(mode):
+@pytest.mark.parametrize("mode,block", [(3, 256), (1, 256), (3, 64)])
+def test_struck_chains_without_data_match_oracle(mode, block):
     """Without data recording the bond pass sums each contact-free owner's records in
     record order itself (one sum per owner for the step kernel); owners in contact keep
     per-record results.  Both paths run here (the striker's neighbours are in contact,
@@ -67,7 +67,7 @@ def test_struck_chains_without_data_match_oracle(mode):
     oracle."""
     s, st = chain_scene(mode)
     n, nb = st.p_count, st.n_bonds
-    prog = ComputeProg(s, st.copy(), record_data=False, reorder_interval=8)
+    prog = ComputeProg(s, st.copy(), record_data=False, reorder_interval=8, block_size=block)
     eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
     in_contact = 0
     for k in (120, 240, 240):

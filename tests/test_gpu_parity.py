@@ -81,6 +81,27 @@ def test_zero_radius_discs_and_wall_bounces_bit_exact(zero):
     prog.close()
 
 
+@pytest.mark.parametrize("block", [64, 128])
+def test_smaller_blocks_bit_exact(block):
+    """The autotuner may pick 64- or 128-slot blocks: scene padding, the step kernel's
+    block argument, re-sort units of 64 blocks and the rebuild kernels all follow the
+    block size.  A compressed box with re-sorts every 3 steps, identical to the oracle
+    (state and contact table) over two calls."""
+    s, st = scenes.granular_box(n_side=40, seed=7, spacing=1.75)
+    n = st.p_count
+    prog = ComputeProg(s, st.copy(), reorder_interval=3, block_size=block)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    for k in (9, 21):
+        prog.step(k)
+        eng.step(k)
+    got = prog.update_state()
+    assert_bits_equal(got, eng.st, n)
+    want = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n]
+    assert np.array_equal(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want)
+    assert prog.counters()["reorders"] >= 5
+    prog.close()
+
+
 def test_granular_contacts_dense_bit_exact():
     """A compressed box (spacing 1.8 < 2 r_max) so that most discs touch several
     neighbours; 30 steps bit-exact including the contact table."""
