@@ -31,6 +31,23 @@ __host__ __device__ inline int nl_first(int w) { return w & ((1 << NL_SHIFT) - 1
 constexpr int DATA_WORDS = 7;
 constexpr int REC_DEAD = INT32_MIN;   // bonded word of a dead bond record in BRB
 
+// Break word of a bond: the bits of BLS.w, so that the bond pass reads a bond's break
+// state in the same 16-byte load as its rest length and strengths.  0 while intact;
+// after a break, bit 31 set, bits 2..30 the step of the break mod 2^29 and bits 0..1
+// the code + 12 for the codes -11..-9 of the bond laws (2D_Simulation.wgsl:322-324),
+// or 0 for any other code (an uploaded one), which is then read from BS.x.
+constexpr uint32_t BRK_SET = 1u << 31;
+constexpr uint32_t BRK_STEP_MASK = (1u << 29) - 1u;
+__host__ __device__ inline uint32_t brk_word(int code, uint32_t step) {
+    const uint32_t c = (code >= -11 && code <= -9) ? (uint32_t)(code + 12) : 0u;
+    return BRK_SET | ((step & BRK_STEP_MASK) << 2) | c;
+}
+// Whether break word w was written before step `step` (a break is seen by the bond's
+// other endpoint from the next step on); equal steps mod 2^29 read as the same step.
+__host__ __device__ inline bool brk_before(uint32_t w, uint32_t step) {
+    return (w & BRK_SET) && ((w >> 2) & BRK_STEP_MASK) != (step & BRK_STEP_MASK);
+}
+
 // Word V.w of the velocity record (bits, not a float):
 //  VW_EXPLICIT  the displacement of the last drift, {del_x, del_y, del_rot}, is the
 //               explicit record DX of the same buffer; otherwise it is the stored
@@ -148,8 +165,8 @@ struct DevArrays {
     Pool<int4> BR;          // {partner id, bond id, bonded, alive}
     Pool<int2> BRB;         // the bond pass's copy: {bond id, bonded, or REC_DEAD once dead}
     Pool<float4> BF;        // {ft, fbt, theta_b, 0}
-    Pool<int2> BS;          // per bond {index word, step of break}
-    Pool<float4> BLS;       // per bond {rest length, normal strength, shear strength, 0}
+    Pool<int2> BS;          // per bond {index word, step of break} (download and counts)
+    Pool<float4> BLS;       // per bond {rest length, normal strength, shear strength, break word bits}
     Pool<const dem_material> MAT;
     // bond pass: owner of each record (global particle id, -1: none) and its results
     Pool<int32_t> LB;       // large-body particle ids, LB_MAX per scene

@@ -648,6 +648,8 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
                                                 v.bond_strengths[2 * k + 1], 0.0f)
                                   : make_float4(bitsf(v.bonds[3 * k + 2]), sh.params.u.bond_normal_strength,
                                                 sh.params.u.bond_shear_strength, 0.0f);
+        // a bond uploaded broken reads as broken before the next step
+        if (idx < 0) BLS[k].w = bitsf((int32_t)brk_word(idx, (uint32_t)sh.step));
     }
     sh.has_strength = v.bond_strengths != nullptr;
     // contact table -> contact lists (plain contacts) and bond records (everything else)

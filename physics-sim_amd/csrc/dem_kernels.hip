@@ -443,10 +443,14 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
     }
     const int bid = rb.x;
     const bool bid_ok = bid >= 0 && bid < S.nb;
-    if (bonded >= 0) {
+    // {rest length, strengths, break word}: one load gives the break state and the law's
+    // constants
+    float4 bl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (bonded >= 0 && bid_ok) {
+        bl = a.BLS[S.b0 + bid];
         // a break recorded by the partner in an earlier step is seen here
-        const int2 bs = bid_ok ? a.BS[S.b0 + bid] : make_int2(0, 0);
-        if (bs.x < 0 && bs.y < step) bonded = bs.x;
+        const uint32_t bw = __float_as_uint(bl.w);
+        if (brk_before(bw, (uint32_t)step)) bonded = (bw & 3u) ? (int)(bw & 3u) - 12 : a.BS[S.b0 + bid].x;
     }
     const float2 kb = partner_k(a, S, j);
     float3 f;
@@ -454,7 +458,6 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
         f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb.x, kb.y, me.r + pb.z, bf.x, dd);
     } else if (u.bonds >= 1 && u.bonds <= 3) {
         BondCtx bc;
-        const float4 bl = bid_ok ? a.BLS[S.b0 + bid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         bc.L = bl.x;
         if (S.has_strength && bid_ok) {
             bc.sn = bl.y;
@@ -467,7 +470,10 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
         f = bond_law<F64>(u, me, pb, db, kb.x, kb.y, bc, bf, dd, code);
         if (code < 0) {
             bonded = code;
-            if (bid_ok) a.BS[S.b0 + bid] = make_int2(code, step);
+            if (bid_ok) {
+                a.BS[S.b0 + bid] = make_int2(code, step);
+                a.BLS[S.b0 + bid].w = __uint_as_float(brk_word(code, (uint32_t)step));
+            }
         }
     } else {
         f = make_float3(0.0f, 0.0f, 0.0f);

@@ -60,11 +60,9 @@ def test_struck_chains_match_oracle(mode):
 
 @pytest.mark.parametrize("mode,block", [(3, 256), (1, 256), (3, 64)])
 def test_struck_chains_without_data_match_oracle(mode, block):
-    """Without data recording the bond pass sums each contact-free owner's records in
-    record order itself (one sum per owner for the step kernel); owners in contact keep
-    per-record results.  Both paths run here (the striker's neighbours are in contact,
-    the rest of the chains are not): state, break codes and contact table equal the
-    oracle."""
+    """Without data recording (the bond pass keeps no data log per record), at 256- and
+    64-slot blocks: state, break codes and contact table equal the oracle, with the
+    striker's neighbours in contact, so contacts merge with bond records."""
     s, st = chain_scene(mode)
     n, nb = st.p_count, st.n_bonds
     prog = ComputeProg(s, st.copy(), record_data=False, reorder_interval=8, block_size=block)
@@ -80,7 +78,42 @@ def test_struck_chains_without_data_match_oracle(mode, block):
     assert_bits(got.bonds[: 3 * nb], eng.st.bonds[: 3 * nb], "bond words (break codes)")
     want = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n]
     assert_bits(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want, "contact table")
-    assert in_contact > 0          # some owners took the per-record path
+    assert in_contact > 0          # contacts merged with bond records
+    prog.close()
+
+
+def _assert_same(got, eng, n, nb):
+    for name in FIELDS:
+        assert_bits(getattr(got, name)[: _n(name, n)], getattr(eng.st, name)[: _n(name, n)], name)
+    assert_bits(got.data[: 7 * n], eng.st.data[: 7 * n], "data")
+    assert_bits(got.bonds[: 3 * nb], eng.st.bonds[: 3 * nb], "bond words (break codes)")
+    want = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n]
+    assert_bits(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want, "contact table")
+
+
+def test_uploaded_broken_bonds_match_oracle():
+    """Bonds uploaded broken while both endpoints' contact slots still hold them (a
+    state saved between the two endpoints' passes, or edited): both endpoints take the
+    bond's word at the first step, a bond law's code (-10) and any other negative word
+    (-3) alike.  Then a torn state downloaded mid-run and restored continues equal to
+    the oracle started from the same state."""
+    s, st = chain_scene(3)
+    n, nb = st.p_count, st.n_bonds
+    cut = np.arange(5, nb, 17)
+    st.bonds[3 * cut] = np.where(np.arange(len(cut)) % 2 == 0, -10, -3).astype(np.int32)
+    prog = ComputeProg(s, st.copy(), record_data=True, reorder_interval=8)
+    eng = orc.OracleEngine(st.copy(), s, mode=orc.MODE_CANONICAL)
+    prog.step(120)
+    eng.step(120)
+    mid = prog.update_state()
+    _assert_same(mid, eng, n, nb)
+    assert prog.broken_bond_count() >= len(cut)
+    prog.restore(mid.copy())
+    eng = orc.OracleEngine(mid.copy(), s, mode=orc.MODE_CANONICAL)
+    for k in (1, 239):
+        prog.step(k)
+        eng.step(k)
+    _assert_same(prog.update_state(), eng, n, nb)
     prog.close()
 
 
