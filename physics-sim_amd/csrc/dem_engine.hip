@@ -13,18 +13,27 @@ hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, b
                        hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 hipError_t launch_bin(const StepArgs& args, int nblocks, int block, hipStream_t st);
 int scan_tiles(int n);
-hipError_t launch_scan(const DevArrays& a, int ncells, const int32_t* gate, hipStream_t st);
-hipError_t launch_rb_bin(const StepArgs& args, int nlb, int grid, int block, hipStream_t st);
-hipError_t launch_rb_scatter(const StepArgs& args, int nlb, int grid, int block, hipStream_t st);
-hipError_t launch_build_nl(const StepArgs& args, int nlb, int grid, int block, hipStream_t st);
-hipError_t launch_rb_store(int32_t* rb, int v, hipStream_t st);
+hipError_t launch_scan(const DevArrays& a, int ncells, const int32_t* gate, hipStream_t st,
+                       hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_rb_bin(const StepArgs& args, int nlb, int grid, int block, hipStream_t st,
+                         hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_rb_scatter(const StepArgs& args, int nlb, int grid, int block, hipStream_t st,
+                             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_build_nl(const StepArgs& args, int nlb, int grid, int block, hipStream_t st,
+                           hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_rb_store(int32_t* rb, int v, hipStream_t st,
+                           hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 hipError_t launch_scatter(const StepArgs& args, int nblocks, int block, hipStream_t st);
 hipError_t launch_cell_sort(const StepArgs& args, int cell_lo, int cell_hi, int p0, hipStream_t st);
-hipError_t launch_reorder_move(const ReorderArgs& r, int nblocks, int block, hipStream_t st);
-hipError_t launch_partition(const ReorderArgs& r, int nblocks, int block, hipStream_t st);
-hipError_t launch_bond_forces(const StepArgs& args, int nrec, int lbs, bool f64, hipStream_t st);
-hipError_t launch_record_slots(const StepArgs& args, int nrec, int lbs, hipStream_t st);
+hipError_t launch_reorder_move(const ReorderArgs& r, int nblocks, int block, hipStream_t st,
+                               hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_partition(const ReorderArgs& r, int nblocks, int block, hipStream_t st,
+                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_bond_forces(const StepArgs& args, int nrec, int lbs, bool f64, hipStream_t st,
+                              hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_record_slots(const StepArgs& args, int nrec, int lbs, hipStream_t st,
+                               hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, int np, double* out, hipStream_t st);
 hipError_t launch_count_contacts(const int32_t* m, int p0, int np, unsigned long long* out, hipStream_t st);
 hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
@@ -836,15 +845,15 @@ hipError_t Engine::run_op(const LaunchOp& op, hipEvent_t ev0, hipEvent_t ev1) {
     switch (op.kind) {
         case OP_STEP:
             return launch_step(op.sa, op.nlb, op.block, op.f64 != 0, op.bonds != 0, op.value != 0, stream_, ev0, ev1);
-        case OP_RB_STORE: return launch_rb_store(op.rb, op.value, stream_);
-        case OP_RB_BIN: return launch_rb_bin(op.sa, op.nlb, op.grid, op.block, stream_);
-        case OP_SCAN: return launch_scan(op.da, op.ncells, op.gate, stream_);
-        case OP_RB_SCATTER: return launch_rb_scatter(op.sa, op.nlb, op.grid, op.block, stream_);
-        case OP_BUILD_NL: return launch_build_nl(op.sa, op.nlb, op.grid, op.block, stream_);
-        case OP_PARTITION: return launch_partition(op.ra, op.nlb, op.block, stream_);
-        case OP_MOVE: return launch_reorder_move(op.ra, op.nlb, op.block, stream_);
-        case OP_BONDS: return launch_bond_forces(op.sa, op.ncells, op.block, op.f64 != 0, stream_);
-        case OP_RSLOTS: return launch_record_slots(op.sa, op.ncells, op.block, stream_);
+        case OP_RB_STORE: return launch_rb_store(op.rb, op.value, stream_, ev0, ev1);
+        case OP_RB_BIN: return launch_rb_bin(op.sa, op.nlb, op.grid, op.block, stream_, ev0, ev1);
+        case OP_SCAN: return launch_scan(op.da, op.ncells, op.gate, stream_, ev0, ev1);
+        case OP_RB_SCATTER: return launch_rb_scatter(op.sa, op.nlb, op.grid, op.block, stream_, ev0, ev1);
+        case OP_BUILD_NL: return launch_build_nl(op.sa, op.nlb, op.grid, op.block, stream_, ev0, ev1);
+        case OP_PARTITION: return launch_partition(op.ra, op.nlb, op.block, stream_, ev0, ev1);
+        case OP_MOVE: return launch_reorder_move(op.ra, op.nlb, op.block, stream_, ev0, ev1);
+        case OP_BONDS: return launch_bond_forces(op.sa, op.ncells, op.block, op.f64 != 0, stream_, ev0, ev1);
+        case OP_RSLOTS: return launch_record_slots(op.sa, op.ncells, op.block, stream_, ev0, ev1);
         default: return hipErrorInvalidValue;
     }
 }
@@ -881,12 +890,10 @@ int Engine::run_ops(const std::vector<LaunchOp>& ops) {
         if (timing_) { t0 = ev(); t1 = ev(); hipEventRecord(t0, stream_); }
         for (const LaunchOp& op : ops) {
             hipEvent_t a = nullptr, b = nullptr;
-            // the step kernel's events carry its own start and end; other launches are
-            // bracketed by events recorded on the stream
-            const bool own = op.kind == OP_STEP;
-            if (timing_) { a = ev(); b = ev(); if (!own) hipEventRecord(a, stream_); }
-            const hipError_t e = own && timing_ ? run_op(op, a, b) : run_op(op);
-            if (timing_) { if (!own) hipEventRecord(b, stream_); evs.push_back({op.timed, op.kind, a, b}); }
+            // every launch's events carry the kernel's own start and end
+            if (timing_) { a = ev(); b = ev(); }
+            const hipError_t e = run_op(op, a, b);
+            if (timing_) evs.push_back({op.timed, op.kind, a, b});
             if (int rc = check(e, kOpNames[op.kind])) return rc;
         }
         if (timing_) {

@@ -1623,13 +1623,18 @@ __global__ void k_bond_pairs(StepArgs args, int reach, int maxb, float factor, i
 
 // ---- launchers (host context) ----
 
+// With events (timing only, never under graph capture) hipExtLaunchKernelGGL stamps
+// them with the kernel's own start and end.
+template <class K, class... A>
+static void launch_k(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, A... args) {
+    if (ev0) hipExtLaunchKernelGGL(kernel, g, b, 0, st, ev0, ev1, 0, args...);
+    else hipLaunchKernelGGL(kernel, g, b, 0, st, args...);
+}
+
 template <class K>
 static void launch_step_kernel(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
                                const StepArgs& a) {
-    // with events (timing only, never under graph capture): hipExtLaunchKernelGGL
-    // stamps them with the kernel's own start and end
-    if (ev0) hipExtLaunchKernelGGL(kernel, g, b, 0, st, ev0, ev1, 0, a);
-    else hipLaunchKernelGGL(kernel, g, b, 0, st, a);
+    launch_k(kernel, g, b, st, ev0, ev1, a);
 }
 
 hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, bool bonds, bool large,
@@ -1648,17 +1653,19 @@ hipError_t launch_step(const StepArgs& args, int nblocks, int block, bool f64, b
     return hipGetLastError();
 }
 
-hipError_t launch_bond_forces(const StepArgs& args, int nrec, int lbs, bool f64, hipStream_t st) {
+hipError_t launch_bond_forces(const StepArgs& args, int nrec, int lbs, bool f64, hipStream_t st,
+                              hipEvent_t ev0, hipEvent_t ev1) {
     const int b = 256;
     const dim3 g((unsigned)((nrec + b - 1) / b));
-    if (f64) hipLaunchKernelGGL(k_bond_forces<true>, g, dim3(b), 0, st, args, nrec, lbs);
-    else hipLaunchKernelGGL(k_bond_forces<false>, g, dim3(b), 0, st, args, nrec, lbs);
+    if (f64) launch_k(k_bond_forces<true>, g, dim3(b), st, ev0, ev1, args, nrec, lbs);
+    else launch_k(k_bond_forces<false>, g, dim3(b), st, ev0, ev1, args, nrec, lbs);
     return hipGetLastError();
 }
 
-hipError_t launch_record_slots(const StepArgs& args, int nrec, int lbs, hipStream_t st) {
+hipError_t launch_record_slots(const StepArgs& args, int nrec, int lbs, hipStream_t st, hipEvent_t ev0,
+                               hipEvent_t ev1) {
     const int b = 256;
-    hipLaunchKernelGGL(k_record_slots, dim3((unsigned)((nrec + b - 1) / b)), dim3(b), 0, st, args, nrec, lbs);
+    launch_k(k_record_slots, dim3((unsigned)((nrec + b - 1) / b)), dim3(b), st, ev0, ev1, args, nrec, lbs);
     return hipGetLastError();
 }
 
@@ -1671,30 +1678,34 @@ hipError_t launch_bin(const StepArgs& args, int nblocks, int block, hipStream_t 
 int scan_tiles(int n) { return tile_count(n); }
 
 // gate: NULL to scan unconditionally, else the rebuild flag
-hipError_t launch_scan(const DevArrays& a, int ncells, const int32_t* gate, hipStream_t st) {
+hipError_t launch_scan(const DevArrays& a, int ncells, const int32_t* gate, hipStream_t st, hipEvent_t ev0,
+                       hipEvent_t ev1) {
     const int nt = scan_tiles(ncells);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), 0, st, a.CNT.p, a.START.p, ncells, a.TSUM.p,
-                       gate);
+    launch_k(k_scan_tiles, dim3(nt), dim3(SCAN_THREADS), st, ev0, ev1, a.CNT.p, a.START.p, ncells, a.TSUM.p,
+             gate);
     return hipGetLastError();
 }
 
-hipError_t launch_rb_bin(const StepArgs& args, int nlb, int grid, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_rb_bin, dim3(grid), dim3(block), 0, st, args, nlb);
+hipError_t launch_rb_bin(const StepArgs& args, int nlb, int grid, int block, hipStream_t st, hipEvent_t ev0,
+                         hipEvent_t ev1) {
+    launch_k(k_rb_bin, dim3(grid), dim3(block), st, ev0, ev1, args, nlb);
     return hipGetLastError();
 }
 
-hipError_t launch_rb_scatter(const StepArgs& args, int nlb, int grid, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_rb_scatter, dim3(grid), dim3(block), 0, st, args, nlb);
+hipError_t launch_rb_scatter(const StepArgs& args, int nlb, int grid, int block, hipStream_t st, hipEvent_t ev0,
+                             hipEvent_t ev1) {
+    launch_k(k_rb_scatter, dim3(grid), dim3(block), st, ev0, ev1, args, nlb);
     return hipGetLastError();
 }
 
-hipError_t launch_build_nl(const StepArgs& args, int nlb, int grid, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_build_nl, dim3(grid), dim3(block), 0, st, args, nlb);
+hipError_t launch_build_nl(const StepArgs& args, int nlb, int grid, int block, hipStream_t st, hipEvent_t ev0,
+                           hipEvent_t ev1) {
+    launch_k(k_build_nl, dim3(grid), dim3(block), st, ev0, ev1, args, nlb);
     return hipGetLastError();
 }
 
-hipError_t launch_rb_store(int32_t* rb, int v, hipStream_t st) {
-    hipLaunchKernelGGL(k_rb_store, dim3(1), dim3(64), 0, st, rb, v);
+hipError_t launch_rb_store(int32_t* rb, int v, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+    launch_k(k_rb_store, dim3(1), dim3(64), st, ev0, ev1, rb, v);
     return hipGetLastError();
 }
 
@@ -1711,13 +1722,15 @@ hipError_t launch_cell_sort(const StepArgs& args, int cell_lo, int cell_hi, int 
     return hipGetLastError();
 }
 
-hipError_t launch_reorder_move(const ReorderArgs& r, int nblocks, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_reorder_move, dim3(nblocks), dim3(block), 0, st, r);
+hipError_t launch_reorder_move(const ReorderArgs& r, int nblocks, int block, hipStream_t st, hipEvent_t ev0,
+                               hipEvent_t ev1) {
+    launch_k(k_reorder_move, dim3(nblocks), dim3(block), st, ev0, ev1, r);
     return hipGetLastError();
 }
 
-hipError_t launch_partition(const ReorderArgs& r, int nblocks, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_partition, dim3((nblocks + PART_BLOCKS - 1) / PART_BLOCKS), dim3(block), 0, st, r, nblocks);
+hipError_t launch_partition(const ReorderArgs& r, int nblocks, int block, hipStream_t st, hipEvent_t ev0,
+                            hipEvent_t ev1) {
+    launch_k(k_partition, dim3((nblocks + PART_BLOCKS - 1) / PART_BLOCKS), dim3(block), st, ev0, ev1, r, nblocks);
     return hipGetLastError();
 }
 
