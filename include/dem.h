@@ -182,6 +182,18 @@ int dem_sync(dem_ctx* ctx);
  * state.rs:708-737).  kinetic energy = Σ ½ m |v|² + ½ I ω² over active particles,
  * accumulated in f64 in particle-id order. */
 int dem_kinetic_energy(dem_ctx* ctx, double* out);
+/* Largest scene, in particle slots (the count rounded up to the block size), that
+ * dem_kinetic_energy_noalloc accepts. */
+#define DEM_KE_NOALLOC_MAX 4194304
+/* dem_kinetic_energy without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Partial results live in a static device array of
+ * the library (one per GPU, 33.6 MB, held until the process ends), reduced on the
+ * device in a fixed order: the result equals dem_kinetic_energy's to f64 rounding
+ * (about 1e-15 relative).  Scenes above DEM_KE_NOALLOC_MAX slots get DEM_ECAPACITY.
+ * Every call on one GPU shares its array: calls that overlap on one GPU race and must
+ * be serialised by the caller.  Synchronous. */
+int dem_kinetic_energy_noalloc(dem_ctx* ctx, double* out);
 /* Number of bonds whose index word is negative (broken). */
 int dem_broken_bond_count(dem_ctx* ctx, int64_t* out);
 /* Counters: [0] steps since upload, [1] neighbour-list rebuilds (broad phases),

@@ -7,8 +7,16 @@
 #include "dem_engine.h"
 
 namespace psim {
-static thread_local std::string g_last_error;
-void set_error(const std::string& msg) { g_last_error = msg; }
+// A fixed buffer, so that setting an error allocates nothing (the no-allocation entry
+// points report their errors through it too).
+static thread_local char g_last_error[1024];
+void set_error(const char* msg) {
+    size_t k = 0;
+    for (; msg && msg[k] && k + 1 < sizeof(g_last_error); k++) g_last_error[k] = msg[k];
+    g_last_error[k] = 0;
+}
+void set_error(const std::string& msg) { set_error(msg.c_str()); }
+int init_static_scratch(int device);
 }  // namespace psim
 
 struct dem_ctx {
@@ -34,7 +42,7 @@ using psim::set_error;
 
 extern "C" {
 
-const char* dem_last_error(void) { return psim::g_last_error.c_str(); }
+const char* dem_last_error(void) { return psim::g_last_error; }
 
 const char* dem_version(void) { return "psim-dem 0.1 (gfx950)"; }
 
@@ -60,6 +68,12 @@ int dem_create(const dem_params* params, int32_t capacity, int32_t bond_capacity
         int rc = c->eng.init(slots, cells, bond_capacity, recs, params->n_materials > 8 ? params->n_materials : 8, 1,
                              block, true);
         if (rc == DEM_OK) rc = c->eng.add_scene(*params, capacity, bond_capacity, (int32_t)recs, &c->sid);
+        if (rc != DEM_OK) {
+            delete c;
+            return rc;
+        }
+        // the no-allocation observables' device scratch: its address on this device
+        rc = psim::init_static_scratch(c->eng.device());
         if (rc != DEM_OK) {
             delete c;
             return rc;
@@ -121,6 +135,15 @@ int dem_kinetic_energy(dem_ctx* ctx, double* out) {
         if (!ctx || !out) { set_error("dem_kinetic_energy: NULL argument"); return DEM_EINVAL; }
         return ctx->eng.kinetic_energy(ctx->sid, out);
     })
+}
+
+int dem_kinetic_energy_noalloc(dem_ctx* ctx, double* out) {
+    // no DEM_GUARD: nothing here allocates or throws
+    if (!ctx || !out) {
+        set_error("dem_kinetic_energy_noalloc: NULL argument");
+        return DEM_EINVAL;
+    }
+    return ctx->eng.kinetic_energy_noalloc(ctx->sid, out);
 }
 
 int dem_broken_bond_count(dem_ctx* ctx, int64_t* out) {

@@ -15,6 +15,7 @@
 namespace psim {
 
 void set_error(const std::string& msg);
+void set_error(const char* msg);
 
 struct DevSet {
     float4* P[2] = {nullptr, nullptr};    // by position parity (cur_)
@@ -120,6 +121,9 @@ public:
     int step(const std::vector<int32_t>& sids, const std::vector<int32_t>& nsteps);
 
     int kinetic_energy(int32_t sid, double* out);
+    // the same sum without allocating: partial results in the library's static device
+    // scratch (at most DEM_KE_NOALLOC_MAX slots), reduced on the device in a fixed order
+    int kinetic_energy_noalloc(int32_t sid, double* out);
     int broken_bonds(int32_t sid, int64_t* out);
     int counters(int32_t sid, int64_t* out, int32_t n_out);
     int datum(int32_t sid, const int32_t* select, double* out);
@@ -141,6 +145,7 @@ public:
     int64_t n_step = 0, n_scan = 0, n_scatter = 0, n_reorder = 0;
 
     hipStream_t stream() const { return stream_; }
+    int device() const { return device_; }
     int block() const { return block_; }
     const SceneHost& scene(int32_t sid) const { return scenes_[sid]; }
     int32_t scene_count() const { return (int32_t)scenes_.size(); }

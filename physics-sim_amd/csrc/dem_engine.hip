@@ -38,6 +38,8 @@ hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, i
 hipError_t launch_count_contacts(const int32_t* m, int p0, int np, unsigned long long* out, hipStream_t st);
 hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
                              int32_t* out_cnt, int32_t* out_j, hipStream_t st);
+hipError_t launch_ke_reduce(double* buf, int n, double* host_out, hipStream_t st);
+double* static_scratch(int device);
 
 namespace {
 inline int32_t fbits(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
@@ -1274,6 +1276,33 @@ int Engine::kinetic_energy(int32_t sid, double* out) {
     double s = 0.0;
     for (int32_t i = 0; i < sh.n; i++) s += ke[i];
     *out = s;
+    return DEM_OK;
+}
+
+// No allocation on any path: the per-id energies (the kernel of kinetic_energy) and the
+// partial sums live in the static scratch, the total is copied into *out.
+int Engine::kinetic_energy_noalloc(int32_t sid, double* out) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) {
+        set_error("kinetic_energy_noalloc: no such scene");
+        return DEM_EINVAL;
+    }
+    const SceneHost& sh = scenes_[sid];
+    if ((int64_t)sh.np > (int64_t)DEM_KE_NOALLOC_MAX) {
+        set_error("dem_kinetic_energy_noalloc: the scene has more particle slots than the static scratch holds "
+                  "(DEM_KE_NOALLOC_MAX = 4194304)");
+        return DEM_ECAPACITY;
+    }
+    double* buf = static_scratch(device_);
+    if (!buf) {
+        set_error("dem_kinetic_energy_noalloc: static scratch not resolved on this device");
+        return DEM_ESTATE;
+    }
+    hipError_t e = launch_kinetic(arrays(set_), cur_, sh.m0, sh.nm, (int)sh.p0, sh.np, buf, stream_);
+    if (e == hipSuccess) e = launch_ke_reduce(buf, sh.n, out, stream_);
+    if (e != hipSuccess) {
+        set_error("dem_kinetic_energy_noalloc: HIP error in the kinetic-energy kernels");
+        return e == hipErrorOutOfMemory ? DEM_ENOMEM : DEM_EDEVICE;
+    }
     return DEM_OK;
 }
 

@@ -13,6 +13,7 @@ DEM_EDEVICE = -3
 DEM_ECAPACITY = -4
 DEM_ESTATE = -5
 DEM_EBUSY = -6
+DEM_KE_NOALLOC_MAX = 4194304   # include/dem.h
 
 UNIFORM_FIELDS = [
     ("walls", C.c_int32), ("hor_bound", C.c_float), ("vert_bound", C.c_float), ("round_bounds", C.c_int32),
@@ -116,6 +117,7 @@ LIB_PATH = os.path.join(os.path.dirname(PKG_DIR), "csrc", "libpsim.so")
 EXPORTS = [
     "dem_last_error", "dem_version", "dem_device_count", "dem_create", "dem_destroy", "dem_set_params",
     "dem_upload_state", "dem_step", "dem_download_state", "dem_sync", "dem_kinetic_energy",
+    "dem_kinetic_energy_noalloc",
     "dem_broken_bond_count", "dem_counters", "dem_get_datum", "dem_debug_broad_phase",
     "dem_debug_contacts", "dem_build_bond_pairs", "dem_last_step_timing",
     "dem_server_create", "dem_server_destroy", "dem_server_submit", "dem_server_tick", "dem_server_status",
@@ -150,6 +152,7 @@ def load_library(path: str = LIB_PATH):
     lib.dem_download_state.argtypes = [C.c_void_p, C.POINTER(DemStateView)]
     lib.dem_sync.argtypes = [C.c_void_p]
     lib.dem_kinetic_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    lib.dem_kinetic_energy_noalloc.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     lib.dem_broken_bond_count.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     lib.dem_counters.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32]
     lib.dem_get_datum.argtypes = [C.c_void_p, IP, C.POINTER(C.c_double)]
