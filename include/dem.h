@@ -272,23 +272,30 @@ typedef struct dem_server_params {
 #define DEM_REQ_QUEUED   0
 #define DEM_REQ_RESIDENT 1
 #define DEM_REQ_DONE     2
+#define DEM_REQ_FAILED   3   /* refused at admission (its parameters or state); dem_server_take
+                                returns the error and its message */
 
 int dem_server_create(const dem_server_params* params, dem_server** out);
 int dem_server_destroy(dem_server* server);
 /* Queues a scene: `params` as for dem_create, `state` as for dem_upload_state, run for
- * `nsteps` steps.  Writes a ticket. */
+ * `nsteps` steps.  Writes a ticket.  Parameters, cell size and material count are checked
+ * here (DEM_EINVAL / DEM_ECAPACITY: nothing is queued); a scene whose state the engine
+ * refuses at admission (e.g. a contact table beyond 14 slots) becomes DEM_REQ_FAILED and
+ * the queue moves on. */
 int dem_server_submit(dem_server* server, const dem_params* params, const dem_state_view* state, int32_t nsteps,
                       int64_t* ticket);
 /* One tick (admit, advance, retire); writes how many scenes finished in it. Synchronous. */
 int dem_server_tick(dem_server* server, int32_t* completed);
-/* DEM_REQ_QUEUED / DEM_REQ_RESIDENT / DEM_REQ_DONE */
+/* DEM_REQ_QUEUED / DEM_REQ_RESIDENT / DEM_REQ_DONE / DEM_REQ_FAILED */
 int dem_server_status(dem_server* server, int64_t ticket, int32_t* status);
 /* Copies a finished scene's state into `out` (n must match; NULL arrays skipped),
- * writes its submit-to-done latency in seconds, and forgets the ticket. */
+ * writes its submit-to-done latency in seconds, and forgets the ticket.  For a failed
+ * scene it returns the admission's error code (message in dem_last_error) and forgets
+ * the ticket. */
 int dem_server_take(dem_server* server, int64_t ticket, dem_state_view* out, double* latency_s);
 /* out[0] completed, [1] resident, [2] queued, [3] p50 latency s, [4] p99 latency s,
  * [5] scenes/s from the first submit to the last completion, [6] particle-steps done,
- * [7] ticks, [8] step launches (max substeps per tick, summed). */
+ * [7] ticks, [8] step launches (max substeps per tick, summed), [9] scenes failed at admission. */
 int dem_server_stats(dem_server* server, double* out, int32_t n_out);
 
 #ifdef __cplusplus

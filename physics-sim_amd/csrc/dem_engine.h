@@ -17,6 +17,15 @@ namespace psim {
 void set_error(const std::string& msg);
 void set_error(const char* msg);
 
+// Large bodies of a radius set (rule in include/dem.h, dem_grid): their count, and the
+// largest radius of the other discs in *rsmax.
+int32_t split_large(const float* radii, int32_t n, float* rsmax);
+// Parameter checks shared by scene creation, dem_set_params and the serving queue.
+int validate_params(const dem_params& p);
+// Whether the grid's cells are wide enough for the 3x3 stencil: 2 rsmax + the rounding
+// margin of the cell index (4 ulp of the grid extent).
+int check_cell_size(const dem_grid& g, float rsmax);
+
 struct DevSet {
     float4* P[2] = {nullptr, nullptr};    // by position parity (cur_)
     float4* V[2] = {nullptr, nullptr};    // by list parity (lcur_), as DX
@@ -150,6 +159,7 @@ public:
     const SceneHost& scene(int32_t sid) const { return scenes_[sid]; }
     int32_t scene_count() const { return (int32_t)scenes_.size(); }
     int64_t slot_cap() const { return slot_cap_; }
+    int32_t material_capacity() const { return mat_cap_; }
     int64_t free_slots() const;
 
 private:

@@ -105,6 +105,7 @@ struct Slot {  // 2D_Simulation.wgsl:19-26
 
 Engine::~Engine() {
     if (stream_) hipStreamSynchronize(stream_);
+    clear_graphs();   // before the buffers their nodes refer to
     for (auto& s : sets_) {
         hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.V[0]); hipFree(s.V[1]); hipFree(s.DX[0]); hipFree(s.DX[1]);
         hipFree(s.A); hipFree(s.M); hipFree(s.ID); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
@@ -118,7 +119,6 @@ Engine::~Engine() {
     hipFree(STAT_); hipFree(TSUM_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
     hipFree(d_ke_); hipFree(d_cnt_);
     if (stage_) hipHostFree(stage_);
-    clear_graphs();
     for (auto e : ev_pool_) hipEventDestroy(e);
     if (stream_) hipStreamDestroy(stream_);
 }
@@ -289,7 +289,7 @@ static void release_region(std::vector<std::pair<int64_t, int64_t>>& used, int64
         if (used[i].first == at) { used.erase(used.begin() + i); return; }
 }
 
-static int validate_params(const dem_params& p) {
+int validate_params(const dem_params& p) {
     if (!p.materials || p.n_materials <= 0) { set_error("params: no materials"); return DEM_EINVAL; }
     if (!(p.grid.cell_size > 0.0f) || p.grid.w <= 0 || p.grid.h <= 0) {
         set_error("params: grid needs cell_size > 0, w > 0, h > 0");
@@ -297,6 +297,18 @@ static int validate_params(const dem_params& p) {
     }
     if ((int64_t)p.grid.w * p.grid.h > (int64_t)INT32_MAX / 4) { set_error("params: grid too large"); return DEM_EINVAL; }
     if (p.u.particles < 0) { set_error("params: negative particle count"); return DEM_EINVAL; }
+    return DEM_OK;
+}
+
+int check_cell_size(const dem_grid& g, float rsmax) {
+    const float cs = g.cell_size;
+    const float ext = cs * (float)std::max(g.w, g.h);
+    const float need = 2.0f * rsmax + 4.0f * (std::nextafter(ext, INFINITY) - ext);
+    if (cs < need) {
+        set_error("grid cell_size " + std::to_string(cs) + " is below 2*max radius (large bodies aside) + rounding "
+                  "margin " + std::to_string(need));
+        return DEM_EINVAL;
+    }
     return DEM_OK;
 }
 
@@ -399,14 +411,7 @@ int Engine::set_params(int32_t sid, const dem_params& p) {
     SceneHost& sh = scenes_[sid];
     if (p.grid.w * p.grid.h + 1 > sh.cell_cap) { set_error("grid larger than the scene's cell reservation"); return DEM_ECAPACITY; }
     if (p.n_materials > sh.mat_cap) { set_error("too many materials for the scene's reservation"); return DEM_ECAPACITY; }
-    if (sh.n > 0) {
-        const float cs = p.grid.cell_size;
-        const float ext = cs * (float)std::max(p.grid.w, p.grid.h);
-        if (cs < 2.0f * sh.rsmax + 4.0f * (std::nextafter(ext, INFINITY) - ext)) {
-            set_error("grid cell_size is below 2*max radius (large bodies aside) + rounding margin");
-            return DEM_EINVAL;
-        }
-    }
+    if (sh.n > 0 && (rc = check_cell_size(p.grid, sh.rsmax))) return rc;
     rb_force_ = true;   // the grid (and with it the skin) may have changed
     const int32_t n_keep = sh.params.u.particles;
     sh.params = p;
@@ -635,14 +640,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         for (int32_t i = 0; i < n; i++) rmax = std::max(rmax, v.radii[i]);
         sh.rmax = rmax;
         sh.rsmax = nlarge > 0 ? rsmax : rmax;
-        const float cs = sh.params.grid.cell_size;
-        const float ext = cs * (float)std::max(sh.params.grid.w, sh.params.grid.h);
-        const float need = 2.0f * sh.rsmax + 4.0f * (std::nextafter(ext, INFINITY) - ext);
-        if (cs < need) {
-            set_error("grid cell_size " + std::to_string(cs) + " is below 2*max radius + rounding margin " +
-                      std::to_string(need));
-            return DEM_EINVAL;
-        }
+        if (int rc = check_cell_size(sh.params.grid, sh.rsmax)) return rc;
     }
     // bonds: index word, rest length (angle kept on the host)
     sh.nb = v.n_bonds;

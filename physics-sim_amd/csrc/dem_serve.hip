@@ -55,6 +55,8 @@ struct Request {
     std::vector<int32_t> fixity, matptr, bonds;
     int32_t sid = -1;
     int status = DEM_REQ_QUEUED;
+    int error = DEM_OK;           // DEM_REQ_FAILED: the admission's error code and message
+    std::string error_text;
     double t_submit = 0, t_admit = 0, t_done = 0;
 
     dem_state_view view() {
@@ -101,7 +103,7 @@ struct dem_server {
     std::vector<int64_t> resident;
     std::unordered_map<int64_t, Request> reqs;
     std::vector<double> latencies;   // submit -> done, seconds, of every finished scene
-    int64_t completed = 0, ticks = 0, launches_steps = 0;
+    int64_t completed = 0, failed = 0, ticks = 0, launches_steps = 0;
     int64_t particle_steps_done = 0;
     double t_first_submit = -1.0, t_last_done = 0.0;
 };
@@ -123,9 +125,20 @@ struct dem_server {
 
 namespace {
 
+// A request the engine refused for a reason other than room (its parameters or its
+// state): it leaves the queue as DEM_REQ_FAILED with the error, and serving goes on.
+void fail(dem_server* s, Request& r, int rc) {
+    r.status = DEM_REQ_FAILED;
+    r.error = rc;
+    r.error_text = dem_last_error();
+    r.t_done = now_s();
+    s->failed++;
+}
+
 // Admits queued requests in arrival order.  When the head does not fit, up to
 // `bypass` later requests may be tried (smaller scenes fill the gaps); the head keeps
-// its place and is tried first again on the next tick.
+// its place and is tried first again on the next tick.  A request that fails for any
+// other reason is dropped from the queue as failed.
 int admit(dem_server* s) {
     const int32_t max_res = s->prm.max_resident > 0 ? s->prm.max_resident : INT32_MAX;
     size_t qi = 0;
@@ -140,12 +153,20 @@ int admit(dem_server* s) {
             qi++;
             continue;
         }
-        if (rc != DEM_OK) return rc;
-        dem_state_view v = r.view();
-        rc = s->eng.upload(sid, v);
+        if (rc == DEM_OK) {
+            dem_state_view v = r.view();
+            rc = s->eng.upload(sid, v);
+            if (rc != DEM_OK) {
+                const std::string why = dem_last_error();
+                s->eng.remove_scene(sid);
+                set_error(why);
+            }
+        }
+        if (rc == DEM_EDEVICE || rc == DEM_ENOMEM) return rc;   // the device, not the request
         if (rc != DEM_OK) {
-            s->eng.remove_scene(sid);
-            return rc;
+            fail(s, r, rc);
+            s->queue.erase(s->queue.begin() + (std::ptrdiff_t)qi);
+            continue;
         }
         r.sid = sid;
         r.status = DEM_REQ_RESIDENT;
@@ -156,19 +177,24 @@ int admit(dem_server* s) {
     return DEM_OK;
 }
 
+// Downloads and frees every finished scene.  The resident list is rebuilt as it goes:
+// after a failure the scenes not yet retired, the failed one among them, stay resident.
 int retire(dem_server* s) {
     std::vector<int64_t> keep;
+    int rc = DEM_OK;
     for (int64_t t : s->resident) {
         Request& r = s->reqs.at(t);
-        if (r.done < r.nsteps) {
+        if (rc != DEM_OK || r.done < r.nsteps) {
             keep.push_back(t);
             continue;
         }
         dem_state_view v = r.view();
-        int rc = s->eng.download(r.sid, v);
-        if (rc != DEM_OK) return rc;
-        rc = s->eng.remove_scene(r.sid);
-        if (rc != DEM_OK) return rc;
+        rc = s->eng.download(r.sid, v);
+        if (rc == DEM_OK) rc = s->eng.remove_scene(r.sid);
+        if (rc != DEM_OK) {
+            keep.push_back(t);
+            continue;
+        }
         r.sid = -1;
         r.status = DEM_REQ_DONE;
         r.t_done = now_s();
@@ -178,7 +204,7 @@ int retire(dem_server* s) {
         s->t_last_done = r.t_done;
     }
     s->resident.swap(keep);
-    return DEM_OK;
+    return rc;
 }
 
 }  // namespace
@@ -228,9 +254,16 @@ int dem_server_submit(dem_server* s, const dem_params* params, const dem_state_v
             set_error("dem_server_submit: state view needs n > 0 and every per-particle array");
             return DEM_EINVAL;
         }
-        if (!params->materials || params->n_materials <= 0) {
-            set_error("dem_server_submit: no materials");
-            return DEM_EINVAL;
+        // what admission would refuse for good is refused here, not queued
+        if (int rc = psim::validate_params(*params)) return rc;
+        if (std::max(params->n_materials, 8) > s->eng.material_capacity()) {
+            set_error("dem_server_submit: more materials than the server's material pool holds");
+            return DEM_ECAPACITY;
+        }
+        {
+            float rsmax = 0.0f;   // the largest radius, large bodies aside
+            psim::split_large(st->radii, st->n, &rsmax);
+            if (int rc = psim::check_cell_size(params->grid, rsmax)) return rc;
         }
         {
             // a scene that can never fit the pools is refused now rather than queued forever
@@ -333,6 +366,14 @@ int dem_server_take(dem_server* s, int64_t ticket, dem_state_view* out, double* 
         auto it = s->reqs.find(ticket);
         if (it == s->reqs.end()) { set_error("dem_server_take: unknown ticket"); return DEM_EINVAL; }
         Request& r = it->second;
+        if (r.status == DEM_REQ_FAILED) {
+            // the admission's error, once: the ticket is forgotten
+            const int rc = r.error;
+            set_error("dem_server_take: the scene failed admission: " + r.error_text);
+            if (latency_s) *latency_s = r.t_done - r.t_submit;
+            s->reqs.erase(it);
+            return rc;
+        }
         if (r.status != DEM_REQ_DONE) { set_error("dem_server_take: scene not finished"); return DEM_ESTATE; }
         if (out->n != r.n) { set_error("dem_server_take: view size differs from the scene"); return DEM_EINVAL; }
         const size_t n = (size_t)r.n;
@@ -381,6 +422,7 @@ int dem_server_stats(dem_server* s, double* out, int32_t n_out) {
             (double)s->particle_steps_done,
             (double)s->ticks,
             (double)s->launches_steps,
+            (double)s->failed,
         };
         const int32_t m = std::min<int32_t>(n_out, (int32_t)(sizeof(v) / sizeof(v[0])));
         for (int32_t i = 0; i < m; i++) out[i] = v[i];

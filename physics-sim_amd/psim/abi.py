@@ -70,7 +70,7 @@ class DemServerParams(C.Structure):
                 ("graph", C.c_int32)]
 
 
-DEM_REQ_QUEUED, DEM_REQ_RESIDENT, DEM_REQ_DONE = 0, 1, 2
+DEM_REQ_QUEUED, DEM_REQ_RESIDENT, DEM_REQ_DONE, DEM_REQ_FAILED = 0, 1, 2, 3
 
 
 def fptr(a):

@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import sys
 from typing import Optional
 
 import numpy as np
@@ -228,7 +229,18 @@ class ComputeProg:
             self.lib.dem_destroy(self.ctx)
             self.ctx = None
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
+        # No HIP calls while the interpreter shuts down: the runtime (and a profiler
+        # hooked into it) may already be torn down, and process exit frees the device
+        # memory anyway.  Callers close explicitly.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

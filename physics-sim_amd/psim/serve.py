@@ -11,6 +11,7 @@ A scene's result equals its solo run through `ComputeProg` bit for bit.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import time
 from typing import Optional
 
@@ -66,6 +67,7 @@ class Server:
         abi.check(self.lib.dem_server_create(C.byref(sp), C.byref(h)))
         self.h = h
         self._shape = {}
+        self.failures = {}
 
     def submit(self, settings: Settings, state: State, nsteps: int, grid: Optional[abi.DemGrid] = None,
                use_f64: Optional[bool] = None, reorder_interval: int = 64) -> int:
@@ -96,10 +98,10 @@ class Server:
         return state, lat.value
 
     def stats(self) -> dict:
-        out = (C.c_double * 9)()
-        abi.check(self.lib.dem_server_stats(self.h, out, 9))
+        out = (C.c_double * 10)()
+        abi.check(self.lib.dem_server_stats(self.h, out, 10))
         keys = ("completed", "resident", "queued", "p50_s", "p99_s", "scenes_per_s", "particle_steps", "ticks",
-                "step_launches")
+                "step_launches", "failed")
         return {k: out[i] for i, k in enumerate(keys)}
 
     def run_stream(self, requests, poll: float = 0.0):
@@ -127,9 +129,17 @@ class Server:
                 time.sleep(max(0.0, min(requests[order[k]][0] - now, 0.01)) if poll == 0.0 else poll)
                 continue
             self.tick()
-            for t in [t for t in pending if self.status(t) == abi.DEM_REQ_DONE]:
-                i = pending.pop(t)
-                _, lat[i] = self.take(t, requests[i][2])
+            for t in list(pending):
+                st_t = self.status(t)
+                if st_t == abi.DEM_REQ_DONE:
+                    i = pending.pop(t)
+                    _, lat[i] = self.take(t, requests[i][2])
+                elif st_t == abi.DEM_REQ_FAILED:
+                    i = pending.pop(t)
+                    try:
+                        self.take(t, requests[i][2])
+                    except abi.DemError as e:
+                        self.failures[t] = e
         return tickets, lat, time.perf_counter() - t0
 
     def close(self):
@@ -137,7 +147,18 @@ class Server:
             self.lib.dem_server_destroy(self.h)
             self.h = None
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
+        # No HIP calls while the interpreter shuts down: the runtime (and a profiler
+        # hooked into it) may already be torn down, and process exit frees the device
+        # memory anyway.  Callers close explicitly.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

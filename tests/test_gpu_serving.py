@@ -126,3 +126,55 @@ def test_oversized_scene_refused():
         srv.submit(s, st, 10)
     assert e.value.code == abi.DEM_ECAPACITY
     srv.close()
+
+
+def test_failed_admission_does_not_block_the_queue():
+    """A request the engine refuses at admission (particle 0 lists more contacts than a
+    particle can hold) sits between valid ones: it becomes DEM_REQ_FAILED, its take
+    returns the error, and the scenes around it are served and equal their solo runs."""
+    good = [scenes.granular_box(n_side=16, seed=50 + k, spacing=1.8) + (20 + 5 * k,) for k in range(2)]
+    s_bad, st0 = scenes.granular_box(n_side=16, seed=52, spacing=1.8)
+    st_bad = State(st0.capacity, 20)
+    for name in ("pos", "vel", "acc", "rot", "rot_vel", "rot_acc", "forces", "radii", "fixity", "material_pointers"):
+        setattr(st_bad, name, getattr(st0, name).copy())
+    st_bad.p_count = st0.p_count
+    c = st_bad.contacts.view(np.int32).reshape(-1, 20, 6)
+    c[0, :17, 0] = 0
+    c[0, :17, 1] = np.arange(1, 18)       # 17 plain contacts on particle 0
+    c[0, :17, 2] = 0
+    srv = Server(slot_capacity=4096, cell_capacity=1 << 14, scene_capacity=4, substeps=10, bypass=0)
+    outs = [good[0][1].copy(), st_bad.copy(), good[1][1].copy()]
+    t0 = srv.submit(good[0][0], outs[0], good[0][2])
+    tb = srv.submit(s_bad, outs[1], 10)
+    t1 = srv.submit(good[1][0], outs[2], good[1][2])
+    for _ in range(100):
+        if srv.status(t0) == abi.DEM_REQ_DONE and srv.status(t1) == abi.DEM_REQ_DONE:
+            break
+        srv.tick()
+    assert srv.status(tb) == abi.DEM_REQ_FAILED
+    assert srv.stats()["failed"] == 1
+    with pytest.raises(abi.DemError) as e:
+        srv.take(tb, outs[1])
+    assert e.value.code == abi.DEM_ECAPACITY and "admission" in str(e.value)
+    for t, out, (s, st, n) in ((t0, outs[0], good[0]), (t1, outs[2], good[1])):
+        got, _ = srv.take(t, out)
+        assert_same(got, solo(s, st, n), "scene around the failed one")
+    srv.close()
+
+
+def test_submit_refuses_what_admission_never_could():
+    """A grid finer than 2 r_max and more materials than the pool holds are refused at
+    submit (nothing queued), instead of blocking the queue."""
+    s, st = scenes.granular_box(n_side=8, seed=3)
+    srv = Server(slot_capacity=1024, cell_capacity=1 << 14, scene_capacity=1, substeps=4)
+    g = ComputeProg.default_grid(s, st)
+    g.cell_size = 1.0           # radii up to 1.0: cells must be wider than 2.0
+    with pytest.raises(abi.DemError) as e:
+        srv.submit(s, st, 5, grid=g)
+    assert e.value.code == abi.DEM_EINVAL
+    s.materials = s.materials * 9   # 9 materials; the pool holds 8 per scene
+    with pytest.raises(abi.DemError) as e:
+        srv.submit(s, st, 5)
+    assert e.value.code == abi.DEM_ECAPACITY
+    assert srv.stats()["queued"] == 0
+    srv.close()
