@@ -2,19 +2,28 @@
 
 Workload (default): the large polydisperse granular collapse - 4,194,304 discs,
 log-normal radii (median 0.35, clip [0.175, 0.525]; the stated radii scaled by
-0.35 so the discs fit the 4096 x 2048 box), hex column of spacing 1.05, gravity,
-contact law kn_eff = 1e5, mu = 0.5, dt = 1e-4.  A "step" is one full time step of
-every disc (LOM + Sim of the reference, here: the step kernel, plus the neighbour-list
-rebuild chain - binning, cell scan, scatter, list build - on the steps that need it).
+0.35 so the discs fit the 4096 x 2048 box), a packed column 2048 mean diameters wide
+deposited in contact on the floor (every disc resting on two others), gravity,
+spring-dashpot contacts kn_eff = 1e5, restitution 0.5, mu = 0.5, dt = 1e-4.  The
+column is advanced --settle steps untimed (past its first neighbour-list rebuild and
+re-sort), the block size and cell side are tuned on that state, then --warmup and
+--steps follow: the timed window lies in the first few hundred of the run's 5,000
+steps, while the pile still holds about three contacts per disc (it loses them as the
+soft pile falls and compacts).  A "step" is one full time step of every disc (LOM +
+Sim of the reference, here: the step kernel, plus the neighbour-list rebuild chain -
+binning, cell scan, scatter, list build - on the steps that need it, and the storage
+re-sort when due).
 
 Prints one JSON line.  `value` is particle-steps/s of the timed region (inputs
 resident in HBM, host clock around K steps bracketed by device syncs).
 `roofline` is for the dominant kernel (the fused step kernel): algorithmic bytes
 per launch = N * (60 + 16 * mean contacts per particle) over its mean duration
 from HIP events on the ctx's stream; `traffic` is the kernel's measured HBM bytes per
-fused launch (counter summary under profiles/, default workload only).  `cpu_baseline` times the CPU oracle's
-restatement of the reference algorithm (AABB bucket grid, 14-slot table) on a
-bounded sample of the same scene.
+fused launch (counter summary under profiles/, default workload only).  The window's
+mean contacts per disc, rebuilds and re-sorts are in `config.window`; a window with
+less than one contact per disc is flagged there.  `cpu_baseline` times the CPU
+oracle's restatement of the reference algorithm (AABB bucket grid, 14-slot table) on
+a bounded sample of the same scene.
 """
 import argparse
 import json
@@ -33,18 +42,19 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_step_plain_column_collapse_4
 
 def measured_traffic(args, cell_size: float):
     """Bytes per launch of the step kernel from the committed counter summary, for the
-    configuration that summary covers (4,194,304 discs, the block and cell side it
-    records); None otherwise."""
+    configuration that summary covers (4,194,304 discs, the settle, block and cell
+    side it records); None otherwise."""
     if not (args.particles == 4194304 and args.per_row == 2048 and args.collision_interval == 1):
         return None
     if not os.path.exists(TRAFFIC_PROFILE):
         return None
     meta = {}
     for line in open(TRAFFIC_PROFILE):
-        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "block", "cell_size"):
+        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "block", "cell_size", "settle"):
             k, v = line.split(":", 1)
             meta[k] = float(v)
-    if meta.get("block") != args.block or abs(meta.get("cell_size", -1.0) - cell_size) > 1e-6:
+    if meta.get("block") != args.block or abs(meta.get("cell_size", -1.0) - cell_size) > 1e-6 or \
+            meta.get("settle") != args.settle:
         return None
     return meta.get("traffic_bytes_per_launch")
 
@@ -54,6 +64,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=256)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--settle", type=int, default=100,
+                    help="untimed steps of the collapse before tuning, warm-up and timing")
     ap.add_argument("--particles", type=int, default=4194304)
     ap.add_argument("--per-row", type=int, default=2048)
     ap.add_argument("--cell", type=float, default=0.0, help="grid cell side (0: just above 2 r_max)")
@@ -242,15 +254,26 @@ def graph_bench(args):
     return out
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(args):
-    """Oracle (reference algorithm, f32) on a bounded sample: the first
-    `cpu_particles` discs of the same column generator, timed over enough steps
-    for about cpu_seconds of work on the host's cores."""
+    """Oracle (reference algorithm, f32) on a bounded sample: a `cpu_particles`-disc
+    column of the same generator and width (the bottom layers of the full pile, in
+    contact the same way), timed over enough steps for about cpu_seconds of work on up
+    to 16 host cores, and on one core for about a third of that."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ctypes as C
     import numpy as np
     import orc
-    from psim import abi, scenes
+    from psim import scenes
     from psim.compute import ComputeProg
     n = args.cpu_particles
     s, st = scenes.column_collapse(n=n, per_row=args.per_row)
@@ -258,19 +281,28 @@ def cpu_baseline(args):
     # reference grid semantics (AABB buckets with capacity) on the box-sized grid
     rmin, rmax = float(st.radii[:n].min()), float(st.radii[:n].max())
     grid.cell_cap = min(int(np.ceil((2 * rmax / rmin + 1.0) ** 2)), n) + 2
-    eng = orc.OracleEngine(st, s, mode=orc.MODE_REFERENCE, grid=grid)
-    cores = min(16, os.cpu_count() or 1)
     u = s.collision_settings()
     u.particles = n
-    v = eng._view()
     L = orc.lib()
-    # calibrate with 2 steps, then time a bounded number of steps
-    t = L.orc_time_reference(C.byref(u), eng.mats, len(eng.mats), C.byref(grid), C.byref(v), 2, cores)
-    steps = max(2, int(args.cpu_seconds / max(t / 2, 1e-6)))
-    t = L.orc_time_reference(C.byref(u), eng.mats, len(eng.mats), C.byref(grid), C.byref(v), steps, cores)
+
+    def timed(cores, seconds):
+        eng = orc.OracleEngine(st, s, mode=orc.MODE_REFERENCE, grid=grid)
+        v = eng._view()
+        args_ = (C.byref(u), eng.mats, len(eng.mats), C.byref(grid), C.byref(v))
+        # calibrate with 2 steps, then time a bounded number of steps from there
+        t = L.orc_time_reference(*args_, 2, cores, s.physics.restitution)
+        steps = max(2, int(seconds / max(t / 2, 1e-6)))
+        t = L.orc_time_reference(*args_, steps, cores, s.physics.restitution)
+        return steps, t
+    cores = min(16, os.cpu_count() or 1)
+    steps, t = timed(cores, args.cpu_seconds)
+    steps1, t1 = timed(1, args.cpu_seconds / 3.0)
     return {"value": n * steps / t, "unit": "particle-steps/s", "cores": cores, "kind": "port",
-            "sample": "CPU restatement of the reference algorithm (oracle, reference slot rules, f32, OpenMP) on "
-                      "the first %d discs of the same column, %d steps in %.1f s" % (n, steps, t)}
+            "value_1core": n * steps1 / t1, "cpu_model": cpu_model(),
+            "sample": "CPU restatement of the reference algorithm (oracle, reference slot rules, f32, OpenMP, "
+                      "contact dashpot e %.1f) on a %d-disc column of the same width, %d steps in %.1f s on %d "
+                      "cores (%d steps in %.1f s on 1 core)" % (s.physics.restitution, n, steps, t, cores, steps1,
+                                                                 t1)}
 
 
 def main():
@@ -285,6 +317,16 @@ def main():
 
     s, st = scenes.column_collapse(n=args.particles, per_row=args.per_row)
     s.physics.collision_interval = args.collision_interval
+    n = st.p_count
+    if args.settle:
+        # the collapse's first steps, untimed; results do not depend on the grid or block
+        pre = ComputeProg(s, st, record_data=False, reorder_interval=args.reorder)
+        pre.step(args.settle)
+        st = pre.update_state().copy()
+        c = pre.counters()
+        pre.close()
+        print("bench: settled %d steps, %.3f contacts/disc" % (args.settle, c["contacts"] / n), file=sys.stderr,
+              flush=True)
     tuned = None
     if not args.no_autotune:
         from psim.autotune import cell_size_for, tune
@@ -298,7 +340,6 @@ def main():
     prog = ComputeProg(s, st, grid=grid, record_data=False, reorder_interval=args.reorder, block_size=args.block)
     prog.sync()
     t_up = time.perf_counter() - t_up   # pool allocation + host layout conversion + PCIe upload
-    n = st.p_count
     print("bench: %d discs, grid %dx%d cell %.4f; warmup %d" % (n, grid.w, grid.h, grid.cell_size, args.warmup),
           file=sys.stderr, flush=True)
     if args.warmup:
@@ -323,14 +364,20 @@ def main():
     t_down = time.perf_counter()
     prog.update_state()                 # PCIe download + rebuild of the reference layouts
     t_down = time.perf_counter() - t_down
-    nc = 0.5 * (c1["contacts"] + c2["contacts"]) / n
+    nc = (c0["contacts"] + c1["contacts"] + c2["contacts"]) / 3.0 / n   # both passes' ends and start
     ms_step = 1e3 * wall / args.steps
     value = n * args.steps / wall
     cells = grid.w * grid.h
-    b_alg = 48 + 12 + 16 + 8.0 * cells / n + 16 * nc       # bytes per particle-step, whole step
+    rebuilds = c1["broad_phases"] - c0["broad_phases"]
+    resorts = c1["reorders"] - c0["reorders"]
+    # bytes per particle-step, whole step: the step kernel's 60 + 16 n_c every step, the
+    # binning's 16 + 8 C/N on the steps whose lists were rebuilt
+    b_alg = 48 + 12 + 16 * nc + (16 + 8.0 * cells / n) * rebuilds / args.steps
     k_bytes = n * (60 + 16 * nc)                            # dominant kernel, per launch
     k_ms = tm["step_ms"] / max(tm["n_step"], 1)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    if nc < 1.0:
+        print("bench: WARNING: the timed window holds %.3f contacts per disc (< 1)" % nc, file=sys.stderr, flush=True)
     out = {
         "metric": "particle-steps/s at 4M discs (%HBM peak); scenes/s + p99 latency batched vs plain",
         "value": value,
@@ -346,10 +393,14 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "granular column collapse, %d discs (log-normal r median 0.35 clip [0.175,0.525]), "
-                        "4096x2048 box, dt 1e-4" % n,
+                        "packed column deposited in contact, 4096x2048 box, spring-dashpot contacts (e 0.5), dt 1e-4, "
+                        "steps %d-%d of the run" % (n, args.settle + args.warmup, args.settle + args.warmup + args.steps),
             "grid": [grid.w, grid.h], "cell_size": grid.cell_size, "reorder_interval": args.reorder,
             "collision_interval": args.collision_interval,
             "block": args.block, "autotune": tuned, "mean_contacts_per_particle": nc,
+            "window": {"first_step": args.settle + args.warmup, "steps": args.steps,
+                       "mean_contacts_per_particle": nc, "rebuilds": rebuilds, "resorts": resorts,
+                       "contact_rich": nc >= 1.0},
             "hbm_frac_whole_step": value * b_alg / (HBM_PEAK_GBS * 1e9),
             "bytes_per_particle_step_alg": b_alg,
             "upload_ms": 1e3 * t_up, "download_ms": 1e3 * t_down,

@@ -104,6 +104,11 @@ typedef struct dem_params {
     int32_t graph;             /* 1: capture each dem_step call's launches in a HIP graph and replay
                                   it when a later call repeats the same launches (same step count,
                                   buffers, rebuild and re-sort pattern); 0: launch directly */
+    float   restitution;       /* extension (the reference's contact law has no dashpot): e in (0, 1)
+                                  adds a normal dashpot 2 zeta sqrt(m_eff kn) v_n to every contact,
+                                  zeta = -ln e / sqrt(pi^2 + ln^2 e), so that a binary collision
+                                  rebounds with restitution e; 0 (or >= 1): no dashpot, the
+                                  reference's law */
 } dem_params;
 
 /* Caller-owned arrays in the reference host layouts (state.rs:69-93).

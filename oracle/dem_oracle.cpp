@@ -102,6 +102,58 @@ typedef struct orc_state {
 
 namespace {
 
+// Every overlapping pair (the narrow-phase predicate of 2D_Simulation.wgsl:159,
+// sqrt(dx*dx + dy*dy) < r_b + r_a) among the first n discs with r != 0 and finite
+// positions, found over a cell list of side >= 2 r_max built here (3x3 stencil):
+// found[a] = ascending partner ids.  O(N); the same set as the O(N^2) brute force.
+void cell_list_pairs(int n, const float* pos, const float* radii, std::vector<std::vector<int32_t>>& found) {
+    found.assign(n, {});
+    auto px = [&](int i) { return pos[2 * i]; };
+    auto py = [&](int i) { return pos[2 * i + 1]; };
+    float rmax = 0.0f, xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    std::vector<int> act;
+    for (int i = 0; i < n; i++) {
+        if (radii[i] == 0.0f) continue;
+        if (!(px(i) == px(i)) || !(py(i) == py(i))) continue;
+        act.push_back(i);
+        rmax = std::max(rmax, radii[i]);
+        xmin = std::min(xmin, px(i)); xmax = std::max(xmax, px(i));
+        ymin = std::min(ymin, py(i)); ymax = std::max(ymax, py(i));
+    }
+    if (act.empty()) return;
+    double cs = std::max(2.0 * rmax, 1e-30);
+    int64_t gw = std::min<int64_t>(std::max<int64_t>((int64_t)((xmax - xmin) / cs) + 1, 1), 1 << 14);
+    int64_t gh = std::min<int64_t>(std::max<int64_t>((int64_t)((ymax - ymin) / cs) + 1, 1), 1 << 14);
+    cs = std::max(cs, std::max((double)(xmax - xmin) / (double)gw, (double)(ymax - ymin) / (double)gh) * 1.000001);
+    auto cell_of = [&](int i, int64_t& cx, int64_t& cy) {
+        cx = std::min<int64_t>(std::max<int64_t>((int64_t)(((double)px(i) - xmin) / cs), 0), gw - 1);
+        cy = std::min<int64_t>(std::max<int64_t>((int64_t)(((double)py(i) - ymin) / cs), 0), gh - 1);
+    };
+    std::vector<int64_t> start(gw * gh + 1, 0);
+    std::vector<int32_t> order(act.size());
+    for (int i : act) { int64_t cx, cy; cell_of(i, cx, cy); start[cy * gw + cx + 1]++; }
+    for (int64_t c = 0; c < gw * gh; c++) start[c + 1] += start[c];
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (int i : act) { int64_t cx, cy; cell_of(i, cx, cy); order[fill[cy * gw + cx]++] = i; }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (size_t k = 0; k < act.size(); k++) {
+        int a = act[k];
+        int64_t cx, cy;
+        cell_of(a, cx, cy);
+        std::vector<int32_t>& out = found[a];
+        for (int64_t yy = std::max<int64_t>(cy - 1, 0); yy <= std::min<int64_t>(cy + 1, gh - 1); yy++)
+            for (int64_t xx = std::max<int64_t>(cx - 1, 0); xx <= std::min<int64_t>(cx + 1, gw - 1); xx++) {
+                int64_t c = yy * gw + xx;
+                for (int64_t e = start[c]; e < start[c + 1]; e++) {
+                    int b = order[e];
+                    if (b == a) continue;
+                    if (lenf(px(b) - px(a), py(b) - py(a)) < (radii[b] + radii[a])) out.push_back(b);
+                }
+            }
+        std::sort(out.begin(), out.end());
+    }
+}
+
 struct Ctx {
     const dem_uniform& u;
     const dem_material* mats;
@@ -170,6 +222,9 @@ struct Ctx {
     }
 
     // linear_model, 2D_Simulation.wgsl:338-373
+    // With cn2 != 0: the normal dashpot of the configurations' spring-dashpot law, an
+    // extension (the reference has none): 2 zeta sqrt(m_eff kn) times the approach speed,
+    // the normal component of del_pos[b] - del_pos[a] over dT.
     V3 linear_model(int a, int b, Slot& sl, int32_t bonded) {
         float nd = wmin(0.0f, distance2(a, b, bonded));
         if (nd == 0.0f) return {0.0f, 0.0f, 0.0f};
@@ -181,6 +236,15 @@ struct Ctx {
         float dx = px(a) - px(b), dy = py(a) - py(b);
         float l = lenf(dx, dy);
         float nx = dx / l, ny = dy / l;
+        if (cn2 != 0.0f) {
+            float m_a = ma.density * PI_F * r(a) * r(a);
+            float m_b = mb.density * PI_F * r(b) * r(b);
+            float me = m_a * m_b / (m_a + m_b);
+            float cn = cn2 * std::sqrt(me * kn);
+            float ax = s.del_pos[2 * b] - s.del_pos[2 * a], ay = s.del_pos[2 * b + 1] - s.del_pos[2 * a + 1];
+            float va = (ax * nx + ay * ny) / u.dT;
+            nf = nf + cn * va;
+        }
         float tx = -ny, ty = nx;
         float rt = rel_tangent(a, b, tx, ty);
         float lim = std::fabs(nf) * u.friction_coefficient;
@@ -519,52 +583,7 @@ struct Ctx {
     // ---- Sim pass, canonical slot order (the HIP path's semantics) ----
     std::vector<std::vector<int32_t>> found;  // per particle, ascending partner ids
 
-    void find_all_pairs() {
-        const int n = u.particles;
-        found.assign(n, {});
-        float rmax = 0.0f, xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
-        std::vector<int> act;
-        for (int i = 0; i < n; i++) {
-            if (r(i) == 0.0f) continue;
-            if (!(px(i) == px(i)) || !(py(i) == py(i))) continue;
-            act.push_back(i);
-            rmax = std::max(rmax, r(i));
-            xmin = std::min(xmin, px(i)); xmax = std::max(xmax, px(i));
-            ymin = std::min(ymin, py(i)); ymax = std::max(ymax, py(i));
-        }
-        if (act.empty()) return;
-        double cs = std::max(2.0 * rmax, 1e-30);
-        int64_t gw = std::min<int64_t>(std::max<int64_t>((int64_t)((xmax - xmin) / cs) + 1, 1), 1 << 14);
-        int64_t gh = std::min<int64_t>(std::max<int64_t>((int64_t)((ymax - ymin) / cs) + 1, 1), 1 << 14);
-        cs = std::max(cs, std::max((double)(xmax - xmin) / (double)gw, (double)(ymax - ymin) / (double)gh) * 1.000001);
-        auto cell_of = [&](int i, int64_t& cx, int64_t& cy) {
-            cx = std::min<int64_t>(std::max<int64_t>((int64_t)(((double)px(i) - xmin) / cs), 0), gw - 1);
-            cy = std::min<int64_t>(std::max<int64_t>((int64_t)(((double)py(i) - ymin) / cs), 0), gh - 1);
-        };
-        std::vector<int64_t> start(gw * gh + 1, 0);
-        std::vector<int32_t> order(act.size());
-        for (int i : act) { int64_t cx, cy; cell_of(i, cx, cy); start[cy * gw + cx + 1]++; }
-        for (int64_t c = 0; c < gw * gh; c++) start[c + 1] += start[c];
-        std::vector<int64_t> fill(start.begin(), start.end() - 1);
-        for (int i : act) { int64_t cx, cy; cell_of(i, cx, cy); order[fill[cy * gw + cx]++] = i; }
-#pragma omp parallel for schedule(dynamic, 256)
-        for (size_t k = 0; k < act.size(); k++) {
-            int a = act[k];
-            int64_t cx, cy;
-            cell_of(a, cx, cy);
-            std::vector<int32_t>& out = found[a];
-            for (int64_t yy = std::max<int64_t>(cy - 1, 0); yy <= std::min<int64_t>(cy + 1, gh - 1); yy++)
-                for (int64_t xx = std::max<int64_t>(cx - 1, 0); xx <= std::min<int64_t>(cx + 1, gw - 1); xx++) {
-                    int64_t c = yy * gw + xx;
-                    for (int64_t e = start[c]; e < start[c + 1]; e++) {
-                        int b = order[e];
-                        if (b == a) continue;
-                        if (lenf(px(b) - px(a), py(b) - py(a)) < (r(b) + r(a))) out.push_back(b);
-                    }
-                }
-            std::sort(out.begin(), out.end());
-        }
-    }
+    void find_all_pairs() { cell_list_pairs(u.particles, s.pos, s.radii, found); }
 
     void sim_canonical(int id) {
         s.data[DATA_SIZE * id + 0] = 0.0f;
@@ -616,6 +635,8 @@ struct Ctx {
         }
     }
 
+    float cn2 = 0.0f;   // dashpot factor 2 zeta (0: the reference's law)
+
     static void canonicalize(Slot* tab, int K) {
         std::stable_sort(tab, tab + K, [](const Slot& x, const Slot& y) {
             bool ex = x.b == -1, ey = y.b == -1;
@@ -641,6 +662,14 @@ struct Ctx {
     }
 };
 
+// 2 zeta for restitution e (zeta = -ln e / sqrt(pi^2 + ln^2 e), f64, rounded once); 0
+// outside (0, 1): the linear spring-dashpot whose binary collisions rebound at e.
+float dashpot_factor(float e) {
+    if (!(e > 0.0f && e < 1.0f)) return 0.0f;
+    const double le = std::log((double)e);
+    return (float)(2.0 * (-le / std::sqrt(M_PI * M_PI + le * le)));
+}
+
 }  // namespace
 
 extern "C" {
@@ -648,12 +677,14 @@ extern "C" {
 enum { ORC_MODE_REFERENCE = 0, ORC_MODE_CANONICAL = 1 };
 
 // Runs `nsteps` time steps on the arrays in `s` (in place).
+// restitution: the dashpot extension (0: none, the reference's law).
 int orc_run(const dem_uniform* u, const dem_material* mats, int32_t n_mats, const dem_grid* g,
-            orc_state* s, int32_t mode, int32_t use_f64, int32_t nsteps) {
+            orc_state* s, int32_t mode, int32_t use_f64, int32_t nsteps, float restitution) {
     if (!u || !mats || n_mats <= 0 || !g || !s || nsteps < 0) return -1;
     if (u->particles > s->n || u->particles < 0) return -1;
     if (mode == ORC_MODE_REFERENCE && (!s->grid || s->slots < 1)) return -1;
     Ctx c{*u, mats, n_mats, *g, *s, use_f64 != 0, mode == ORC_MODE_CANONICAL, {}, {}};
+    c.cn2 = dashpot_factor(restitution);
     if (c.canonical)
         for (int i = 0; i < u->particles; i++) Ctx::canonicalize(c.slot(i * s->slots), s->slots);
     for (int k = 0; k < nsteps; k++) c.step();
@@ -750,6 +781,24 @@ int64_t orc_contact_pairs(int32_t n, const float* pos, const float* radii, int32
     return over ? -1 : total;
 }
 
+// Stage 4 on a cell list: the pair set of orc_contact_pairs in O(N) (cell_list_pairs),
+// for inputs too large for the brute force.  Same outputs and return value.
+int64_t orc_contact_pairs_grid(int32_t n, const float* pos, const float* radii, int32_t max_per, int32_t* counts,
+                               int32_t* partners) {
+    std::vector<std::vector<int32_t>> found;
+    cell_list_pairs(n, pos, radii, found);
+    int64_t total = 0;
+    bool over = false;
+    for (int a = 0; a < n; a++) {
+        const int c = (int)found[a].size();
+        for (int k = 0; k < max_per; k++) partners[(int64_t)a * max_per + k] = k < c ? found[a][k] : -1;
+        if (c > max_per) over = true;
+        counts[a] = c;
+        total += c;
+    }
+    return over ? -1 : total;
+}
+
 // State::regen_bonds restated (state.rs:291-367).  O(N^2), in (i, j) order.
 // bonds_out: 3 words per bond; contacts_out: slots*6 words per particle (all -1
 // words on entry is not required: the function initialises them).  Returns the
@@ -817,8 +866,9 @@ double orc_kinetic_energy(int32_t n, const float* vel, const float* rot_vel, con
 // concurrency: LOM drift in parallel, then the AABB binning with atomic cell
 // counters, then the Sim pass in parallel.  Returns seconds.
 double orc_time_reference(const dem_uniform* u, const dem_material* mats, int32_t n_mats, const dem_grid* g,
-                          orc_state* s, int32_t nsteps, int32_t threads) {
+                          orc_state* s, int32_t nsteps, int32_t threads, float restitution) {
     Ctx c{*u, mats, n_mats, *g, *s, false, false, {}, {}};
+    c.cn2 = dashpot_factor(restitution);
     if (threads <= 1) {
         auto t0 = std::chrono::steady_clock::now();
         for (int k = 0; k < nsteps; k++) c.step();
@@ -831,6 +881,7 @@ double orc_time_reference(const dem_uniform* u, const dem_material* mats, int32_
     for (int k = 0; k < nsteps; k++) {
         // LOM: invocation 0 first (it owns the coll_cont counters), the rest in parallel.
         Ctx c0{*u, mats, n_mats, *g, *s, false, true, {}, {}};  // canonical flag: skip binning inside lom()
+        c0.cn2 = c.cn2;
         if (n > 0) c0.lom(0);
 #pragma omp parallel for schedule(static)
         for (int id = 1; id < n; id++) c0.lom(id);

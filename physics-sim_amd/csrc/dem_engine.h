@@ -25,6 +25,8 @@ int validate_params(const dem_params& p);
 // Whether the grid's cells are wide enough for the 3x3 stencil: 2 rsmax + the rounding
 // margin of the cell index (4 ulp of the grid extent).
 int check_cell_size(const dem_grid& g, float rsmax);
+// 2 zeta of the contact dashpot for restitution e (dem_params.restitution); 0: none.
+float dashpot_factor(float e);
 
 struct DevSet {
     float4* P[2] = {nullptr, nullptr};    // by position parity (cur_)

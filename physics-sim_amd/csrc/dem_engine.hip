@@ -437,6 +437,14 @@ float Engine::skin_of(const SceneHost& sh) const {
     return std::max(cs - need, 0.0f);
 }
 
+// 2 zeta of a linear dashpot whose binary collisions rebound with restitution e:
+// zeta = -ln e / sqrt(pi^2 + ln^2 e), evaluated in f64 and rounded once; 0 outside (0, 1).
+float dashpot_factor(float e) {
+    if (!(e > 0.0f && e < 1.0f)) return 0.0f;
+    const double le = std::log((double)e);
+    return (float)(2.0 * (-le / std::sqrt(M_PI * M_PI + le * le)));
+}
+
 SceneDev Engine::scene_dev(const SceneHost& sh, int32_t sid) const {
     SceneDev d{};
     d.u = sh.params.u;
@@ -466,6 +474,7 @@ SceneDev Engine::scene_dev(const SceneHost& sh, int32_t sid) const {
     d.rsmax = sh.rsmax;
     d.lb0 = sid * LB_MAX;
     d.nlb = (int32_t)sh.large.size();
+    d.cn2 = dashpot_factor(sh.params.restitution);
     if (!sh.mats.empty()) {
         d.rho1 = sh.mats[0].density;
         d.kn1 = sh.mats[0].normal_stiffness;

@@ -80,6 +80,9 @@ __device__ __forceinline__ int cell_of(const SceneDev& S, float x, float y, int&
 struct Self {
     float x, y, r, drot, dx, dy;
     float kn, ks;
+    float rho;      // density (mass of the dashpot)
+    float cn2;      // dashpot factor 2 zeta (0: no dashpot, the reference's law)
+    float dt;
 };
 
 __device__ __forceinline__ uint32_t vword(const float4& v) { return __float_as_uint(v.w); }
@@ -94,13 +97,13 @@ __device__ __forceinline__ float4 del_from(const DevArrays& a, int j, const floa
 }
 __device__ __forceinline__ float4 del_of(const DevArrays& a, int j, float dt) { return del_from(a, j, a.V[j], dt); }
 
-// Normal and shear stiffness of the partner in slot j: a one-material scene takes them
-// from the scene table (the record of material 0), without a load.
-__device__ __forceinline__ float2 partner_k(const DevArrays& a, const SceneDev& S, int j) {
-    if (S.nm <= 1) return make_float2(S.kn1, S.ks1);
+// Normal and shear stiffness and density of the partner in slot j: a one-material scene
+// takes them from the scene table (the record of material 0), without a load.
+__device__ __forceinline__ float3 partner_k(const DevArrays& a, const SceneDev& S, int j) {
+    if (S.nm <= 1) return make_float3(S.kn1, S.ks1, S.rho1);
     const int mb = a.M[j] & PF_MAT_MASK;
     const dem_material MB = a.MAT[S.m0 + (mb < S.nm ? mb : 0)];
-    return make_float2(MB.normal_stiffness, MB.shear_stiffness);
+    return make_float3(MB.normal_stiffness, MB.shear_stiffness, MB.density);
 }
 
 // Carries the velocity record (and an explicit displacement) of slot s over to the
@@ -176,18 +179,30 @@ __device__ __forceinline__ float rel_tangent(const Self& a, const float4& pb, co
     return (rtx * tx + rty * ty) + rr;
 }
 
-// linear_model, :338-373.  L = r_a + r_b (contact) or the bond length.
+// linear_model, :338-373.  L = r_a + r_b (contact) or the bond length.  kb = the
+// partner's {kn, ks, density}.  With a.cn2 != 0 the normal force carries a dashpot
+// (an extension; the reference's law has none): + 2 zeta sqrt(m_eff kn) times the
+// approach speed along the normal, the relative displacement of the last drift over dT.
 template <bool F64, class DD>
-__device__ __forceinline__ float3 linear_model(float mu, const Self& a, const float4& pb, const float4& db, float kn_b,
-                                               float ks_b, float L, float& ft, DD& dd) {
+__device__ __forceinline__ float3 linear_model(float mu, const Self& a, const float4& pb, const float4& db,
+                                               const float3& kb, float L, float& ft, DD& dd) {
     float nd = fminf(0.0f, dist2<F64>(a, pb.x, pb.y, L));
     if (nd == 0.0f) return make_float3(0.0f, 0.0f, 0.0f);
-    float kn = 1.0f / (1.0f / a.kn + 1.0f / kn_b);
-    float ks = 1.0f / (1.0f / a.ks + 1.0f / ks_b);
+    float kn = 1.0f / (1.0f / a.kn + 1.0f / kb.x);
+    float ks = 1.0f / (1.0f / a.ks + 1.0f / kb.y);
     float nf = -nd * kn;
     float dx = a.x - pb.x, dy = a.y - pb.y;
     float l = lenf(dx, dy);
     float nx = dx / l, ny = dy / l;
+    if (a.cn2 != 0.0f) {
+        float ma = a.rho * PI_F * a.r * a.r;
+        float mb = kb.z * PI_F * pb.z * pb.z;
+        float me = ma * mb / (ma + mb);
+        float cn = a.cn2 * sqrtf(me * kn);
+        float ax = db.x - a.dx, ay = db.y - a.dy;
+        float va = (ax * nx + ay * ny) / a.dt;
+        nf = nf + cn * va;
+    }
     float tx = -ny, ty = nx;
     float rt = rel_tangent(a, pb, db, tx, ty);
     float lim = fabsf(nf) * mu;
@@ -206,8 +221,7 @@ struct BondCtx {
 // Sets `code` to the break code when the bond tears.
 template <bool F64, class DD>
 __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, const float4& pb, const float4& db,
-                                           float kn_b, float ks_b, const BondCtx& bc, float4& bf, DD& dd,
-                                           int& code) {
+                                           const float3& kb, const BondCtx& bc, float4& bf, DD& dd, int& code) {
     code = 0;
     if (u.bonds == 3) {
         float kn = u.bond_normal_stiffness;
@@ -238,7 +252,7 @@ __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, 
             m = 0.0f;
         }
         dd.add(-nf, bf.y, m);
-        float3 lm = linear_model<F64>(u.friction_coefficient, a, pb, db, kn_b, ks_b, bc.L, bf.x, dd);
+        float3 lm = linear_model<F64>(u.friction_coefficient, a, pb, db, kb, bc.L, bf.x, dd);
         return make_float3(fx + lm.x, fy + lm.y, m + lm.z);
     } else if (u.bonds == 2) {
         float nd = dist1<F64>(a, pb);
@@ -275,7 +289,7 @@ __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, 
             fx = 0.0f;
             fy = 0.0f;
         }
-        float3 lm = linear_model<F64>(u.friction_coefficient, a, pb, db, kn_b, ks_b, bc.L, bf.x, dd);
+        float3 lm = linear_model<F64>(u.friction_coefficient, a, pb, db, kb, bc.L, bf.x, dd);
         return make_float3(lm.x + fx, lm.y + fy, lm.z + 0.0f);
     }
     return make_float3(0.0f, 0.0f, 0.0f);
@@ -452,10 +466,10 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
         const uint32_t bw = __float_as_uint(bl.w);
         if (brk_before(bw, (uint32_t)step)) bonded = (bw & 3u) ? (int)(bw & 3u) - 12 : a.BS[S.b0 + bid].x;
     }
-    const float2 kb = partner_k(a, S, j);
+    const float3 kb = partner_k(a, S, j);
     float3 f;
     if (bonded < 0 || u.bonds == 0) {
-        f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb.x, kb.y, me.r + pb.z, bf.x, dd);
+        f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb, me.r + pb.z, bf.x, dd);
     } else if (u.bonds >= 1 && u.bonds <= 3) {
         BondCtx bc;
         bc.L = bl.x;
@@ -467,7 +481,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
             bc.ss = u.bond_shear_strength;
         }
         int code = 0;
-        f = bond_law<F64>(u, me, pb, db, kb.x, kb.y, bc, bf, dd, code);
+        f = bond_law<F64>(u, me, pb, db, kb, bc, bf, dd, code);
         if (code < 0) {
             bonded = code;
             if (bid_ok) {
@@ -727,6 +741,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         Self me;
         me.x = p.x; me.y = p.y; me.r = p.z; me.drot = dme.z; me.dx = dme.x; me.dy = dme.y;
         me.kn = ma_kn; me.ks = ma_ks;
+        me.rho = ma_rho; me.cn2 = S.cn2; me.dt = dt;
         Data3 dd{0.0f, 0.0f, 0.0f};
         float nfx = 0.0f, nfy = 0.0f, nm = 0.0f;
         const int oc = min(cc_old, KMAX);
@@ -776,8 +791,8 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                     if (m < oc && co[m] == ob) ft = cf[m];
                 const float4 pb = a.P[cur][j];
                 const float4 db = del_of(a, j, dt);
-                const float2 kb = partner_k(a, S, j);
-                const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb.x, kb.y, me.r + pb.z, ft, dd);
+                const float3 kb = partner_k(a, S, j);
+                const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb, me.r + pb.z, ft, dd);
                 nfx += f.x;
                 nfy += f.y;
                 nm += f.z;
@@ -806,8 +821,8 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             const int j = a.INV[S.p0 + ob];
             const float4 pb = a.P[cur][j];
             const float4 db = del_of(a, j, dt);
-            const float2 kb = partner_k(a, S, j);
-            const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb.x, kb.y, me.r + pb.z, ft, dd);
+            const float3 kb = partner_k(a, S, j);
+            const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb, me.r + pb.z, ft, dd);
             nfx += f.x;
             nfy += f.y;
             nm += f.z;
@@ -1002,6 +1017,7 @@ __device__ __forceinline__ void bond_pass_body(const StepArgs& args, const Scene
     Self me;
     me.x = p.x; me.y = p.y; me.r = p.z; me.drot = d.z; me.dx = d.x; me.dy = d.y;
     me.kn = MA.normal_stiffness; me.ks = MA.shear_stiffness;
+    me.rho = MA.density; me.cn2 = S.cn2; me.dt = S.u.dT;
     const uint32_t fl = (uint32_t)lfw.x;
     const bool narrow = (fl & SF_SIM) && (fl & SF_NARROW);
     DataLog dl;

@@ -74,6 +74,7 @@ dem_params ComputeProg::params(const Settings& s) {
     p.reorder_interval = opt_.reorder_interval;
     p.block_size = opt_.block_size;
     p.graph = opt_.graph ? 1 : 0;
+    p.restitution = s.physics.restitution;
     return p;
 }
 

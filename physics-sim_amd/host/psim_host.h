@@ -95,6 +95,9 @@ struct PhysicsSettings {       // settings.rs:176-197, defaults :336-357
     float moment_contribution_factor = 1.0f;
     bool local_damping = false;
     float local_damping_alpha = 0.1f;
+    // extension, not saved: restitution of the contact dashpot (dem_params.restitution);
+    // 0 = none, the reference's contact law
+    float restitution = 0.0f;
 };
 
 struct Settings {

@@ -48,7 +48,8 @@ class DemGrid(C.Structure):
 class DemParams(C.Structure):
     _fields_ = [("u", DemUniform), ("materials", C.POINTER(DemMaterial)), ("n_materials", C.c_int32),
                 ("grid", DemGrid), ("use_f64", C.c_int32), ("record_data", C.c_int32),
-                ("reorder_interval", C.c_int32), ("block_size", C.c_int32), ("graph", C.c_int32)]
+                ("reorder_interval", C.c_int32), ("block_size", C.c_int32), ("graph", C.c_int32),
+                ("restitution", C.c_float)]
 
 
 FP = C.POINTER(C.c_float)

@@ -85,6 +85,7 @@ class ComputeProg:
         p.reorder_interval = int(self.reorder_interval)
         p.block_size = int(self.block_size)
         p.graph = int(self.graph)
+        p.restitution = float(settings.physics.restitution)
         return p
 
     def set_params(self, settings: Settings = None):

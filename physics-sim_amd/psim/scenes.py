@@ -10,13 +10,15 @@ same state.
                    spacing 2.0 filling a 128 x 128 walled box, gravity -9.81,
                    effective kn = 1e5 (material kn = ks = 2e5: kn_eff is the series
                    combination 1/(1/kn_a + 1/kn_b), 2D_Simulation.wgsl:345-346),
-                   mu = 0.5, density 1, dt = 1e-4.
+                   mu = 0.5, density 1, dt = 1e-4, contact restitution 0.5 (the
+                   dashpot extension; 0 gives the reference's undamped law).
   column_collapse  polydisperse column collapsing into a wide box; the full size is
                    4,194,304 discs in a 4096 x 2048 box.  Radii are log-normal
                    (median 0.35, sigma 0.25, clipped to [0.175, 0.525]): the stated
                    radii (median 1.0, clip [0.5, 1.5]) scaled by 0.35, because at
                    full size 4.19M discs of median radius 1 cover 1.75 times the
-                   box.  Hex column of spacing 2 r_max = 1.05, 2048 discs per row.
+                   box.  A packed pile 2048 mean diameters wide, deposited disc by
+                   disc (each resting on two discs below it), centred on the floor.
   bonded_chains    horizontal chains of r = 0.5 discs at spacing 1.0, neighbours
                    bonded (parallel bonds), ends pinned, hit by a dense r = 8 disc.
   bonded_assembly  dense packing of r ~ U[0.8, 1.2] discs bonded where the gap is
@@ -60,12 +62,14 @@ def _new_state(n, max_contacts=14) -> State:
 
 
 def granular_box(n_side: int = 64, seed: int = 0, spacing: float = 2.0, rmin: float = 0.5, rmax: float = 1.0,
-                 dt: float = 1e-4, mu: float = 0.5):
-    """Plain granular box settle (4,096 discs at n_side = 64)."""
+                 dt: float = 1e-4, mu: float = 0.5, restitution: float = 0.5):
+    """Plain granular box settle (4,096 discs at n_side = 64); contacts with the
+    spring-dashpot law at `restitution` (0: the reference's undamped law)."""
     rng = np.random.default_rng(seed)
     n = n_side * n_side
     box = n_side * spacing
     s = _base_settings(n, box, box, dt, rmax, rmin, mu=mu)
+    s.physics.restitution = float(restitution)
     st = _new_state(n)
     r = rng.uniform(rmin, rmax, n).astype(np.float32)
     ii, jj = np.meshgrid(np.arange(n_side), np.arange(n_side), indexing="xy")
@@ -80,21 +84,85 @@ def granular_box(n_side: int = 64, seed: int = 0, spacing: float = 2.0, rmin: fl
     return s, st
 
 
+_PILE = None
+
+
+def _pile_lib():
+    global _PILE
+    if _PILE is None:
+        import ctypes as C
+        import os
+        path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "host", "libpsim_pile.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libpsim_pile.so not built (%s); run __graft_entry__.build()" % path)
+        L = C.CDLL(path)
+        D = C.POINTER(C.c_double)
+        L.psim_deposit_pile.argtypes = [C.c_int32, C.POINTER(C.c_float), C.c_double, C.c_double, C.c_double,
+                                        C.c_double, D, C.c_uint64, D, D]
+        _PILE = L
+    return _PILE
+
+
+def deposit_pile(radii: np.ndarray, left: float, width: float, floor: float, overlap: float = 5e-4,
+                 seed: int = 0, load: float = 0.0):
+    """A packed pile (drop-and-roll deposition, host/pile.cpp): disc after disc, each
+    dropped on a line that sweeps the column [left, left + width] in layers and placed
+    at the lowest spot near that line where it rests on two discs (or a disc and the
+    floor), overlapping them by max(overlap (r_i + r_j), d_i) and nothing more.
+    With load = 0, d_i = 0.  Otherwise the pile is deposited twice: the first pass gives
+    each disc's depth z below the pile's surface, and in the second a disc is pressed
+    into its two supports by the force that depth puts on them, d_i = load * z * r_i
+    (load = 1.155 phi rho g / kn for a bulk fraction phi under gravity g; 1.155 =
+    1 / cos 30 deg for two supports), so that the pile starts near its static
+    compression.  Returns f64 (x, y)."""
+    import ctypes as C
+    r = np.ascontiguousarray(radii, dtype=np.float32)
+    n = len(r)
+    D = C.POINTER(C.c_double)
+
+    def run(delta):
+        x = np.empty(n, dtype=np.float64)
+        y = np.empty(n, dtype=np.float64)
+        rc = _pile_lib().psim_deposit_pile(n, r.ctypes.data_as(C.POINTER(C.c_float)), left, width, floor, overlap,
+                                           None if delta is None else delta.ctypes.data_as(D), seed,
+                                           x.ctypes.data_as(D), y.ctypes.data_as(D))
+        if rc != 0:
+            raise ValueError("deposit_pile: bad arguments")
+        return x, y
+    x, y = run(None)
+    if load > 0.0 and n:
+        # surface height over bins of four diameters, then each disc's depth below it
+        bw = 8.0 * float(r.max())
+        b = np.minimum(((x - left) / bw).astype(np.int64), int(width / bw))
+        top = np.full(int(width / bw) + 1, -np.inf)
+        np.maximum.at(top, b, y + r)
+        depth = np.maximum(top[b] - y, 0.0)
+        x, y = run(np.ascontiguousarray(load * depth * r, dtype=np.float64))
+    return x, y
+
+
 def column_collapse(n: int = 4194304, per_row: int = 2048, seed: int = 2, width: float = 4096.0,
-                    height: float = 2048.0, dt: float = 1e-4, mu: float = 0.5):
-    """Large polydisperse granular collapse (4,194,304 discs at the default size)."""
+                    height: float = 2048.0, dt: float = 1e-4, mu: float = 0.5, overlap: float = 5e-4,
+                    restitution: float = 0.5):
+    """Large polydisperse granular collapse (4,194,304 discs at the default size): a
+    packed column as wide as `per_row` mean diameters, deposited on the floor of the
+    box and centred in it (deposit_pile: every disc resting on two others, pressed into
+    them by the weight above it, so that the pile starts in contact near its static
+    compression); its flanks are vertical, so it collapses sideways under gravity.  Radii log-normal,
+    median 0.35, sigma 0.25, clipped to [0.175, 0.525]: the stated radii scaled by 0.35,
+    since 4.19M discs of median radius 1 cover 1.75 times the 4096 x 2048 box.  The
+    contact law of the granular box: spring-dashpot at `restitution`."""
     rng = np.random.default_rng(seed)
     rmed, sigma, lo, hi = 0.35, 0.25, 0.175, 0.525
     r = np.clip(rmed * np.exp(sigma * rng.standard_normal(n)), lo, hi).astype(np.float32)
-    spacing = 2 * hi
-    rows = int(math.ceil(n / per_row))
-    k = np.arange(n)
-    row, col = k // per_row, k % per_row
-    x = -width / 2 + hi + col * spacing + (row % 2) * (spacing / 2)
-    y = -height / 2 + hi + row * spacing * math.sqrt(3) / 2
+    w = 2.0 * float(np.mean(r)) * per_row
+    if w + 2 * hi > width:
+        raise ValueError("column wider than the box: %.1f" % w)
+    x, y = deposit_pile(r, -w / 2.0, w, -height / 2.0, overlap, seed)
     if y.max() + hi > height / 2:
-        raise ValueError("column taller than the box: %d rows" % rows)
+        raise ValueError("column taller than the box: top %.1f" % (y.max() + hi))
     s = _base_settings(n, width, height, dt, hi, lo, mu=mu)
+    s.physics.restitution = float(restitution)
     st = _new_state(n)
     st.pos[0::2][:n] = x.astype(np.float32)
     st.pos[1::2][:n] = y.astype(np.float32)

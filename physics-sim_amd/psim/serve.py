@@ -37,6 +37,7 @@ def scene_params(settings: Settings, state: State, grid: Optional[abi.DemGrid] =
     p.record_data = int(record_data)
     p.reorder_interval = int(reorder_interval)
     p.block_size = int(block_size)
+    p.restitution = float(settings.physics.restitution)
     return p, mats
 
 

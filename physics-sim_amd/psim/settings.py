@@ -79,6 +79,10 @@ class PhysicsSettings:  # settings.rs:176-197, defaults :336-357
     moment_contribution_factor: float = 1.0
     local_damping: bool = False
     local_damping_alpha: float = 0.1
+    # Extension, not a reference setting (not in the uniform or the saved state): the
+    # restitution of the contact dashpot the benchmark configurations call for
+    # (dem_params.restitution); 0 = no dashpot, the reference's contact law.
+    restitution: float = 0.0
 
 
 DEFAULT_MATERIALS = [1.0, 1.0, 1.0, 0.01, 100.0, 50.0, 1.0, 0.0, 0.0, 0.01, 100.0, 50.0]  # settings.rs:266

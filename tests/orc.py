@@ -43,7 +43,8 @@ def lib():
             subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
         L = C.CDLL(ORC_PATH)
         L.orc_run.argtypes = [C.POINTER(abi.DemUniform), C.POINTER(abi.DemMaterial), C.c_int32,
-                              C.POINTER(abi.DemGrid), C.POINTER(OrcState), C.c_int32, C.c_int32, C.c_int32]
+                              C.POINTER(abi.DemGrid), C.POINTER(OrcState), C.c_int32, C.c_int32, C.c_int32,
+                              C.c_float]
         L.orc_canonicalize.argtypes = [FP, C.c_int32, C.c_int32]
         L.orc_grid_capacity.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32, C.c_int32,
                                         C.POINTER(abi.DemGrid)]
@@ -51,12 +52,15 @@ def lib():
         L.orc_sort_cells.argtypes = [C.c_int32, IP, C.c_int32, IP, IP]
         L.orc_contact_pairs.argtypes = [C.c_int32, FP, FP, C.c_int32, IP, IP]
         L.orc_contact_pairs.restype = C.c_int64
+        L.orc_contact_pairs_grid.argtypes = [C.c_int32, FP, FP, C.c_int32, IP, IP]
+        L.orc_contact_pairs_grid.restype = C.c_int64
         L.orc_regen_bonds.argtypes = [C.c_int32, FP, FP, C.c_int32, C.c_int32, IP, C.c_int64, FP]
         L.orc_regen_bonds.restype = C.c_int64
         L.orc_kinetic_energy.argtypes = [C.c_int32, FP, FP, FP, IP, C.POINTER(abi.DemMaterial), C.c_int32]
         L.orc_kinetic_energy.restype = C.c_double
         L.orc_time_reference.argtypes = [C.POINTER(abi.DemUniform), C.POINTER(abi.DemMaterial), C.c_int32,
-                                         C.POINTER(abi.DemGrid), C.POINTER(OrcState), C.c_int32, C.c_int32]
+                                         C.POINTER(abi.DemGrid), C.POINTER(OrcState), C.c_int32, C.c_int32,
+                                         C.c_float]
         L.orc_time_reference.restype = C.c_double
         _orc = L
     return _orc
@@ -133,7 +137,7 @@ class OracleEngine:
         u.particles = self.st.p_count
         v = self._view()
         rc = lib().orc_run(C.byref(u), self.mats, len(self.mats), C.byref(self.grid), C.byref(v), self.mode,
-                           int(bool(self.use_f64)), int(n))
+                           int(bool(self.use_f64)), int(n), C.c_float(self.settings.physics.restitution))
         assert rc == 0, rc
 
     def set_properties(self, words, selections):

@@ -131,12 +131,42 @@ def test_granular_box_no_overlap_and_inside():
 
 
 def test_column_collapse_small():
+    """The deposited pile: inside the box, radii in range, no pair closer than the
+    designed overlap (5e-4 of the radius sum, recomputed in f64 from the f32
+    positions, up to their rounding), three to five contacts per disc by the brute-force
+    pair search, and most discs above the floor resting on two discs below them."""
+    import orc
     s, st = scenes.column_collapse(n=4096, per_row=64, width=256.0, height=128.0)
     n = st.p_count
     r = st.radii[:n]
     assert r.min() >= 0.175 and r.max() <= 0.525
     p = st.pos[: 2 * n].reshape(-1, 2)
-    assert np.all(p[:, 1] - r >= -64.0) and np.all(p[:, 0] - r >= -128.0)
+    assert np.all(p[:, 1] - r >= -64.0) and np.all(np.abs(p[:, 0]) + r <= 128.0)
+    cnt = np.zeros(n, dtype=np.int32)
+    part = np.zeros(n * 16, dtype=np.int32)
+    tot = orc.lib().orc_contact_pairs(n, st.pos.ctypes.data_as(orc.FP), st.radii.ctypes.data_as(orc.FP), 16,
+                                      cnt.ctypes.data_as(orc.IP), part.ctypes.data_as(orc.IP))
+    assert 3.0 < tot / n < 5.0
+    part = part.reshape(n, 16)
+    pd = p.astype(np.float64)
+    worst = 0.0
+    on_floor = 0
+    supports = [0, 0, 0]   # discs above the floor touching 0, 1, >= 2 discs below them
+    for i in range(n):
+        js = part[i, :cnt[i]]
+        d = np.hypot(pd[js, 0] - pd[i, 0], pd[js, 1] - pd[i, 1])
+        worst = max(worst, float(np.max(1.0 - d / (r[js].astype(np.float64) + r[i]), initial=0.0)))
+        if pd[i, 1] - r[i] < -64.0 + 1e-3:
+            on_floor += 1
+            continue
+        below = int(np.sum(pd[js, 1] < pd[i, 1]))
+        supports[min(below, 2)] += 1
+    assert 0 < on_floor < n // 10
+    # drop-and-roll leaves most discs in a valley; a few balance on one disc
+    assert supports[2] >= 0.8 * (n - on_floor) and supports[0] <= 0.01 * n
+    # f32 rounding of coordinates below 128 moves a distance by at most ~2.2e-5, i.e.
+    # 6.2e-5 of the smallest radius sum (0.35)
+    assert worst < 5e-4 + 7e-5
 
 
 def test_assemble_bonds_slots_sorted_by_partner():
