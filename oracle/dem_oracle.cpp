@@ -248,7 +248,10 @@ struct Ctx {
         float tx = -ny, ty = nx;
         float rt = rel_tangent(a, b, tx, ty);
         float lim = std::fabs(nf) * u.friction_coefficient;
-        sl.ft = wmin(wmax(sl.ft + rt * ks, -lim), lim);
+        // WGSL clamp (min(max(.))) written with comparisons, so that the sign of a zero
+        // result (mu = 0) is defined
+        const float t = sl.ft + rt * ks;
+        sl.ft = t < -lim ? -lim : (t > lim ? lim : t);
         float m = -r(a) * sl.ft;
         V3 f{nx * nf + tx * sl.ft, ny * nf + ty * sl.ft, m};
         add_data(a, nf, sl.ft, m);
@@ -606,6 +609,11 @@ struct Ctx {
                     sl.ft = 0.0f;
                 }
             }
+            // new partners fill free slots in ascending id while the particle holds fewer
+            // than 14 (2D_Simulation.wgsl:125: contacts and bonds share the 14 slots;
+            // existing slots are never evicted)
+            int occupied = 0;
+            for (int j = 0; j < K; j++) occupied += tab[j].b != -1;
             for (int32_t b : f) {
                 bool present = false;
                 int empty = -1;
@@ -613,7 +621,8 @@ struct Ctx {
                     if (tab[j].b == b) { present = true; break; }
                     if (tab[j].b == -1 && empty < 0) empty = j;
                 }
-                if (present || empty < 0) continue;
+                if (present || empty < 0 || occupied >= REF_MAX_CONTACTS) continue;
+                occupied++;
                 tab[empty].a = id;
                 tab[empty].b = b;
                 tab[empty].ft = 0.0f;

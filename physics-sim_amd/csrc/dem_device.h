@@ -12,7 +12,9 @@
 
 namespace psim {
 
-constexpr int KMAX = 16;         // contacts (unbonded) per particle
+// Slots per particle, contacts and live bond records together (the reference's contact
+// table, 2D_Simulation.wgsl:125): contact lists hold at most KMAX less the live records.
+constexpr int KMAX = 14;
 constexpr int NLMAX = 16;        // neighbour-list candidates per particle (more: stencil fallback)
 
 // Row 0 of a neighbour list: the candidate count in the top bits (NL_OVERFLOW: more

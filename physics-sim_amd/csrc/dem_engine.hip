@@ -694,8 +694,9 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
             std::sort(rec.begin(), rec.end());
             con.erase(std::unique(con.begin(), con.end(),
                                   [](auto& x, auto& y) { return x.first == y.first; }), con.end());
-            if ((int)con.size() > KMAX) {
-                set_error("particle " + std::to_string(i) + " has more than 16 contacts");
+            if ((int)(con.size() + rec.size()) > KMAX) {
+                // the reference's table holds 14 slots per particle (2D_Simulation.wgsl:125)
+                set_error("particle " + std::to_string(i) + " occupies more than 14 contact slots");
                 return DEM_ECAPACITY;
             }
             CC[i] = (int32_t)con.size();

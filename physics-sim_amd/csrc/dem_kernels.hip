@@ -206,7 +206,10 @@ __device__ __forceinline__ float3 linear_model(float mu, const Self& a, const fl
     float tx = -ny, ty = nx;
     float rt = rel_tangent(a, pb, db, tx, ty);
     float lim = fabsf(nf) * mu;
-    ft = fminf(fmaxf(ft + rt * ks, -lim), lim);
+    // clamp(ft + rt ks, -lim, lim) by comparisons: min/max leave the sign of a zero
+    // result open (mu = 0), comparisons keep it, on the CPU and the GPU alike
+    const float t = ft + rt * ks;
+    ft = t < -lim ? -lim : (t > lim ? lim : t);
     float m = -a.r * ft;
     dd.add(nf, ft, m);
     return make_float3(nx * nf + tx * ft, ny * nf + ty * ft, m);
@@ -393,24 +396,6 @@ __device__ __forceinline__ bool overlaps(const float4& pj, const float4& p) {
     return sqrtf(d2) < R;
 }
 
-// Adds partner id `oj` to the ascending list kept in the next-list buffer
-// (column s, stride cap); the largest drops when KMAX are held.
-__device__ __forceinline__ void list_insert(const DevArrays& a, int s, int oj, int& n) {
-    const int cap = (int)a.cap;
-    int pos = n;
-    while (pos > 0 && a.CPn[(pos - 1) * cap + s] > oj) {
-        if (pos < KMAX) a.CPn[pos * cap + s] = a.CPn[(pos - 1) * cap + s];
-        pos--;
-    }
-    if (pos < KMAX) {
-        a.CPn[pos * cap + s] = oj;
-        if (n < KMAX) n++;
-        else atomicAdd(&a.STAT[0], 1ull);
-    } else {
-        atomicAdd(&a.STAT[0], 1ull);
-    }
-}
-
 template <bool BONDS>
 __device__ __forceinline__ bool bonded_partner(const DevArrays& a, int b_lo, int b_hi, int oj) {
     if (!BONDS) return false;
@@ -579,151 +564,9 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         ma_ks = MA.shear_stiffness;
     }
 
-    // ---- narrow phase: every overlapping partner ----
-    int n = 0;
-    const bool narrow = (fl & SF_SIM) && (fl & SF_NARROW);
-    // listed: a neighbour-list candidate (never a live bond partner)
-    auto test = [&](int j, const float4& q, bool listed = false) {
-        if (j != s && overlaps(q, p)) {
-            const int oj = a.ID[j];
-            if (listed || !is_bonded(oj)) list_insert(a, s, oj, n);
-        }
-    };
-    const int w0 = (narrow && act) ? nlw : 0;
-    const int nc = nl_count(w0);
-    // Register path (plain kernels): at most four candidates and four old contacts.
-    // The hits stay in registers as {partner id, slot}, sorted by id,
-    // and the force loop reads each partner by slot: no id -> slot lookup and no
-    // re-read of the new list.  Same partners, same order, same arithmetic.
-    const bool fast = rec_regs && narrow && act && nc > 0 && nc <= 4 && cc_old <= 4;
-    int fo[4] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
-    int fj[4] = {0, 0, 0, 0};
-    if (narrow && act && nc > 0) {
-        // neighbour list: every particle within r_a + r_b + skin at the last build; none
-        // outside it can touch s while no particle has moved more than 0.45 skin since.
-        // The first four candidates' positions are loaded together.
-        const int j0 = nl_first(w0);
-        const int j1 = nc > 1 ? a.NL[cap + s] : -1;
-        const int j2 = nc > 2 ? a.NL[2 * cap + s] : -1;
-        const int j3 = nc > 3 ? a.NL[3 * cap + s] : -1;
-        const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const float4 q0 = a.P[cur][j0];
-        const float4 q1 = j1 >= 0 ? a.P[cur][j1] : z;
-        const float4 q2 = j2 >= 0 ? a.P[cur][j2] : z;
-        const float4 q3 = j3 >= 0 ? a.P[cur][j3] : z;
-        if (fast) {
-            const int jj[4] = {j0, j1, j2, j3};
-            const float4 qq[4] = {q0, q1, q2, q3};
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (jj[k] >= 0 && jj[k] != s && overlaps(qq[k], p)) {
-                    fo[k] = a.ID[jj[k]];
-                    fj[k] = jj[k];
-                    n++;
-                }
-            }
-            // ascending partner id (misses carry INT32_MAX and sort last)
-            auto cswap = [&](int x, int y) {
-                if (fo[y] < fo[x]) {
-                    const int to = fo[x], tj = fj[x];
-                    fo[x] = fo[y]; fj[x] = fj[y];
-                    fo[y] = to; fj[y] = tj;
-                }
-            };
-            cswap(0, 1);
-            cswap(2, 3);
-            cswap(0, 2);
-            cswap(1, 3);
-            cswap(1, 2);
-        } else {
-            test(j0, q0, true);
-            if (j1 >= 0) test(j1, q1, true);
-            if (j2 >= 0) test(j2, q2, true);
-            if (j3 >= 0) test(j3, q3, true);
-        }
-        for (int k = 4; k < nc; k += 2) {
-            const int ja = a.NL[k * cap + s];
-            const int jb = k + 1 < nc ? a.NL[(k + 1) * cap + s] : -1;
-            const float4 qa = a.P[cur][ja];
-            const float4 qb = jb >= 0 ? a.P[cur][jb] : z;
-            test(ja, qa, true);
-            if (jb >= 0) test(jb, qb, true);
-        }
-    } else if (narrow && act && nc < 0) {
-        // more candidates than a list holds, or a large body: the stencil around the
-        // particle's cell at the last build, over the last cell list (cells are wider
-        // than 2 r_small + skin, so the 3x3 stencil holds every small disc that can touch
-        // a small disc until the rebuild; a large body reaches r + r_small + skin), then
-        // the scene's large bodies from their list
-        const float2 xr0 = a.XR[s];
-        int cx, cy;
-        cell_of(S, xr0.x, xr0.y, cx, cy);
-        const int pos0 = S.p0 - a.START[S.c0];  // cell-list position -> slot-list index
-        const bool large = LARGE && S.nlb > 0 && p.z > S.rsmax;
-        auto small_test = [&](int j, const float4& q) {
-            if (!LARGE || S.nlb == 0 || q.z <= S.rsmax) test(j, q);   // large bodies come from the list
-        };
-        if (!large) {
-            const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
-            const int rlo = max(cy - 1, 0), rhi = min(cy + 1, S.h - 1);
-            int e0[3], e1[3];
-#pragma unroll
-            for (int r = 0; r < 3; r++) {
-                const int ry = rlo + r;
-                if (ry <= rhi) {
-                    e0[r] = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
-                    e1[r] = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
-                } else {
-                    e0[r] = 0;
-                    e1[r] = 0;
-                }
-            }
-            // the three stencil rows advance together, two candidates each per round: the
-            // slot-id loads of every row are in flight before the first position load
-            int e[3] = {e0[0], e0[1], e0[2]};
-            while (e[0] < e1[0] || e[1] < e1[1] || e[2] < e1[2]) {
-                int j[6];
-#pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    j[2 * r] = e[r] < e1[r] ? a.SIDX[e[r]] : -1;
-                    j[2 * r + 1] = e[r] + 1 < e1[r] ? a.SIDX[e[r] + 1] : -1;
-                }
-                float4 q[6];
-#pragma unroll
-                for (int k = 0; k < 6; k++) q[k] = j[k] >= 0 ? a.P[cur][j[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-                for (int k = 0; k < 6; k++)
-                    if (j[k] >= 0) small_test(j[k], q[k]);
-#pragma unroll
-                for (int r = 0; r < 3; r++) e[r] += 2;
-            }
-        } else {
-            const int reach = max(1, (int)ceilf((p.z + S.rsmax + S.skin) * S.inv_cs) + 1);
-            const int xa = max(cx - reach, 0), xb = min(cx + reach, S.w - 1);
-            for (int ry = max(cy - reach, 0); ry <= min(cy + reach, S.h - 1); ry++) {
-                const int e0 = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
-                const int e1 = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
-                for (int e = e0; e < e1; e += 2) {
-                    const int ja = a.SIDX[e];
-                    const int jb = e + 1 < e1 ? a.SIDX[e + 1] : -1;
-                    const float4 qa = a.P[cur][ja];
-                    const float4 qb = jb >= 0 ? a.P[cur][jb] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    small_test(ja, qa);
-                    if (jb >= 0) small_test(jb, qb);
-                }
-            }
-        }
-        if (LARGE) {
-            for (int k = 0; k < S.nlb; k++) {
-                const int jl = a.INV[S.p0 + a.LB[S.lb0 + k]];
-                test(jl, a.P[cur][jl]);   // test skips s itself
-            }
-        }
-    }
-
     if (!act) {
         // never binned into a real cell, never a partner; state kept in both buffers
-        copy_list(a, s, cc_old);
+        copy_list(a, s, cc_of(pf));
         if ((fl & SF_SIM) && S.record) {
             const float4 d0 = a.DATA0[s];
             a.DATA0[s] = make_float4(0.0f, 0.0f, 0.0f, d0.w);
@@ -732,6 +575,19 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         if (args.flip) a.P[nxt][s] = p;
         return;
     }
+
+    int n = 0;   // entries of the new contact list
+    const int oc = min(cc_old, KMAX);
+    const bool narrow = (fl & SF_SIM) && (fl & SF_NARROW);
+    const int w0 = narrow ? nlw : 0;
+    const int nc = nl_count(w0);
+    // The list path: every candidate of a list sorted by partner id is tested, forced
+    // and appended in that order, so the contacts come out ascending with no sorting;
+    // taken while the list cannot overflow the KMAX slots that contacts share with bond
+    // records (2D_Simulation.wgsl:125).  Otherwise (stencil fallback, a list that could
+    // overflow) the general path collects the partners first and applies the
+    // reference's keep order before forcing them.
+    const bool stream = narrow && nc > 0 && nc + (b_hi - b_lo) <= KMAX;
 
     float ax, ay, al;
 
@@ -744,18 +600,11 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         me.rho = ma_rho; me.cn2 = S.cn2; me.dt = dt;
         Data3 dd{0.0f, 0.0f, 0.0f};
         float nfx = 0.0f, nfy = 0.0f, nm = 0.0f;
-        const int oc = min(cc_old, KMAX);
-        if (!narrow) {
-            // no narrow phase this step: the existing contacts carry over
-            n = oc;
-            for (int q = 0; q < n; q++) a.CPn[q * cap + s] = a.CP[q * cap + s];
-        }
 
-        // 2. forces over bond records and contacts merged in ascending partner id;
-        //    the new list is written as it is evaluated
-        int bq = b_lo;
         // bond records: the bond pass of this launch evaluated each one (force, moment,
-        // record and break state); the owner adds them in partner order
+        // record and break state); the owner adds them in partner order, merged with its
+        // contacts
+        int bq = b_lo;
         auto eval_record = [&](int q) {
             const float4 rf = a.RF[q];
             if (rf.w == 0.0f) return;
@@ -767,28 +616,173 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 replay(dd, a.RD[2 * q + 1]);
             }
         };
+        auto rec_id = [&](int q) { return rec_regs ? rx[(q - b_lo) & 3] : a.BR[q].x; };
 
-        if (fast) {
-            // previous list (at most four entries) for the tangential-force carry-over
-            int co[4];
-            float cf[4];
+        if (stream) {
+            if (BONDS) load_records();
+            // the old list's entry under the merge pointer (ascending partner ids), for the
+            // tangential force a contact carries over
+            int pp = 0;
+            int ob = oc > 0 ? a.CP[s] : INT32_MAX;
+            float fo = oc > 0 ? a.CF[s] : 0.0f;
+            // four candidates' slots and positions in flight at a time
+            for (int k0 = 0; k0 < nc; k0 += 4) {
+                int jj[4];
+                float4 qq[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                co[k] = k < oc ? a.CP[k * cap + s] : -1;
-                cf[k] = k < oc ? a.CF[k * cap + s] : 0.0f;
+                for (int t = 0; t < 4; t++) {
+                    const int k = k0 + t;
+                    jj[t] = k < nc ? (k == 0 ? nl_first(w0) : a.NL[k * cap + s]) : -1;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; t++) qq[t] = jj[t] >= 0 ? a.P[cur][jj[t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int j = jj[t];
+                    if (j < 0 || j == s || !overlaps(qq[t], p)) continue;
+                    const int oj = a.ID[j];
+                    if (BONDS) {
+                        while (bq < b_hi && rec_id(bq) < oj) eval_record(bq++);
+                    }
+                    while (ob < oj) {
+                        pp++;
+                        ob = pp < oc ? a.CP[pp * cap + s] : INT32_MAX;
+                        fo = pp < oc ? a.CF[pp * cap + s] : 0.0f;
+                    }
+                    float ft = ob == oj ? fo : 0.0f;
+                    const float4 db = del_of(a, j, dt);
+                    const float3 kb = partner_k(a, S, j);
+                    const float3 f = linear_model<F64>(u.friction_coefficient, me, qq[t], db, kb, me.r + qq[t].z, ft, dd);
+                    nfx += f.x;
+                    nfy += f.y;
+                    nm += f.z;
+                    a.CPn[n * cap + s] = oj;
+                    a.CFn[n * cap + s] = ft;
+                    n++;
+                }
             }
-            for (int k = 0; k < n; k++) {
-                // the hit of rank k sits in element 0 (the arrays shift after each)
-                const int ob = fo[0], j = fj[0];
+        } else {
+            if (narrow && nc != 0) {
+                // general path: every overlapping partner, from the list or (nc < 0) from the
+                // stencil of the last cell list plus the scene's large bodies
+                // capacity for contacts: the KMAX slots less the live bond records
+                int alive = 0;
                 if (BONDS) {
-                    // records with a smaller partner id come first (at most four: registers)
-                    load_records();
-                    while (bq < b_hi && rx[(bq - b_lo) & 3] < ob) eval_record(bq++);
+                    if (rec_regs) {
+                        load_records();
+                        alive = __popc(ralive);
+                    } else {
+                        for (int q = b_lo; q < b_hi; q++) alive += a.BR[q].w != 0;
+                    }
+                }
+                const int cap_c = max(KMAX - alive, 0);
+                // Existing contacts found again are kept (bit per old-list position); new
+                // partners go into an ascending list in the next-list buffer, the smallest
+                // cap_c of them kept (the reference fills free slots and never evicts).
+                uint32_t keep = 0;
+                int nnew = 0, found_new = 0;
+                auto test = [&](int j, const float4& q, bool listed) {
+                    if (j == s || !overlaps(q, p)) return;
+                    const int oj = a.ID[j];
+                    if (!listed && is_bonded(oj)) return;
+                    int at = -1;
+                    for (int m = 0; m < oc; m++)
+                        if (a.CP[m * cap + s] == oj) at = m;
+                    if (at >= 0) {
+                        keep |= 1u << at;
+                        return;
+                    }
+                    found_new++;
+                    int pos = nnew;
+                    while (pos > 0 && a.CPn[(pos - 1) * cap + s] > oj) {
+                        if (pos < cap_c) a.CPn[pos * cap + s] = a.CPn[(pos - 1) * cap + s];
+                        pos--;
+                    }
+                    if (pos < cap_c) {
+                        a.CPn[pos * cap + s] = oj;
+                        if (nnew < cap_c) nnew++;
+                    }
+                };
+                if (nc > 0) {
+                    for (int k = 0; k < nc; k++) {
+                        const int j = k == 0 ? nl_first(w0) : a.NL[k * cap + s];
+                        test(j, a.P[cur][j], true);
+                    }
+                } else {
+                    // more candidates than a list holds, or a large body: the stencil around
+                    // the particle's cell at the last build, over the last cell list (cells
+                    // are wider than 2 r_small + skin, so the 3x3 stencil holds every small
+                    // disc that can touch a small disc until the rebuild; a large body
+                    // reaches r + r_small + skin), then the scene's large bodies from their list
+                    const float2 xr0 = a.XR[s];
+                    int cx, cy;
+                    cell_of(S, xr0.x, xr0.y, cx, cy);
+                    const int pos0 = S.p0 - a.START[S.c0];  // cell-list position -> slot-list index
+                    const bool large = LARGE && S.nlb > 0 && p.z > S.rsmax;
+                    const int reach = large ? max(1, (int)ceilf((p.z + S.rsmax + S.skin) * S.inv_cs) + 1) : 1;
+                    const int xa = max(cx - reach, 0), xb = min(cx + reach, S.w - 1);
+                    for (int ry = max(cy - reach, 0); ry <= min(cy + reach, S.h - 1); ry++) {
+                        const int e0 = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
+                        const int e1 = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
+                        for (int e = e0; e < e1; e += 2) {
+                            const int ja = a.SIDX[e];
+                            const int jb = e + 1 < e1 ? a.SIDX[e + 1] : -1;
+                            const float4 qa = a.P[cur][ja];
+                            const float4 qb = jb >= 0 ? a.P[cur][jb] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                            // large bodies come from their list
+                            if (!LARGE || S.nlb == 0 || qa.z <= S.rsmax) test(ja, qa, false);
+                            if (jb >= 0 && (!LARGE || S.nlb == 0 || qb.z <= S.rsmax)) test(jb, qb, false);
+                        }
+                    }
+                    if (LARGE) {
+                        for (int k = 0; k < S.nlb; k++) {
+                            const int jl = a.INV[S.p0 + a.LB[S.lb0 + k]];
+                            test(jl, a.P[cur][jl], false);   // test skips s itself
+                        }
+                    }
+                }
+                // the kept old contacts come first; the new ones fill what is left
+                const int nk = __popc(keep);
+                const int room = max(cap_c - nk, 0);
+                if (nnew > room) nnew = room;
+                if (found_new > nnew) atomicAdd(&a.STAT[0], (unsigned long long)(found_new - nnew));
+                // merge kept old entries (ascending, from CP) with the new ones (CPn[0, nnew))
+                // into CPn[0, nk + nnew), from the back
+                int i_new = nnew - 1, i_old = oc - 1;
+                for (int w = nk + nnew - 1; w >= 0; w--) {
+                    while (i_old >= 0 && !((keep >> i_old) & 1u)) i_old--;
+                    const int vo = i_old >= 0 ? a.CP[i_old * cap + s] : INT32_MIN;
+                    const int vn = i_new >= 0 ? a.CPn[i_new * cap + s] : INT32_MIN;
+                    if (vo > vn) {
+                        a.CPn[w * cap + s] = vo;
+                        i_old--;
+                    } else {
+                        a.CPn[w * cap + s] = vn;
+                        i_new--;
+                    }
+                }
+                n = nk + nnew;
+            } else if (!narrow) {
+                // no narrow phase this step: the existing contacts carry over
+                n = oc;
+                for (int q = 0; q < n; q++) a.CPn[q * cap + s] = a.CP[q * cap + s];
+            }
+            // forces over the list in CPn, merged with the bond records in partner order;
+            // partners are found by id
+            int pp = 0;  // read position in the previous list (ascending ids)
+            for (int q = 0; q < n; q++) {
+                const int ob = a.CPn[q * cap + s];
+                if (BONDS) {
+                    while (bq < b_hi && a.BR[bq].x < ob) eval_record(bq++);
                 }
                 float ft = 0.0f;
-#pragma unroll
-                for (int m = 0; m < 4; m++)
-                    if (m < oc && co[m] == ob) ft = cf[m];
+                if (narrow) {
+                    while (pp < oc && a.CP[pp * cap + s] < ob) pp++;
+                    if (pp < oc && a.CP[pp * cap + s] == ob) ft = a.CF[pp * cap + s];
+                } else {
+                    ft = a.CF[q * cap + s];
+                }
+                const int j = a.INV[S.p0 + ob];
                 const float4 pb = a.P[cur][j];
                 const float4 db = del_of(a, j, dt);
                 const float3 kb = partner_k(a, S, j);
@@ -796,37 +790,8 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 nfx += f.x;
                 nfy += f.y;
                 nm += f.z;
-                a.CPn[k * cap + s] = ob;
-                a.CFn[k * cap + s] = ft;
-#pragma unroll
-                for (int m = 0; m < 3; m++) {
-                    fo[m] = fo[m + 1];
-                    fj[m] = fj[m + 1];
-                }
+                a.CFn[q * cap + s] = ft;
             }
-        }
-        int pp = 0;  // read position in the previous list (ascending ids)
-        for (int q = 0; q < (fast ? 0 : n); q++) {
-            const int ob = a.CPn[q * cap + s];
-            if (BONDS) {
-                while (bq < b_hi && a.BR[bq].x < ob) eval_record(bq++);
-            }
-            float ft = 0.0f;
-            if (narrow) {
-                while (pp < oc && a.CP[pp * cap + s] < ob) pp++;
-                if (pp < oc && a.CP[pp * cap + s] == ob) ft = a.CF[pp * cap + s];
-            } else {
-                ft = a.CF[q * cap + s];
-            }
-            const int j = a.INV[S.p0 + ob];
-            const float4 pb = a.P[cur][j];
-            const float4 db = del_of(a, j, dt);
-            const float3 kb = partner_k(a, S, j);
-            const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb, me.r + pb.z, ft, dd);
-            nfx += f.x;
-            nfy += f.y;
-            nm += f.z;
-            a.CFn[q * cap + s] = ft;
         }
         if (BONDS) {
             // the records after the last contact, four results in flight at a time
@@ -1078,13 +1043,12 @@ __device__ __forceinline__ bool block_scene_f64(const StepArgs& args) {
     return sid >= 0 && args.scenes[sid].use_f64 != 0;
 }
 
-// Plain f32 scenes: 8 waves per SIMD (at most 64 VGPRs, no spill) to hide the latency
-// of the candidate loop (a kernel with scratch costs more per launch, so the register
-// budget is kept without spills); launches with a large body take
+// Plain f32 scenes: 6 waves per SIMD, the most at which the list path fits its
+// registers (78 VGPRs) without spilling; launches with a large body take
 // k_step_plain_large, whose stencil fallback also covers them.  Launches holding an
 // f64 scene (k_step_plain_f64) and bonded scenes carry more live state and keep the
 // default register budget; both handle large bodies.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_plain(StepArgs args) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) k_step_plain(StepArgs args) {
     step_slot<false, false, false>(args);
 }
 // 7 waves per SIMD: 71 VGPRs without spills.
@@ -1277,8 +1241,8 @@ __device__ __forceinline__ void rb_scatter_slot(const StepArgs& args, int s, int
 }
 
 // Neighbour list of slot s: the slots within r_a + r_b + skin (x 1.0001) in the 3x3
-// stencil of its cell; more than NLMAX marks the particle for the stencil fallback
-// (count NL_OVERFLOW).  Records the build position.
+// stencil of its cell, sorted by partner id; more than NLMAX marks the particle for the
+// stencil fallback (count NL_OVERFLOW).  Records the build position.
 __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int lbs) {
     const DevArrays& a = args.a;
     const int sid = args.blk_scene[s / lbs];
@@ -1317,9 +1281,33 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
         }
         return false;
     };
-    // the three stencil rows advance together, two candidates each per round, so
-    // that six slot-id loads and then six position loads are in flight at once
-    // (the list's order is free: the step kernel inserts partners in id order)
+    // The list is kept sorted by partner id (the step kernel forces its hits in that
+    // order): candidates go into a register array by insertion, with compile-time
+    // indices only.  The three stencil rows advance together, two candidates each per
+    // round, so that six slot-id loads and then six position loads are in flight at once.
+    int ls[NLMAX], li[NLMAX];   // slots and ids, ascending id
+    int cnt = 0;
+    auto add = [&](int j) {
+        const int id = a.ID[j];
+        if (cnt < NLMAX) {
+            int pos = 0;
+#pragma unroll
+            for (int t = 0; t < NLMAX; t++) pos += (t < cnt && li[t] < id) ? 1 : 0;
+#pragma unroll
+            for (int t = NLMAX - 1; t > 0; t--)
+                if (t > pos && t <= cnt) {
+                    ls[t] = ls[t - 1];
+                    li[t] = li[t - 1];
+                }
+#pragma unroll
+            for (int t = 0; t < NLMAX; t++)
+                if (t == pos) {
+                    ls[t] = j;
+                    li[t] = id;
+                }
+        }
+        cnt++;
+    };
     int e[3], e1[3];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
@@ -1333,7 +1321,6 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
         }
     }
     const float lr = p.z + S.skin;
-    int cnt = 0, first = 0;
     while (e[0] < e1[0] || e[1] < e1[1] || e[2] < e1[2]) {
         int j[6];
 #pragma unroll
@@ -1351,11 +1338,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
             const float xx = dx * dx;
             const float yy = dy * dy;
             const float L = (lr + q[k].z) * 1.0001f;
-            if (xx + yy < L * L && (S.nlb == 0 || q[k].z <= S.rsmax) && !bonded(j[k])) {
-                if (cnt == 0) first = j[k];
-                else if (cnt < NLMAX) a.NL[cnt * cap + s] = j[k];
-                cnt++;
-            }
+            if (xx + yy < L * L && (S.nlb == 0 || q[k].z <= S.rsmax) && !bonded(j[k])) add(j[k]);
         }
 #pragma unroll
         for (int r = 0; r < 3; r++) e[r] += 2;
@@ -1368,13 +1351,14 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
         const float xx = dx * dx;
         const float yy = dy * dy;
         const float L = (lr + q.z) * 1.0001f;
-        if (xx + yy < L * L && !bonded(jl)) {
-            if (cnt == 0) first = jl;
-            else if (cnt < NLMAX) a.NL[cnt * cap + s] = jl;
-            cnt++;
-        }
+        if (xx + yy < L * L && !bonded(jl)) add(jl);
     }
-    a.NL[s] = nl_word(cnt <= NLMAX ? cnt : -1, first);
+    if (cnt <= NLMAX) {
+#pragma unroll
+        for (int t = 1; t < NLMAX; t++)
+            if (t < cnt) a.NL[t * cap + s] = ls[t];
+    }
+    a.NL[s] = nl_word(cnt <= NLMAX ? cnt : -1, cnt > 0 && cnt <= NLMAX ? ls[0] : 0);
     if (cnt > NLMAX) atomicAdd(&a.STAT[5], 1ull);   // particles on the stencil fallback
 }
 

@@ -265,3 +265,46 @@ def test_graph_replays_equal_direct_launches(scene):
     assert b.counters()["graph_captures"] == 0
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("preload", [False, True])
+def test_contact_slot_cap_matches_oracle(preload):
+    """A disc of radius 2 (a large body among r = 0.3 discs) touching 20 discs holds 14
+    contacts, as the reference's table does (2D_Simulation.wgsl:125): slots already held
+    are kept (preloaded with the 14 largest ids) and new partners take the free ones in
+    ascending id.  State and contact table equal the oracle after 1 and 25 more steps;
+    the dropped pairs are counted."""
+    from test_oracle import _crowded
+    s, st = _crowded(slots_preload=list(range(7, 21)) if preload else ())
+    n = st.p_count
+    prog = ComputeProg(s, st.copy(), reorder_interval=4)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    for k in (1, 25):
+        prog.step(k)
+        eng.step(k)
+        got = prog.update_state()
+        assert_bits_equal(got, eng.st, n)
+        want = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n]
+        assert np.array_equal(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want)
+    assert prog.counters()["contact_overflow"] > 0
+    prog.close()
+
+
+def test_wide_cells_general_path_bit_exact():
+    """Cells three times wider than needed: each neighbour list holds more candidates than
+    the slots allow (or overflows into the stencil fallback), so every disc takes the
+    general path.  A compressed box, identical to the oracle after 20 steps."""
+    s, st = scenes.granular_box(n_side=24, seed=13, spacing=1.75)
+    n = st.p_count
+    g = ComputeProg.default_grid(s, st)
+    grid = ComputeProg.default_grid(s, st, cell_size=3.0 * g.cell_size)
+    prog = ComputeProg(s, st.copy(), grid=grid, reorder_interval=5)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    prog.step(20)
+    eng.step(20)
+    got = prog.update_state()
+    assert_bits_equal(got, eng.st, n)
+    want = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n]
+    assert np.array_equal(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want)
+    assert prog.counters()["list_overflow"] > 0
+    prog.close()

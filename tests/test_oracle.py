@@ -300,3 +300,43 @@ def test_dashpot_restitution_closed_form(e):
         return (v[2] - v[0]) / 2.0
     assert abs(collide(0.0) - 1.0) < 1e-3
     assert abs(collide(e) - e) < 0.01 * e
+
+
+def _crowded(big_r=2.0, n_small=20, slots_preload=()):
+    """A disc of radius big_r ringed by n_small discs of radius 0.3, every one
+    overlapping it; no gravity, no friction, at rest.  Disc 0 is the big one; its table
+    is preloaded with plain contacts to the small discs in `slots_preload`."""
+    s, st = scenes.granular_box(n_side=5, seed=0, restitution=0.0)
+    s.physics.gravity = False
+    s.physics.friction_coefficient = 0.0
+    s.simulation.hor_bound = s.simulation.vert_bound = 64.0
+    n = 1 + n_small
+    st.p_count = n
+    st.radii[:] = 0.0
+    st.radii[0] = big_r
+    st.radii[1:n] = 0.3
+    ang = 2 * np.pi * np.arange(n_small) / n_small
+    st.pos[:2 * n] = 0.0
+    st.pos[2:2 * n:2] = ((big_r + 0.25) * np.cos(ang)).astype(np.float32)
+    st.pos[3:2 * n:2] = ((big_r + 0.25) * np.sin(ang)).astype(np.float32)
+    c = st.contacts.view(np.int32).reshape(-1, st.max_contacts, 6)
+    for k, b in enumerate(slots_preload):
+        c[0, k] = [0, b, 0, -1, -1, -1]
+        c[b, 0] = [b, 0, 0, -1, -1, -1]
+    return s, st
+
+
+def test_contact_slots_capped_at_14_existing_kept():
+    """The 14-slot table of 2D_Simulation.wgsl:125-227: a disc touching 20 others holds
+    14 of them.  Slots already occupied are never evicted (a contact is deleted only when
+    its partner is no longer found, :185-204) and new partners only take free slots
+    (:206-227), here in ascending id: preloaded with the 14 largest ids, the table keeps
+    them; from empty, it takes the 14 smallest.  Every small disc holds the big one."""
+    for preload, want in ((list(range(7, 21)), list(range(7, 21))), ((), list(range(1, 15)))):
+        s, st = _crowded(slots_preload=preload)
+        eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+        eng.step(1)
+        c = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)
+        assert c[0, :, 1].tolist() == want
+        for b in range(1, 21):
+            assert c[b, 0, 1] == 0 and c[b, 1, 1] == -1
