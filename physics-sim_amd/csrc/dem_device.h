@@ -116,6 +116,7 @@ struct SceneDev {
     int32_t lb0, nlb;      // large bodies: particle ids LB[lb0, lb0 + nlb)
     float rho1, kn1, ks1;  // density and stiffnesses of material 0 (the whole scene when nm == 1)
     float cn2;             // contact dashpot factor 2 zeta of dem_params.restitution (0: none)
+    float kne1, kse1;      // 1 / (1 / k + 1 / k) of material 0's normal and shear stiffness (f32)
 };
 
 // Large bodies: at most LB_MAX per scene, the discs above the first gap of a factor 2

@@ -479,6 +479,12 @@ SceneDev Engine::scene_dev(const SceneHost& sh, int32_t sid) const {
         d.rho1 = sh.mats[0].density;
         d.kn1 = sh.mats[0].normal_stiffness;
         d.ks1 = sh.mats[0].shear_stiffness;
+        // the series stiffness of two material-0 discs, the f32 expression of
+        // 2D_Simulation.wgsl:345-346 evaluated once (every contact of a one-material scene
+        // forms the same value)
+        const float kn = d.kn1, ks = d.ks1;
+        d.kne1 = 1.0f / (1.0f / kn + 1.0f / kn);
+        d.kse1 = 1.0f / (1.0f / ks + 1.0f / ks);
     }
     return d;
 }

@@ -80,6 +80,8 @@ __device__ __forceinline__ int cell_of(const SceneDev& S, float x, float y, int&
 struct Self {
     float x, y, r, drot, dx, dy;
     float kn, ks;
+    float kne, kse;  // series stiffnesses against material 0 when the scene has one material
+    bool one;        //   (the same f32 expression, formed once on the host), else unused
     float rho;      // density (mass of the dashpot)
     float cn2;      // dashpot factor 2 zeta (0: no dashpot, the reference's law)
     float dt;
@@ -188,8 +190,8 @@ __device__ __forceinline__ float3 linear_model(float mu, const Self& a, const fl
                                                const float3& kb, float L, float& ft, DD& dd) {
     float nd = fminf(0.0f, dist2<F64>(a, pb.x, pb.y, L));
     if (nd == 0.0f) return make_float3(0.0f, 0.0f, 0.0f);
-    float kn = 1.0f / (1.0f / a.kn + 1.0f / kb.x);
-    float ks = 1.0f / (1.0f / a.ks + 1.0f / kb.y);
+    float kn = a.one ? a.kne : 1.0f / (1.0f / a.kn + 1.0f / kb.x);
+    float ks = a.one ? a.kse : 1.0f / (1.0f / a.ks + 1.0f / kb.y);
     float nf = -nd * kn;
     float dx = a.x - pb.x, dy = a.y - pb.y;
     float l = lenf(dx, dy);
@@ -598,6 +600,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         me.x = p.x; me.y = p.y; me.r = p.z; me.drot = dme.z; me.dx = dme.x; me.dy = dme.y;
         me.kn = ma_kn; me.ks = ma_ks;
         me.rho = ma_rho; me.cn2 = S.cn2; me.dt = dt;
+        me.one = S.nm <= 1; me.kne = S.kne1; me.kse = S.kse1;
         Data3 dd{0.0f, 0.0f, 0.0f};
         float nfx = 0.0f, nfy = 0.0f, nm = 0.0f;
 
@@ -616,41 +619,65 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 replay(dd, a.RD[2 * q + 1]);
             }
         };
-        auto rec_id = [&](int q) { return rec_regs ? rx[(q - b_lo) & 3] : a.BR[q].x; };
+        // partner id of record q (selects, not an indexed register array, which would
+        // live in scratch memory)
+        auto rec_id = [&](int q) {
+            if (!rec_regs) return a.BR[q].x;
+            const int k = q - b_lo;
+            return k == 0 ? rx[0] : (k == 1 ? rx[1] : (k == 2 ? rx[2] : rx[3]));
+        };
 
         if (stream) {
             if (BONDS) load_records();
-            // the old list's entry under the merge pointer (ascending partner ids), for the
-            // tangential force a contact carries over
-            int pp = 0;
-            int ob = oc > 0 ? a.CP[s] : INT32_MAX;
-            float fo = oc > 0 ? a.CF[s] : 0.0f;
-            // four candidates' slots and positions in flight at a time
+            // The old list (ascending partner ids) through a window of four entries in
+            // registers: ow[0] is the entry under the merge pointer, the window shifts as
+            // it advances and loads the entry four ahead; a hit takes its tangential force
+            // from there.
+            int ow[4];
+            float fw[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                ow[t] = t < oc ? a.CP[t * cap + s] : INT32_MAX;
+                fw[t] = t < oc ? a.CF[t * cap + s] : 0.0f;
+            }
+            int pn = 4;   // old-list index of the next entry to load into the window
+            // four candidates at a time: slots, then their positions, velocity records and
+            // ids in one round of loads (before the overlap tests, so that a hit waits on
+            // nothing)
             for (int k0 = 0; k0 < nc; k0 += 4) {
                 int jj[4];
-                float4 qq[4];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     const int k = k0 + t;
                     jj[t] = k < nc ? (k == 0 ? nl_first(w0) : a.NL[k * cap + s]) : -1;
                 }
+                float4 qq[4], vq[4];
+                int iq[4];
 #pragma unroll
-                for (int t = 0; t < 4; t++) qq[t] = jj[t] >= 0 ? a.P[cur][jj[t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                for (int t = 0; t < 4; t++) {
+                    const bool ok = jj[t] >= 0;
+                    qq[t] = ok ? a.P[cur][jj[t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    vq[t] = ok ? a.V[jj[t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    iq[t] = ok ? a.ID[jj[t]] : INT32_MAX;
+                }
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     const int j = jj[t];
                     if (j < 0 || j == s || !overlaps(qq[t], p)) continue;
-                    const int oj = a.ID[j];
+                    const int oj = iq[t];
                     if (BONDS) {
                         while (bq < b_hi && rec_id(bq) < oj) eval_record(bq++);
                     }
-                    while (ob < oj) {
-                        pp++;
-                        ob = pp < oc ? a.CP[pp * cap + s] : INT32_MAX;
-                        fo = pp < oc ? a.CF[pp * cap + s] : 0.0f;
+                    while (ow[0] < oj) {
+                        ow[0] = ow[1]; fw[0] = fw[1];
+                        ow[1] = ow[2]; fw[1] = fw[2];
+                        ow[2] = ow[3]; fw[2] = fw[3];
+                        ow[3] = pn < oc ? a.CP[pn * cap + s] : INT32_MAX;
+                        fw[3] = pn < oc ? a.CF[pn * cap + s] : 0.0f;
+                        pn++;
                     }
-                    float ft = ob == oj ? fo : 0.0f;
-                    const float4 db = del_of(a, j, dt);
+                    float ft = ow[0] == oj ? fw[0] : 0.0f;
+                    const float4 db = del_from(a, j, vq[t], dt);
                     const float3 kb = partner_k(a, S, j);
                     const float3 f = linear_model<F64>(u.friction_coefficient, me, qq[t], db, kb, me.r + qq[t].z, ft, dd);
                     nfx += f.x;
@@ -983,6 +1010,7 @@ __device__ __forceinline__ void bond_pass_body(const StepArgs& args, const Scene
     me.x = p.x; me.y = p.y; me.r = p.z; me.drot = d.z; me.dx = d.x; me.dy = d.y;
     me.kn = MA.normal_stiffness; me.ks = MA.shear_stiffness;
     me.rho = MA.density; me.cn2 = S.cn2; me.dt = S.u.dT;
+    me.one = S.nm <= 1; me.kne = S.kne1; me.kse = S.kse1;
     const uint32_t fl = (uint32_t)lfw.x;
     const bool narrow = (fl & SF_SIM) && (fl & SF_NARROW);
     DataLog dl;
@@ -1043,25 +1071,26 @@ __device__ __forceinline__ bool block_scene_f64(const StepArgs& args) {
     return sid >= 0 && args.scenes[sid].use_f64 != 0;
 }
 
-// Plain f32 scenes: 6 waves per SIMD, the most at which the list path fits its
-// registers (78 VGPRs) without spilling; launches with a large body take
+// Plain f32 scenes: 5 waves per SIMD, the most at which the list path (four candidates'
+// positions, velocity records and ids in flight) fits its registers (94 VGPRs) without
+// spilling; launches with a large body take
 // k_step_plain_large, whose stencil fallback also covers them.  Launches holding an
 // f64 scene (k_step_plain_f64) and bonded scenes carry more live state and keep the
 // default register budget; both handle large bodies.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) k_step_plain(StepArgs args) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_step_plain(StepArgs args) {
     step_slot<false, false, false>(args);
 }
-// 7 waves per SIMD: 71 VGPRs without spills.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) k_step_plain_large(StepArgs args) {
+// 5 waves per SIMD: 94 VGPRs without spills.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_step_plain_large(StepArgs args) {
     step_slot<false, false, true>(args);
 }
 __global__ void __launch_bounds__(256) k_step_plain_f64(StepArgs args) {
     if (block_scene_f64(args)) step_slot<true, false, true>(args);
     else step_slot<false, false, true>(args);
 }
-// Bonded launches: the f32 variant fits 80 VGPRs without spills (6 waves per SIMD
-// instead of 5); the f64 one keeps the default budget (it would spill).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) k_step_bonds_f32(StepArgs args) {
+// Bonded launches: the f32 variant fits 106 VGPRs without spills at 4 waves per SIMD;
+// the f64 one keeps the default budget.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_step_bonds_f32(StepArgs args) {
     step_slot<false, true, true>(args);
 }
 __global__ void __launch_bounds__(256) k_step_bonds_f64(StepArgs args) {
