@@ -513,6 +513,19 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     float4 p = a.P[cur][s];
     const int nlw = a.NL[s];     // issued with the other own-state loads, used if the narrow phase runs
     const int cap = (int)a.cap;  // KMAX * cap < 2^31 (checked at pool creation)
+    // Issued with the own state as well, before the counts that say how many of them are
+    // valid are known (rows past a count hold stale words of this slot's column, in
+    // bounds): neighbour-list rows 1-3 and the old contact list's first four entries.
+    int nlr[3];
+    int ow[4];
+    float fw[4];
+#pragma unroll
+    for (int t = 0; t < 3; t++) nlr[t] = a.NL[(t + 1) * cap + s];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        ow[t] = a.CP[t * cap + s];
+        fw[t] = a.CF[t * cap + s];
+    }
     int b_lo = 0, b_hi = 0;
     // bond records: at most four are held in registers (partner ids, alive bits) for
     // the bonded-partner test of the contact search and the merge with the contacts
@@ -633,12 +646,12 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             // registers: ow[0] is the entry under the merge pointer, the window shifts as
             // it advances and loads the entry four ahead; a hit takes its tangential force
             // from there.
-            int ow[4];
-            float fw[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                ow[t] = t < oc ? a.CP[t * cap + s] : INT32_MAX;
-                fw[t] = t < oc ? a.CF[t * cap + s] : 0.0f;
+                if (t >= oc) {
+                    ow[t] = INT32_MAX;
+                    fw[t] = 0.0f;
+                }
             }
             int pn = 4;   // old-list index of the next entry to load into the window
             // four candidates at a time: slots, then their positions, velocity records and
@@ -649,7 +662,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     const int k = k0 + t;
-                    jj[t] = k < nc ? (k == 0 ? nl_first(w0) : a.NL[k * cap + s]) : -1;
+                    jj[t] = k < nc ? (k == 0 ? nl_first(w0) : (k0 == 0 ? nlr[t - 1] : a.NL[k * cap + s])) : -1;
                 }
                 float4 qq[4], vq[4];
                 int iq[4];
