@@ -308,3 +308,58 @@ def test_wide_cells_general_path_bit_exact():
     assert np.array_equal(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want)
     assert prog.counters()["list_overflow"] > 0
     prog.close()
+
+
+@pytest.mark.parametrize("mouse", [False, True])
+def test_planet_gravity_round_walls_bit_exact(mouse):
+    """Planet-mode gravity (the reference's default, settings.rs:339) toward the origin
+    or the mouse point, inside a circular wall (round_bounds): 900 discs fall together
+    and collide; state and contact table identical to the oracle after 60 and 300 steps."""
+    s, st = scenes.granular_box(n_side=30, seed=17, spacing=2.0)
+    s.physics.planet_mode = True
+    s.physics.mouse_gravity = mouse
+    s.physics.gravity_acceleration = 20.0
+    s.world_pos = (7.0, -5.0) if mouse else (0.0, 0.0)
+    s.simulation.round_walls = True
+    s.simulation.wall_radius = 43.0
+    s.simulation.timestep = 5e-4
+    n = st.p_count
+    prog = ComputeProg(s, st.copy(), reorder_interval=16)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    for k in (60, 240):
+        prog.step(k)
+        eng.step(k)
+        got = prog.update_state()
+        assert_bits_equal(got, eng.st, n)
+        want = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n]
+        assert np.array_equal(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want)
+    r = np.hypot(got.pos[0:2 * n:2], got.pos[1:2 * n:2]) + got.radii[:n]
+    assert r.max() <= 43.0 + 1e-3        # held by the circular wall
+    assert prog.counters()["contacts"] > n // 8   # discs in contact
+    prog.close()
+
+
+def test_external_forces_and_collision_interval_bit_exact():
+    """External forces and moments on a third of the discs (forces[id].{x, y, rot}) and a
+    narrow phase every third step (collision_interval 3: contacts carry over in between);
+    state, data words and contact table identical to the oracle over three calls."""
+    s, st = scenes.granular_box(n_side=24, seed=19, spacing=1.8)
+    n = st.p_count
+    s.physics.collision_interval = 3
+    rng = np.random.default_rng(19)
+    f = st.forces.reshape(-1, 6)
+    sel = rng.random(n) < 1.0 / 3.0
+    f[:n][sel, 0] = rng.normal(0, 50.0, sel.sum())
+    f[:n][sel, 1] = rng.normal(0, 50.0, sel.sum())
+    f[:n][sel, 2] = rng.normal(0, 5.0, sel.sum())
+    prog = ComputeProg(s, st.copy(), reorder_interval=8)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    for k in (7, 11, 40):
+        prog.step(k)
+        eng.step(k)
+        got = prog.update_state()
+        assert_bits_equal(got, eng.st, n)
+        assert np.array_equal(got.data[: 7 * n].view(np.int32), eng.st.data[: 7 * n].view(np.int32))
+        want = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n]
+        assert np.array_equal(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want)
+    prog.close()
