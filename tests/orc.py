@@ -62,8 +62,38 @@ def lib():
                                          C.POINTER(abi.DemGrid), C.POINTER(OrcState), C.c_int32, C.c_int32,
                                          C.c_float]
         L.orc_time_reference.restype = C.c_double
+        D = C.POINTER(C.c_double)
+        L.orc64_run.argtypes = [C.POINTER(abi.DemUniform), C.POINTER(abi.DemMaterial), C.c_int32, C.c_int32,
+                                FP, FP, FP, FP, FP, FP, FP, FP, IP, IP, IP, C.c_int32, FP, FP, C.c_int32,
+                                C.c_int32, C.c_float, D, D, D, D, D, D, D, IP, D]
         _orc = L
     return _orc
+
+
+def run64(state: State, settings, nsteps: int) -> dict:
+    """The f64 oracle ("ref64", oracle/dem_oracle64.cpp): `nsteps` steps from the f32
+    `state` in double throughout; returns the final state arrays (f64), the bond index
+    words and the kinetic energy after every step."""
+    n = state.p_count
+    u = settings.collision_settings()
+    u.particles = n
+    mats = abi.materials_array(settings.materials)
+    out = {k: np.zeros(m * n, dtype=np.float64) for k, m in
+           (("pos", 2), ("vel", 2), ("acc", 2), ("rot", 1), ("rot_vel", 1), ("rot_acc", 1), ("data", 7))}
+    out["bonds"] = np.zeros(max(state.n_bonds, 1), dtype=np.int32)
+    out["ke"] = np.zeros(max(nsteps, 1), dtype=np.float64)
+    D = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    s = state
+    rc = lib().orc64_run(C.byref(u), mats, len(mats), n, abi.fptr(s.pos), abi.fptr(s.vel), abi.fptr(s.acc),
+                         abi.fptr(s.rot), abi.fptr(s.rot_vel), abi.fptr(s.rot_acc), abi.fptr(s.forces),
+                         abi.fptr(s.radii), abi.iptr(s.fixity), abi.iptr(s.material_pointers), abi.iptr(s.bonds),
+                         s.n_bonds, abi.fptr(s.bond_strengths) if s.bond_strengths is not None else None,
+                         abi.fptr(s.contacts), s.max_contacts, int(nsteps),
+                         C.c_float(settings.physics.restitution), D(out["pos"]), D(out["vel"]), D(out["acc"]),
+                         D(out["rot"]), D(out["rot_vel"]), D(out["rot_acc"]), D(out["data"]),
+                         out["bonds"].ctypes.data_as(IP), D(out["ke"]))
+    assert rc == 0, rc
+    return out
 
 
 class OracleEngine:
