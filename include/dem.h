@@ -217,6 +217,22 @@ int dem_counters(dem_ctx* ctx, int64_t* out, int32_t n_out);
  * Requires record_data. */
 int dem_get_datum(dem_ctx* ctx, const int32_t* select, double* out);
 
+/* Sums over a group of particles given by id (ids[n_ids], each in [0, N)), computed on
+ * the device from the resident state: the grip force and grip displacement of a
+ * tensile test without reading the state back.  The reference gets the same
+ * quantities by reading everything back and summing on the host (State::get_datum,
+ * state.rs:708-737, over the selected grip particles).  With m = density * pi * r * r
+ * in f32 (the mass the step divides by) and a the stored acceleration:
+ *   out[0] Σ m a_x      out[1] Σ m a_y      (net force on the group: the reaction of
+ *                                            a grip whose velocity is prescribed)
+ *   out[2] Σ x          out[3] Σ y          (positions)
+ *   out[4] Σ v_x        out[5] Σ v_y
+ *   out[6] Σ |data[0]|  (normal-force datum, as get_datum; 0 without record_data)
+ *   out[7] n_ids
+ * Terms are f64 products of the f32 state, summed in f64 in a fixed order (equal to
+ * any other summation order to f64 rounding).  Synchronous. */
+int dem_group_sums(dem_ctx* ctx, const int32_t* ids, int32_t n_ids, double* out);
+
 /* Stage-level views for parity checks.  Each copies device arrays to the host
  * after running the broad phase (stages 1-3) on the current positions.
  *   cell_keys[N]     centre-cell key cy*w+cx of particle i (by particle id),
@@ -233,17 +249,20 @@ int dem_debug_contacts(dem_ctx* ctx, int32_t max_per, int32_t* counts, int32_t* 
 
 /* Bond builder on the cell list: the rule of State::regen_bonds
  * (state.rs:291-367) - pair (i, j), j > i, bonded iff
- * sqrt((xj-xi)^2 + (yj-yi)^2) < (ri + rj) * 1.05, at most max_bonds per lower
- * index i taken in increasing j - evaluated on the device over the current
- * positions.  Writes up to bond_cap pairs (i, j) in (i, j) order and their
- * count; returns DEM_ECAPACITY if more exist. */
-int dem_build_bond_pairs(dem_ctx* ctx, int32_t max_bonds, float factor,
+ * sqrt((xj-xi)^2 + (yj-yi)^2) < (ri + rj) * factor + gap in f32 (the reference:
+ * factor 1.05, gap 0; a bonded-assembly packing: factor 1, gap 0.1 r_min, i.e. every
+ * pair closer than gap), at most max_bonds per lower index i taken in increasing
+ * j - evaluated on the device over the current positions.  Writes up to bond_cap
+ * pairs (i, j) in (i, j) order and their count; returns DEM_ECAPACITY if more
+ * exist.  gap >= 0. */
+int dem_build_bond_pairs(dem_ctx* ctx, int32_t max_bonds, float factor, float gap,
                          int32_t* pairs, int32_t bond_cap, int32_t* n_out);
 
 /* Timing of the last dem_step call measured with HIP events on the ctx's
  * stream: total milliseconds and per-kernel milliseconds
  * (out[0] total, out[1] force+drift kernel, out[2] cell scan, out[3] scatter,
- *  out[4] re-sort), and the number of launches of each. */
+ *  out[4] re-sort, out[5] bond pass), and the number of launches of each: both
+ * arrays hold 6 entries. */
 int dem_last_step_timing(dem_ctx* ctx, int32_t enable, double* ms_out, int64_t* launches_out);
 
 /* ---- Serving loop: continuous batching of independent scenes ----

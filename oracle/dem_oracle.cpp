@@ -808,12 +808,14 @@ int64_t orc_contact_pairs_grid(int32_t n, const float* pos, const float* radii, 
     return over ? -1 : total;
 }
 
-// State::regen_bonds restated (state.rs:291-367).  O(N^2), in (i, j) order.
-// bonds_out: 3 words per bond; contacts_out: slots*6 words per particle (all -1
-// words on entry is not required: the function initialises them).  Returns the
-// bond count, or -1 if bond_cap is too small.
+// State::regen_bonds restated (state.rs:291-367).  O(N^2), in (i, j) order.  The
+// bond test is dist < (r_i + r_j) * factor + gap in f32: the reference's is factor
+// 1.05, gap 0 (state.rs:302); a bonded assembly bonds every pair closer than a gap
+// (factor 1).  bonds_out: 3 words per bond; contacts_out: slots*6 words per particle
+// (all -1 words on entry is not required: the function initialises them).  Returns
+// the bond count, or -1 if bond_cap is too small.
 int64_t orc_regen_bonds(int32_t n, const float* pos, const float* radii, int32_t max_bonds, int32_t slots,
-                        int32_t* bonds_out, int64_t bond_cap, float* contacts_out) {
+                        int32_t* bonds_out, int64_t bond_cap, float* contacts_out, float factor, float gap) {
     Slot* tab = reinterpret_cast<Slot*>(contacts_out);
     std::memset(contacts_out, 0xff, sizeof(Slot) * (size_t)n * slots);
     int64_t bond_index = 0;
@@ -824,7 +826,7 @@ int64_t orc_regen_bonds(int32_t n, const float* pos, const float* radii, int32_t
             float dx = pos[2 * j] - pos[2 * i], dy = pos[2 * j + 1] - pos[2 * i + 1];
             float dist = std::sqrt(dx * dx + dy * dy);
             float sum = radii[i] + radii[j];
-            if (dist < sum * 1.05f) {
+            if (dist < sum * factor + gap) {
                 if (col_num < max_bonds) {
                     if (bond_index >= bond_cap) return -1;
                     float mag = std::sqrt(dx * dx + dy * dy);

@@ -181,11 +181,18 @@ int dem_debug_contacts(dem_ctx* ctx, int32_t max_per, int32_t* counts, int32_t* 
     })
 }
 
-int dem_build_bond_pairs(dem_ctx* ctx, int32_t max_bonds, float factor, int32_t* pairs, int32_t bond_cap,
-                         int32_t* n_out) {
+int dem_build_bond_pairs(dem_ctx* ctx, int32_t max_bonds, float factor, float gap, int32_t* pairs,
+                         int32_t bond_cap, int32_t* n_out) {
     DEM_GUARD({
         if (!ctx || !pairs || !n_out || bond_cap < 0) { set_error("dem_build_bond_pairs: bad argument"); return DEM_EINVAL; }
-        return ctx->eng.bond_pairs(ctx->sid, max_bonds, factor, pairs, bond_cap, n_out);
+        return ctx->eng.bond_pairs(ctx->sid, max_bonds, factor, gap, pairs, bond_cap, n_out);
+    })
+}
+
+int dem_group_sums(dem_ctx* ctx, const int32_t* ids, int32_t n_ids, double* out) {
+    DEM_GUARD({
+        if (!ctx || !out || (n_ids > 0 && !ids)) { set_error("dem_group_sums: NULL argument"); return DEM_EINVAL; }
+        return ctx->eng.group_sums(ctx->sid, ids, n_ids, out);
     })
 }
 
@@ -195,12 +202,12 @@ int dem_last_step_timing(dem_ctx* ctx, int32_t enable, double* ms_out, int64_t* 
         psim::Engine& e = ctx->eng;
         if (ms_out) {
             ms_out[0] = e.t_total; ms_out[1] = e.t_step; ms_out[2] = e.t_scan; ms_out[3] = e.t_scatter;
-            ms_out[4] = e.t_reorder;
+            ms_out[4] = e.t_reorder; ms_out[5] = e.t_bonds;
         }
         if (launches_out) {
-            launches_out[0] = e.n_step + e.n_scan + e.n_scatter + e.n_reorder;
+            launches_out[0] = e.n_step + e.n_scan + e.n_scatter + e.n_reorder + e.n_bonds;
             launches_out[1] = e.n_step; launches_out[2] = e.n_scan; launches_out[3] = e.n_scatter;
-            launches_out[4] = e.n_reorder;
+            launches_out[4] = e.n_reorder; launches_out[5] = e.n_bonds;
         }
         if (enable >= 0) e.enable_timing(enable != 0);
         return DEM_OK;

@@ -140,7 +140,9 @@ public:
     int datum(int32_t sid, const int32_t* select, double* out);
     int debug_broad_phase(int32_t sid, int32_t* keys, int32_t* sorted_ids, int32_t* cell_start);
     int debug_contacts(int32_t sid, int32_t max_per, int32_t* counts, int32_t* partners);
-    int bond_pairs(int32_t sid, int32_t max_bonds, float factor, int32_t* pairs, int32_t cap, int32_t* n_out);
+    int bond_pairs(int32_t sid, int32_t max_bonds, float factor, float gap, int32_t* pairs, int32_t cap,
+                   int32_t* n_out);
+    int group_sums(int32_t sid, const int32_t* ids, int32_t n_ids, double* out);
     int sync();
 
     // timing of the last step() call
@@ -152,8 +154,8 @@ public:
     }
     int64_t graph_captures() const { return graph_captures_; }
     int64_t graph_replays() const { return graph_replays_; }
-    double t_total = 0, t_step = 0, t_scan = 0, t_scatter = 0, t_reorder = 0;
-    int64_t n_step = 0, n_scan = 0, n_scatter = 0, n_reorder = 0;
+    double t_total = 0, t_step = 0, t_scan = 0, t_scatter = 0, t_reorder = 0, t_bonds = 0;
+    int64_t n_step = 0, n_scan = 0, n_scatter = 0, n_reorder = 0, n_bonds = 0;
 
     hipStream_t stream() const { return stream_; }
     int device() const { return device_; }
@@ -225,6 +227,10 @@ private:
     int2* d_lf_ = nullptr;        // launch flags ring
     int64_t lf_cap_ = 0;          // entries (launches * scene_cap)
     double* d_ke_ = nullptr;      // per-id kinetic energy scratch
+    // group sums: 64 x 7 block partials + 7 totals (GROUP_SCRATCH_DOUBLES), then the ids
+    static constexpr int GROUP_SCRATCH_DOUBLES = 64 * 7 + 8;
+    void* d_group_ = nullptr;
+    int64_t group_bytes_ = 0;
     unsigned long long* d_cnt_ = nullptr;
     std::vector<SceneHost> scenes_;
     std::vector<std::pair<int64_t, int64_t>> used_slots_, used_cells_, used_bonds_, used_recs_, used_mats_;

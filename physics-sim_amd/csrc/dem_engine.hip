@@ -37,7 +37,9 @@ hipError_t launch_record_slots(const StepArgs& args, int nrec, int lbs, hipStrea
 hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, int np, double* out, hipStream_t st);
 hipError_t launch_count_contacts(const int32_t* m, int p0, int np, unsigned long long* out, hipStream_t st);
 hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
-                             int32_t* out_cnt, int32_t* out_j, hipStream_t st);
+                             float gap, int32_t* out_cnt, int32_t* out_j, hipStream_t st);
+hipError_t launch_group_sums(const DevArrays& a, int cur, int m0, int nm, int p0, const int32_t* ids, int n_ids,
+                             int with_data, double* partials, double* out, hipStream_t st);
 hipError_t launch_ke_reduce(double* buf, int n, double* host_out, hipStream_t st);
 double* static_scratch(int device);
 
@@ -118,6 +120,7 @@ Engine::~Engine() {
     hipFree(RS_); hipFree(RP_); hipFree(LB_);
     hipFree(STAT_); hipFree(TSUM_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
     hipFree(d_ke_); hipFree(d_cnt_);
+    if (d_group_) hipFree(d_group_);
     if (stage_) hipHostFree(stage_);
     for (auto e : ev_pool_) hipEventDestroy(e);
     if (stream_) hipStreamDestroy(stream_);
@@ -918,16 +921,16 @@ int Engine::run_ops(const std::vector<LaunchOp>& ops) {
             float ms = 0;
             hipEventElapsedTime(&ms, t0, t1);
             t_total = ms;
-            t_step = t_scan = t_scatter = t_reorder = 0;
-            n_step = n_scan = n_scatter = n_reorder = 0;
+            t_step = t_scan = t_scatter = t_reorder = t_bonds = 0;
+            n_step = n_scan = n_scatter = n_reorder = n_bonds = 0;
             for (auto& x : evs) {
                 float m = 0;
                 hipEventElapsedTime(&m, x.a, x.b);
                 switch (x.timed) {
-                    // "step": the step kernel with the bond pass before it, per step kernel
-                    case 0: t_step += m; n_step += x.kind == OP_STEP ? 1 : 0; break;
+                    case 0: t_step += m; n_step++; break;
                     case 1: t_scan += m; n_scan++; break;
                     case 2: t_scatter += m; n_scatter++; break;
+                    case 4: t_bonds += m; n_bonds++; break;
                     default: t_reorder += m; n_reorder++; break;
                 }
             }
@@ -1062,7 +1065,7 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
         }
         if (any_bonds && sim) {
             // bond records first: one thread per record, results read by the step kernel
-            LaunchOp& bp = new_op(ops, OP_BONDS, 0);
+            LaunchOp& bp = new_op(ops, OP_BONDS, 4);
             bp.sa = make_args(lfk);
             bp.ncells = (int)hw_recs_;
             bp.block = block_;
@@ -1320,6 +1323,46 @@ int Engine::kinetic_energy_noalloc(int32_t sid, double* out) {
     return DEM_OK;
 }
 
+int Engine::group_sums(int32_t sid, const int32_t* ids, int32_t n_ids, double* out) {
+    if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    const SceneHost& sh = scenes_[sid];
+    if (n_ids < 0) { set_error("group sums: n_ids < 0"); return DEM_EINVAL; }
+    for (int32_t k = 0; k < n_ids; k++)
+        if (ids[k] < 0 || ids[k] >= sh.n) {
+            set_error("group sums: id " + std::to_string(ids[k]) + " outside [0, " + std::to_string(sh.n) + ")");
+            return DEM_EINVAL;
+        }
+    // ids, then 64 x 7 partials and the 7 totals, in one buffer grown on demand
+    const int64_t need = (int64_t)(GROUP_SCRATCH_DOUBLES * sizeof(double)) + (int64_t)n_ids * sizeof(int32_t);
+    hipError_t e = hipSuccess;
+    if (group_bytes_ < need) {
+        if (d_group_) hipFree(d_group_);
+        d_group_ = nullptr;
+        group_bytes_ = 0;
+        e = hipMalloc(&d_group_, (size_t)need);
+        if (e != hipSuccess) return check(e, "group sums");
+        group_bytes_ = need;
+    }
+    double* partials = reinterpret_cast<double*>(d_group_);
+    double* total = partials + GROUP_SCRATCH_DOUBLES - 8;
+    int32_t* d_ids = reinterpret_cast<int32_t*>(partials + GROUP_SCRATCH_DOUBLES);
+    double host[8] = {0};
+    if (n_ids > 0) {
+        e = hipMemcpyAsync(d_ids, ids, sizeof(int32_t) * (size_t)n_ids, hipMemcpyHostToDevice, stream_);
+        const int rec = (with_data_ && sh.params.record_data) ? 1 : 0;
+        if (e == hipSuccess)
+            e = launch_group_sums(arrays(set_), cur_, sh.m0, sh.nm, (int)sh.p0, d_ids, n_ids, rec, partials, total,
+                                  stream_);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, total, sizeof(double) * 7, hipMemcpyDeviceToHost, stream_);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+        int rc = check(e, "group sums");
+        if (rc) return rc;
+    }
+    for (int c = 0; c < 7; c++) out[c] = host[c];
+    out[7] = (double)n_ids;
+    return DEM_OK;
+}
+
 int Engine::broken_bonds(int32_t sid, int64_t* out) {
     if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
     const SceneHost& sh = scenes_[sid];
@@ -1450,9 +1493,14 @@ int Engine::debug_contacts(int32_t sid, int32_t max_per, int32_t* counts, int32_
     return DEM_OK;
 }
 
-int Engine::bond_pairs(int32_t sid, int32_t max_bonds, float factor, int32_t* pairs, int32_t cap, int32_t* n_out) {
+int Engine::bond_pairs(int32_t sid, int32_t max_bonds, float factor, float gap, int32_t* pairs, int32_t cap,
+                       int32_t* n_out) {
     if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
     if (max_bonds <= 0 || max_bonds > 16) { set_error("max_bonds must be in [1, 16]"); return DEM_EINVAL; }
+    if (!(factor > 0.0f) || !(gap >= 0.0f) || !std::isfinite(factor) || !std::isfinite(gap)) {
+        set_error("bond pairs: factor must be > 0 and gap >= 0, both finite");
+        return DEM_EINVAL;
+    }
     const SceneHost& sh = scenes_[sid];
     const int n = sh.n;
     // bin the current positions (also refreshes the cell lists)
@@ -1464,7 +1512,7 @@ int Engine::bond_pairs(int32_t sid, int32_t max_bonds, float factor, int32_t* pa
     if ((rc = check(e, "bond pairs"))) return rc;
     float rmax = 0.0f;
     for (int i = 0; i < sh.np; i++) rmax = std::max(rmax, P[i].z);
-    const int reach = std::max(1, (int)std::ceil((double)factor * 2.0 * rmax / sh.params.grid.cell_size));
+    const int reach = std::max(1, (int)std::ceil(((double)factor * 2.0 * rmax + gap) / sh.params.grid.cell_size));
     int32_t* d_cnt = nullptr;
     int32_t* d_j = nullptr;
     e = dalloc(&d_cnt, std::max(n, 1));
@@ -1472,7 +1520,8 @@ int Engine::bond_pairs(int32_t sid, int32_t max_bonds, float factor, int32_t* pa
     std::vector<int2> lf(scene_cap_, make_int2(0, 0));
     lf[sid] = make_int2((int)SF_ACTIVE, 0);
     if (e == hipSuccess) e = hipMemcpyAsync(d_lf_, lf.data(), sizeof(int2) * scene_cap_, hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess) e = launch_bond_pairs(make_args(d_lf_), active_blocks(), block_, reach, max_bonds, factor, d_cnt, d_j, stream_);
+    if (e == hipSuccess) e = launch_bond_pairs(make_args(d_lf_), active_blocks(), block_, reach, max_bonds, factor, gap, d_cnt, d_j,
+                                                  stream_);
     std::vector<int32_t> cnt(std::max(n, 1)), jj((size_t)std::max(n, 1) * max_bonds);
     if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(jj.data(), d_j, sizeof(int32_t) * (size_t)n * max_bonds, hipMemcpyDeviceToHost, stream_);

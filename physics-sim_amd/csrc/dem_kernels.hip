@@ -412,8 +412,8 @@ __device__ __forceinline__ bool bonded_partner(const DevArrays& a, int b_lo, int
 // One bond endpoint record of the particle `me` (position p, scene S; partner in slot j) in a launch
 // whose force half runs: the narrow-phase check of the bonded partner, the break
 // state seen from the partner, the bond or contact law; writes the record (and the
-// bond's break state) back.  Returns {fx, fy, moment, 1}, or all 0 when the record
-// is dead or dies here (2D_Simulation.wgsl:185-204, 230-262).
+// bond's break state) back.  Returns {fx, fy, moment, 1}, all 0 when the record is
+// dead, or {0, 0, 0, -1} when it dies here (2D_Simulation.wgsl:185-204, 230-262).
 template <bool F64, class DD>
 __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const SceneDev& S, const Self& me,
                                                    const float4& p, bool narrow, int step, int cur, int q, int j,
@@ -432,12 +432,10 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
         if (!found && (bonded < 0 || u.bonds == 0)) {
             a.BR[q].w = 0;
             a.BRB[q].y = REC_DEAD;
-            // the partner may come back into contact: it returns to the neighbour
-            // lists, which leave out live bond partners, at the rebuild after this
-            // launch's drift, or after the next launch's if this one has none (both flags)
-            a.RB[0] = 1;
-            a.RB[1] = 1;
-            return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            // the partner may come back into contact: the owner's step kernel of this
+            // launch puts it into the owner's neighbour list (which leaves out live bond
+            // partners), told by the alive word -1
+            return make_float4(0.0f, 0.0f, 0.0f, -1.0f);
         } else if (!found && bonded > 0 && u.bonds <= 1) {
             bf.x = 0.0f;
         }
@@ -530,8 +528,8 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     // bond records: at most four are held in registers (partner ids, alive bits) for
     // the bonded-partner test of the contact search and the merge with the contacts
     // (loaded on first use: list candidates are never live bond partners, since the
-    // lists leave those out and a record's death forces a rebuild, so only the stencil
-    // fallback and the merge with contacts need the records)
+    // lists leave those out and take a partner in only when its record dies, so only
+    // the stencil fallback and the merge with contacts need the records)
     int rx[4] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
     int ralive = 0;
     bool rloaded = false;
@@ -621,9 +619,21 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         // record and break state); the owner adds them in partner order, merged with its
         // contacts
         int bq = b_lo;
+        // records that died in this launch's bond pass (bit q - b_lo; `dead_far`: one
+        // past the 32nd): their partners join this particle's neighbour list below
+        uint32_t dead = 0;
+        bool dead_far = false;
+        auto died = [&](int q) {
+            const int k = q - b_lo;
+            if (k < 32) dead |= 1u << k;
+            else dead_far = true;
+        };
         auto eval_record = [&](int q) {
             const float4 rf = a.RF[q];
-            if (rf.w == 0.0f) return;
+            if (!(rf.w > 0.0f)) {
+                if (rf.w < 0.0f) died(q);
+                return;
+            }
             nfx += rf.x;
             nfy += rf.y;
             nm += rf.z;
@@ -842,7 +852,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                                       m > 3 ? a.RF[bq + 3] : z};
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    if (k < m && rf[k].w != 0.0f) {
+                    if (k < m && rf[k].w > 0.0f) {
                         nfx += rf[k].x;
                         nfy += rf[k].y;
                         nm += rf[k].z;
@@ -850,9 +860,41 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                             replay(dd, a.RD[2 * (bq + k)]);
                             replay(dd, a.RD[2 * (bq + k) + 1]);
                         }
+                    } else if (k < m && rf[k].w < 0.0f) {
+                        died(bq + k);
                     }
                 }
                 bq += m;
+            }
+            // A record that died here (its bond broken and the discs apart) hands the pair
+            // to the contact search: the partner goes into this particle's list, in id
+            // order, so that the list again holds every disc that can touch it before the
+            // next rebuild (adding a candidate keeps a neighbour list valid).  A list that
+            // would overflow raises the rebuild flags instead (the rebuild after this
+            // launch's drift, or after the next launch's if this one has none); a particle
+            // on the stencil fallback (nc < 0) finds the partner in the cells.
+            if (dead != 0u || dead_far) {
+                if (dead_far || nc + __popc(dead) > NLMAX) {
+                    a.RB[0] = 1;
+                    a.RB[1] = 1;
+                } else if (nc >= 0) {
+                    int cnt = nc;
+                    int first = nl_first(w0);
+                    for (uint32_t mk = dead; mk != 0u; mk &= mk - 1u) {
+                        const int oj = a.BR[b_lo + __ffs(mk) - 1].x;
+                        const int js = a.INV[S.p0 + oj];
+                        int k = cnt - 1;
+                        for (; k >= 0; k--) {
+                            const int sk = k == 0 ? first : a.NL[k * cap + s];
+                            if (a.ID[sk] < oj) break;
+                            a.NL[(k + 1) * cap + s] = sk;
+                        }
+                        if (k < 0) first = js;
+                        else a.NL[(k + 1) * cap + s] = js;
+                        cnt++;
+                    }
+                    a.NL[s] = nl_word(cnt, first);
+                }
             }
         }
         // single-buffered: written only when it changes
@@ -1312,7 +1354,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
     const int pos0 = S.p0 - a.START[S.c0];
     const int rlo = max(cy - 1, 0), rhi = min(cy + 1, S.h - 1);
     // live bond partners stay out of the list: their records carry the pair until the
-    // record dies, which raises the rebuild flag
+    // record dies, when the step kernel adds the partner to the list
     const int2 brs = a.BRS[s];
     auto bonded = [&](int j) {
         if (brs.y <= brs.x) return false;
@@ -1599,6 +1641,54 @@ __global__ void k_kinetic(DevArrays a, int cur, int m0, int nm, int p0, int np, 
     if (m.y >= 0 && m.y < np) out_by_id[m.y] = e;
 }
 
+// Group sums (grip force and displacement of a tensile test): per listed id, the f64
+// terms {m ax, m ay, x, y, vx, vy, |data0|} with m the step's f32 mass; each block
+// reduces its ids in a fixed tree order, the partials are added in block order.
+constexpr int GROUP_BLOCKS = 64;
+constexpr int GROUP_TERMS = 7;
+__global__ void __launch_bounds__(256) k_group_partials(DevArrays a, int cur, int m0, int nm, int p0,
+                                                         const int32_t* ids, int n_ids, int with_data,
+                                                         double* partials) {
+    __shared__ double s_t[GROUP_TERMS][256];
+    double t[GROUP_TERMS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = (int)(blockIdx.x * 256 + threadIdx.x); k < n_ids; k += (int)(gridDim.x * 256)) {
+        const int slot = a.INV[p0 + ids[k]];
+        const int mat = a.M[slot] & PF_MAT_MASK;
+        const float4 p = a.P[cur][slot];
+        const float density = a.MAT[m0 + (mat < nm ? mat : 0)].density;
+        const float rr = p.z;
+        const float mass = density * PI_F * rr * rr;
+        const float4 acc = a.A[slot];
+        const float4 v = a.V[slot];
+        t[0] += (double)mass * (double)acc.x;
+        t[1] += (double)mass * (double)acc.y;
+        t[2] += (double)p.x;
+        t[3] += (double)p.y;
+        t[4] += (double)v.x;
+        t[5] += (double)v.y;
+        if (with_data) t[6] += fabs((double)a.DATA0[slot].x);
+    }
+#pragma unroll
+    for (int c = 0; c < GROUP_TERMS; c++) s_t[c][threadIdx.x] = t[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int c = 0; c < GROUP_TERMS; c++) s_t[c][threadIdx.x] += s_t[c][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < GROUP_TERMS) partials[blockIdx.x * GROUP_TERMS + threadIdx.x] = s_t[threadIdx.x][0];
+}
+
+__global__ void k_group_total(const double* partials, int nblocks, double* out) {
+    const int c = (int)threadIdx.x;
+    if (c >= GROUP_TERMS) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; b++) s += partials[b * GROUP_TERMS + c];
+    out[c] = s;
+}
+
 // Sum of the contact counts of a scene (mean contacts per particle) - integer,
 // order independent.  A grid of COUNT_BLOCKS strides over the scene, so only that
 // many same-address atomics meet in L2.
@@ -1618,9 +1708,10 @@ __global__ void __launch_bounds__(256) k_count_contacts(const int32_t* m, int p0
 }
 
 // ---- bond builder (State::regen_bonds rule on the cell list) ----
-// For particle id i: every j > i with sqrt((xj-xi)^2 + (yj-yi)^2) < (ri + rj) * factor,
+// For particle id i: every j > i with sqrt((xj-xi)^2 + (yj-yi)^2) < (ri + rj) * factor + gap,
 // the `maxb` smallest j kept.  Stencil of (2*reach+1)^2 cells.
-__global__ void k_bond_pairs(StepArgs args, int reach, int maxb, float factor, int32_t* out_cnt, int32_t* out_j) {
+__global__ void k_bond_pairs(StepArgs args, int reach, int maxb, float factor, float gap, int32_t* out_cnt,
+                             int32_t* out_j) {
     const DevArrays& a = args.a;
     const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int sid = args.blk_scene[blockIdx.x];
@@ -1649,7 +1740,7 @@ __global__ void k_bond_pairs(StepArgs args, int reach, int maxb, float factor, i
             const float4 q = a.P[args.cur][js];
             const float dx = q.x - p.x, dy = q.y - p.y;
             const float dist = sqrtf(dx * dx + dy * dy);
-            if (!(dist < (p.z + q.z) * factor)) continue;
+            if (!(dist < (p.z + q.z) * factor + gap)) continue;
             // keep the maxb smallest j, ascending
             int pos = 0;
             for (int k = 0; k < n; k++) pos += best[k] < j ? 1 : 0;
@@ -1790,8 +1881,20 @@ hipError_t launch_count_contacts(const int32_t* m, int p0, int np, unsigned long
 }
 
 hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
-                             int32_t* out_cnt, int32_t* out_j, hipStream_t st) {
-    hipLaunchKernelGGL(k_bond_pairs, dim3(nblocks), dim3(block), 0, st, args, reach, maxb, factor, out_cnt, out_j);
+                             float gap, int32_t* out_cnt, int32_t* out_j, hipStream_t st) {
+    hipLaunchKernelGGL(k_bond_pairs, dim3(nblocks), dim3(block), 0, st, args, reach, maxb, factor, gap, out_cnt,
+                       out_j);
+    return hipGetLastError();
+}
+
+// partials: GROUP_BLOCKS * GROUP_TERMS doubles; out: GROUP_TERMS doubles (device)
+hipError_t launch_group_sums(const DevArrays& a, int cur, int m0, int nm, int p0, const int32_t* ids, int n_ids,
+                             int with_data, double* partials, double* out, hipStream_t st) {
+    int nb = (n_ids + 255) / 256;
+    nb = nb < 1 ? 1 : (nb > GROUP_BLOCKS ? GROUP_BLOCKS : nb);
+    hipLaunchKernelGGL(k_group_partials, dim3(nb), dim3(256), 0, st, a, cur, m0, nm, p0, ids, n_ids, with_data,
+                       partials);
+    hipLaunchKernelGGL(k_group_total, dim3(1), dim3(64), 0, st, (const double*)partials, nb, out);
     return hipGetLastError();
 }
 

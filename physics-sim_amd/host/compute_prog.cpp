@@ -128,13 +128,13 @@ int64_t ComputeProg::broken_bond_count() {
     return k;
 }
 
-State& ComputeProg::regen_bonds(const Settings& settings, float factor) {
+State& ComputeProg::regen_bonds(const Settings& settings, float factor, float gap) {
     State& st = update_state();
     const int32_t n = st.p_count;
     const int32_t cap = std::max(n * settings.setup.max_bonds, 1);
     std::vector<int32_t> pairs(2 * (size_t)cap);
     int32_t nout = 0;
-    check(dem_build_bond_pairs(ctx_, settings.setup.max_bonds, factor, pairs.data(), cap, &nout));
+    check(dem_build_bond_pairs(ctx_, settings.setup.max_bonds, factor, gap, pairs.data(), cap, &nout));
     pairs.resize(2 * (size_t)nout);
     assemble_bonds(st, pairs, settings.setup.max_contacts, st.bonds, st.contacts);
     st.bond_strengths.clear();

@@ -173,8 +173,9 @@ public:
     void set_params(const Settings& settings);
     double kinetic_energy();
     int64_t broken_bond_count();
-    // State::regen_bonds with the pair search on the device (dem_build_bond_pairs)
-    State& regen_bonds(const Settings& settings, float factor = 1.05f);
+    // State::regen_bonds with the pair search on the device (dem_build_bond_pairs):
+    // bonded iff |d| < (ri + rj) * factor + gap (the reference: factor 1.05, gap 0)
+    State& regen_bonds(const Settings& settings, float factor = 1.05f, float gap = 0.0f);
 
     const dem_grid& grid() const { return grid_; }
 

@@ -120,7 +120,7 @@ EXPORTS = [
     "dem_upload_state", "dem_step", "dem_download_state", "dem_sync", "dem_kinetic_energy",
     "dem_kinetic_energy_noalloc",
     "dem_broken_bond_count", "dem_counters", "dem_get_datum", "dem_debug_broad_phase",
-    "dem_debug_contacts", "dem_build_bond_pairs", "dem_last_step_timing",
+    "dem_debug_contacts", "dem_build_bond_pairs", "dem_group_sums", "dem_last_step_timing",
     "dem_server_create", "dem_server_destroy", "dem_server_submit", "dem_server_tick", "dem_server_status",
     "dem_server_take", "dem_server_stats",
 ]
@@ -159,7 +159,8 @@ def load_library(path: str = LIB_PATH):
     lib.dem_get_datum.argtypes = [C.c_void_p, IP, C.POINTER(C.c_double)]
     lib.dem_debug_broad_phase.argtypes = [C.c_void_p, IP, IP, IP]
     lib.dem_debug_contacts.argtypes = [C.c_void_p, C.c_int32, IP, IP]
-    lib.dem_build_bond_pairs.argtypes = [C.c_void_p, C.c_int32, C.c_float, IP, C.c_int32, IP]
+    lib.dem_build_bond_pairs.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_float, IP, C.c_int32, IP]
+    lib.dem_group_sums.argtypes = [C.c_void_p, IP, C.c_int32, C.POINTER(C.c_double)]
     lib.dem_last_step_timing.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     lib.dem_server_create.argtypes = [C.POINTER(DemServerParams), C.POINTER(C.c_void_p)]
     lib.dem_server_destroy.argtypes = [C.c_void_p]

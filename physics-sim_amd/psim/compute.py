@@ -154,11 +154,12 @@ class ComputeProg:
         return np.array(out[:], dtype=np.float64)
 
     def timing(self, enable: Optional[bool] = None):
-        ms = (C.c_double * 5)()
-        cnt = (C.c_int64 * 5)()
+        ms = (C.c_double * 6)()
+        cnt = (C.c_int64 * 6)()
         abi.check(self.lib.dem_last_step_timing(self.ctx, -1 if enable is None else int(enable), ms, cnt))
         return {"total_ms": ms[0], "step_ms": ms[1], "scan_ms": ms[2], "scatter_ms": ms[3], "reorder_ms": ms[4],
-                "launches": cnt[0], "n_step": cnt[1], "n_scan": cnt[2], "n_scatter": cnt[3], "n_reorder": cnt[4]}
+                "bonds_ms": ms[5], "launches": cnt[0], "n_step": cnt[1], "n_scan": cnt[2], "n_scatter": cnt[3],
+                "n_reorder": cnt[4], "n_bonds": cnt[5]}
 
     # -- parity views ------------------------------------------------------------
     def debug_broad_phase(self):
@@ -177,25 +178,27 @@ class ComputeProg:
         abi.check(self.lib.dem_debug_contacts(self.ctx, max_per, abi.iptr(counts), abi.iptr(partners)))
         return counts, partners.reshape(n, max_per)
 
-    def bond_pairs(self, max_bonds: int = 6, factor: float = 1.05, cap: Optional[int] = None) -> np.ndarray:
+    def bond_pairs(self, max_bonds: int = 6, factor: float = 1.05, cap: Optional[int] = None,
+                   gap: float = 0.0) -> np.ndarray:
         n = self.state.p_count
         cap = int(cap if cap is not None else max(n * max_bonds, 1))
         pairs = np.zeros(2 * cap, dtype=np.int32)
         nout = C.c_int32()
-        abi.check(self.lib.dem_build_bond_pairs(self.ctx, int(max_bonds), C.c_float(factor), abi.iptr(pairs),
-                                                cap, C.byref(nout)))
+        abi.check(self.lib.dem_build_bond_pairs(self.ctx, int(max_bonds), C.c_float(factor), C.c_float(gap),
+                                                abi.iptr(pairs), cap, C.byref(nout)))
         return pairs[: 2 * nout.value].reshape(-1, 2)
 
     def regen_bonds(self, settings: Optional[Settings] = None, factor: float = 1.05,
-                    strengths: Optional[np.ndarray] = None) -> State:
+                    strengths: Optional[np.ndarray] = None, gap: float = 0.0) -> State:
         """State::regen_bonds (state.rs:291-367) with the pair search on the device:
-        bond (i, j), j > i, when |pj - pi| < (ri + rj) * 1.05, at most max_bonds per
-        lower index in increasing j; bond = {1, atan2(dx, dy) bits, |d| bits}; each
-        endpoint gets {a, b, 0, 0, 0, bond index} in its first free slot.  The bonds
-        and slots are assembled on the host and the scene is re-uploaded."""
+        bond (i, j), j > i, when |pj - pi| < (ri + rj) * factor + gap (the reference:
+        factor 1.05, gap 0), at most max_bonds per lower index in increasing j;
+        bond = {1, atan2(dx, dy) bits, |d| bits}; each endpoint gets
+        {a, b, 0, 0, 0, bond index} in its first free slot.  The bonds and slots are
+        assembled on the host and the scene is re-uploaded."""
         s = settings or self.settings
         st = self.update_state()
-        pairs = self.bond_pairs(s.setup.max_bonds, factor)
+        pairs = self.bond_pairs(s.setup.max_bonds, factor, gap=gap)
         st.bonds, st.contacts = assemble_bonds(st, pairs, s.setup.max_contacts)
         st.bond_strengths = strengths
         nb = st.n_bonds
@@ -204,6 +207,18 @@ class ComputeProg:
                                % (self.bond_capacity, nb))
         self.restore(st)
         return st
+
+    def group_sums(self, ids) -> dict:
+        """Sums over the particles `ids` computed on the device (dem_group_sums): the
+        net force on the group (Σ m a, m the step's f32 mass), Σ position, Σ velocity,
+        Σ |normal-force datum| and the count.  The grip force and displacement of a
+        tensile test; the reference reads the state back for the same sums
+        (State::get_datum, state.rs:708-737)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        out = (C.c_double * 8)()
+        abi.check(self.lib.dem_group_sums(self.ctx, abi.iptr(ids), len(ids), out))
+        return {"force": (out[0], out[1]), "pos": (out[2], out[3]), "vel": (out[4], out[5]),
+                "normal_force": out[6], "count": int(out[7])}
 
     # -- experiment-script engine interface (psim.scripts.ScriptReplay) ----------
     @property

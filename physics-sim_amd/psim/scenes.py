@@ -21,8 +21,12 @@ same state.
                    disc (each resting on two discs below it), centred on the floor.
   bonded_chains    horizontal chains of r = 0.5 discs at spacing 1.0, neighbours
                    bonded (parallel bonds), ends pinned, hit by a dense r = 8 disc.
-  bonded_assembly  dense packing of r ~ U[0.8, 1.2] discs bonded where the gap is
-                   below 0.08 (0.1 r_min), top and bottom rows pulled apart.
+  bonded_assembly  1,048,576 discs, r ~ U[0.8, 1.2]: a seeded triangular lattice at
+                   area fraction 0.86 relaxed on the device into a dense disordered
+                   packing, every pair closer than 0.08 (0.1 r_min) bonded (parallel
+                   bonds, Weibull(4, 30) strengths), bottom and top rows gripped and
+                   pulled apart at 1e-3 strain per 1,000 steps from the affine
+                   velocity field of that rate.  Needs a GPU.
 """
 from __future__ import annotations
 
@@ -265,37 +269,133 @@ def serving_stream(n_scenes: int = 96, rate: float = 50.0, seed: int = 4, n_min:
     return out
 
 
-def bonded_assembly(n_side: int = 32, seed: int = 3, spacing: float = 2.0, rmin: float = 0.95, rmax: float = 1.05,
-                    pull_v: float = 0.5, dt: float = 1e-4, kn: float = 1e5, ks: float = 1e5, mu: float = 0.5):
-    """Bonded-particle assembly for a tensile test: an n_side x n_side lattice of discs
-    whose neighbours nearly touch (r ~ U[rmin, rmax] at spacing 2.0, so neighbour gaps
-    lie within +-0.1), parallel bonds (mode 3, tearing on), no gravity; the bottom row
-    and the top row are grips: their y velocity is fixed (fixity y = 1) at -pull_v and
-    +pull_v.  Bonds come from the bond builder (State::regen_bonds rule, factor 1.05)
-    with per-bond strengths drawn afterwards (Weibull, see `assembly_strengths`).
-    Returns (settings, state, bottom-grip ids, top-grip ids)."""
+def assembly_packing(rows: int = 1024, per_row: int = 1024, seed: int = 3, rmin: float = 0.8, rmax: float = 1.2,
+                     phi: float = 0.86, dt: float = 1e-4):
+    """The unrelaxed packing of the bonded assembly: `rows` x `per_row` discs,
+    r ~ U[rmin, rmax] (seeded), on a triangular lattice whose spacing a gives the
+    area fraction `phi` (a^2 = 2 pi <r^2> / (sqrt(3) phi)); at phi = 0.86, above the
+    jamming fraction of this radius law (about 0.84), neighbours overlap by up to
+    2 r_max - a.  The settings are those of the relaxation: walls at the lattice's
+    extent, no gravity, frictionless contacts with restitution 0.05 (kn_eff 1e5).
+    Row 0 is the bottom row, row rows-1 the top; particle id = row * per_row + column.
+    Returns (settings, state, spacing)."""
     rng = np.random.default_rng(seed)
-    n = n_side * n_side
-    box = n_side * spacing * 4.0
-    s = _base_settings(n, box, box, dt, rmax, rmin, kn=2 * kn, ks=2 * ks, mu=mu)
+    n = rows * per_row
+    r = rng.uniform(rmin, rmax, n).astype(np.float32)
+    a = math.sqrt(2.0 * math.pi * float(np.mean(r.astype(np.float64) ** 2)) / (math.sqrt(3.0) * phi))
+    width = (per_row + 0.5) * a
+    height = rows * a * math.sqrt(3.0) / 2.0
+    s = _base_settings(n, width, height, dt, rmax, rmin, mu=0.0)
+    s.physics.gravity = False
+    s.physics.restitution = 0.05
+    st = _new_state(n)
+    j, i = np.divmod(np.arange(n), per_row)
+    st.pos[0:2 * n:2] = (-width / 2 + a * (i + 0.5 + 0.5 * (j % 2))).astype(np.float32)
+    st.pos[1:2 * n:2] = (-height / 2 + a * math.sqrt(3.0) / 2.0 * (j + 0.5)).astype(np.float32)
+    st.radii[:n] = r
+    return s, st, a
+
+
+def bonded_assembly(rows: int = 1024, per_row: int = 1024, seed: int = 3, rmin: float = 0.8, rmax: float = 1.2,
+                    phi: float = 0.89, relax_steps: int = 3000, quench_every: int = 500, gap_frac: float = 0.1,
+                    kn: float = 1500.0, ks: float = 1500.0, mu: float = 0.5, dt: float = 1e-4,
+                    strain_per_1000: float = 1e-3, strength_k: float = 4.0, strength_scale: float = 30.0,
+                    margin: float = 0.25):
+    """Bonded-particle assembly for a tensile test (1,048,576 discs at the default size).
+
+    1. assembly_packing: a seeded triangular lattice of r ~ U[0.8, 1.2] discs at area
+       fraction 0.86, relaxed on the device (ComputeProg: frictionless, no gravity,
+       walls at the lattice's extent) for `relax_steps` steps, velocities zeroed every
+       `quench_every` steps: a dense disordered packing.
+    2. Every pair closer than gap_frac * rmin (0.08) is bonded (the device bond
+       builder, rule |d| < r_a + r_b + 0.08, at most 6 bonds per lower id as in
+       State::regen_bonds); a bond's rest length is the distance at bonding, so the
+       assembly starts free of force.  Parallel bonds (mode 3, tearing on),
+       strengths {normal, shear} ~ Weibull(strength_k, strength_scale) per bond.
+       Bond stiffness kn = ks = 1500: a bond tears in tension at kn * delta = strength,
+       so the median bond fails near 1 % strain of its length (30 / (1500 * 2)), the
+       failure strain of the reference's own tensile fixture
+       (saved_states/PFC_assembly_test.bin: strength 1e7 over stiffness 8e11 on bonds
+       1.24e-3 long).  Contacts: the same linear law with material kn = ks = 2 kn
+       (series stiffness kn), mu = 0.5.
+    3. Grips: the bottom and top lattice rows, their y velocity prescribed (fixity y).
+       They move apart at the strain rate `strain_per_1000` per 1,000 steps; every
+       disc starts with the affine velocity field of that rate, v_y = rate (y - y_mid),
+       so the pull starts without a shock at the grips (a grip started alone from rest
+       sends a strain of v / c through the bonds: at this size v_grip is about 9 and
+       the bonds' wave speed about 40, far above their failure strain).  The grip
+       rows move at exactly -+v_grip.
+    The test box is larger than the specimen by `margin` of its size on each side.
+    Needs a GPU (steps 1-2 run through ComputeProg).  Returns (settings, state,
+    bottom ids, top ids, info) with info = {bonds, spacing, L0 (grip separation),
+    width, strain_rate (per unit time), v_grip}."""
+    from .compute import ComputeProg
+    s0, st, a = assembly_packing(rows, per_row, seed, rmin, rmax, phi, dt)
+    n = st.p_count
+    s0.setup.max_bonds = 6
+    prog = ComputeProg(s0, st, bond_capacity=6 * n, record_data=False, reorder_interval=64)
+    try:
+        done = 0
+        while done < relax_steps:
+            k = min(quench_every, relax_steps - done)
+            prog.step(k)
+            done += k
+            cur = prog.update_state()
+            cur.vel[:] = 0.0
+            cur.rot_vel[:] = 0.0
+            prog.restore(cur)
+        built = prog.regen_bonds(s0, factor=1.0, gap=gap_frac * rmin)
+    finally:
+        prog.close()
+    st = built.copy()
+    nb = st.n_bonds
+    width0, height0 = float(s0.simulation.hor_bound), float(s0.simulation.vert_bound)
+    s = _base_settings(n, width0 * (1 + 2 * margin), height0 * (1 + 2 * margin), dt, rmax, rmin,
+                       kn=kn, ks=ks, mu=mu)
     s.physics.gravity = False
     s.physics.bonds = 3
     s.physics.bond_tearing = True
     s.physics.bond_normal_stiffness = f32(kn)
     s.physics.bond_shear_stiffness = f32(ks)
-    st = _new_state(n)
-    ii, jj = np.meshgrid(np.arange(n_side), np.arange(n_side), indexing="xy")
-    st.pos[0:2 * n:2] = (-n_side * spacing / 2 + spacing / 2 + ii.ravel() * spacing).astype(np.float32)
-    st.pos[1:2 * n:2] = (-n_side * spacing / 2 + spacing / 2 + jj.ravel() * spacing).astype(np.float32)
-    st.radii[:n] = rng.uniform(rmin, rmax, n).astype(np.float32)
-    bottom = np.nonzero(jj.ravel() == 0)[0]
-    top = np.nonzero(jj.ravel() == n_side - 1)[0]
+    s.setup.max_bonds = 6
+    st.vel[:] = 0.0
+    st.rot_vel[:] = 0.0
+    st.acc[:] = 0.0
+    st.rot_acc[:] = 0.0
+    st.bond_strengths = assembly_strengths(nb, seed, strength_k, strength_scale)
+    bottom = np.arange(per_row)
+    top = np.arange((rows - 1) * per_row, rows * per_row)
+    y = st.pos[1:2 * n:2].astype(np.float64)
+    yb, yt = float(y[bottom].mean()), float(y[top].mean())
+    rate = strain_per_1000 / (1000.0 * dt)              # strain per unit time
+    ym = 0.5 * (yb + yt)
+    st.vel[1:2 * n:2] = (rate * (y - ym)).astype(np.float32)
+    v_grip = 0.5 * rate * (yt - yb)
+    st.vel[2 * bottom + 1] = np.float32(-v_grip)
+    st.vel[2 * top + 1] = np.float32(v_grip)
     fx = st.fixity.reshape(-1, 6)
-    for rows, v in ((bottom, -pull_v), (top, pull_v)):
-        fx[rows, 1] = 1
-        fx[rows, 4] = 1
-        st.vel[2 * rows + 1] = np.float32(v)
-    return s, st, bottom, top
+    for rows_ in (bottom, top):
+        fx[rows_, 1] = 1
+        fx[rows_, 4] = 1
+    info = {"bonds": nb, "spacing": a, "L0": yt - yb, "width": width0, "strain_rate": rate,
+            "v_grip": v_grip, "relax_steps": relax_steps}
+    return s, st, bottom, top, info
+
+
+def grip_stress(prog, bottom, top, width: float) -> float:
+    """Tensile stress on the grips from the device group sums: the grips' net force
+    (Σ m a_y over each grip row, the pull of the specimen on it) averaged over the two
+    grips, per unit width (discs are unit-thickness slabs)."""
+    fb = prog.group_sums(bottom)["force"][1]
+    ft = prog.group_sums(top)["force"][1]
+    return 0.5 * (fb - ft) / width
+
+
+def grip_strain(prog, bottom, top, L0: float) -> float:
+    """Engineering strain between the grips: mean y of the top row minus that of the
+    bottom row, over its value at the start."""
+    gb, gt = prog.group_sums(bottom), prog.group_sums(top)
+    return (gt["pos"][1] / gt["count"] - gb["pos"][1] / gb["count"]) / L0 - 1.0
 
 
 def assembly_strengths(n_bonds: int, seed: int = 3, k: float = 4.0, scale: float = 30.0):

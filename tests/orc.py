@@ -54,7 +54,8 @@ def lib():
         L.orc_contact_pairs.restype = C.c_int64
         L.orc_contact_pairs_grid.argtypes = [C.c_int32, FP, FP, C.c_int32, IP, IP]
         L.orc_contact_pairs_grid.restype = C.c_int64
-        L.orc_regen_bonds.argtypes = [C.c_int32, FP, FP, C.c_int32, C.c_int32, IP, C.c_int64, FP]
+        L.orc_regen_bonds.argtypes = [C.c_int32, FP, FP, C.c_int32, C.c_int32, IP, C.c_int64, FP, C.c_float,
+                                      C.c_float]
         L.orc_regen_bonds.restype = C.c_int64
         L.orc_kinetic_energy.argtypes = [C.c_int32, FP, FP, FP, IP, C.POINTER(abi.DemMaterial), C.c_int32]
         L.orc_kinetic_energy.restype = C.c_double

@@ -59,7 +59,7 @@ def test_assemble_bonds_equals_python():
     bonds_o = np.zeros(3 * 6 * n, dtype=np.int32)
     cont_o = np.zeros(14 * 6 * n, dtype=np.float32)
     nb = orc.lib().orc_regen_bonds(n, st.pos.ctypes.data_as(orc.FP), st.radii.ctypes.data_as(orc.FP), 6, 14,
-                                   bonds_o.ctypes.data_as(orc.IP), 6 * n, cont_o.ctypes.data_as(orc.FP))
+                                   bonds_o.ctypes.data_as(orc.IP), 6 * n, cont_o.ctypes.data_as(orc.FP), 1.05, 0.0)
     c = cont_o.view(np.int32).reshape(n, 14, 6)
     pairs = sorted((c[i, k, 5], i, c[i, k, 1]) for i in range(n) for k in range(14) if c[i, k, 0] == i and c[i, k, 1] > i)
     pairs = np.array([(i, j) for _, i, j in pairs], dtype=np.int32)

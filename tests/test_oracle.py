@@ -185,8 +185,12 @@ def test_energy_conserved_frictionless():
     assert abs(e1 - e0) < 0.02 * e0
 
 
-def test_regen_bonds_bruteforce():
-    """State::regen_bonds restated (oracle) equals an independent numpy pair scan."""
+@pytest.mark.parametrize("factor,gap", [(1.05, 0.0), (1.0, 0.08)])
+def test_regen_bonds_bruteforce(factor, gap):
+    """State::regen_bonds restated (oracle) equals an independent numpy pair scan, with
+    the reference's rule (factor 1.05) and with a gap rule (every pair closer than
+    0.08, the bonded-assembly rule); for the gap rule the pairs also agree with the
+    f64 geometry away from the threshold's f32 rounding."""
     rng = np.random.default_rng(3)
     n = 60
     pos = rng.uniform(-6, 6, 2 * n).astype(np.float32)
@@ -194,7 +198,7 @@ def test_regen_bonds_bruteforce():
     bonds = np.zeros(3 * 400, dtype=np.int32)
     cont = np.zeros(14 * 6 * n, dtype=np.float32)
     nb = orc.lib().orc_regen_bonds(n, pos.ctypes.data_as(orc.FP), r.ctypes.data_as(orc.FP), 6, 14,
-                                   bonds.ctypes.data_as(orc.IP), 400, cont.ctypes.data_as(orc.FP))
+                                   bonds.ctypes.data_as(orc.IP), 400, cont.ctypes.data_as(orc.FP), factor, gap)
     want = []
     for i in range(n):
         c = 0
@@ -202,10 +206,15 @@ def test_regen_bonds_bruteforce():
             dx = np.float32(pos[2 * j] - pos[2 * i])
             dy = np.float32(pos[2 * j + 1] - pos[2 * i + 1])
             d = np.sqrt(np.float32(dx * dx + dy * dy))
-            if d < np.float32(np.float32(r[i] + r[j]) * np.float32(1.05)):
+            if d < np.float32(np.float32(r[i] + r[j]) * np.float32(factor)) + np.float32(gap):
                 if c < 6:
                     want.append((i, j, d))
                     c += 1
+            if gap > 0 and c < 6:
+                g64 = np.hypot(float(pos[2 * j]) - float(pos[2 * i]), float(pos[2 * j + 1]) - float(pos[2 * i + 1])) \
+                    - float(r[i]) - float(r[j])
+                if abs(g64 - gap) > 1e-5:
+                    assert ((i, j, d) in want) == (g64 < gap)
     assert nb == len(want) > 20
     tab = cont.view(np.int32).reshape(n, 14, 6)
     for k, (i, j, d) in enumerate(want):
