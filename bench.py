@@ -84,7 +84,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, scenes/s")
     ap.add_argument("--serve-substeps", type=int, default=50)
     ap.add_argument("--no-bonded", action="store_true", help="skip the 1M-disc bonded-assembly leg")
-    ap.add_argument("--bonded-steps", type=int, default=100)
+    ap.add_argument("--bonded-steps", type=int, default=50000,
+                    help="steps of the bonded-assembly pull (50,000: strain 0.05)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also time a plain device copy of this many MiB (the reference kernel for the HBM "
                          "traffic counters and the achievable copy bandwidth)")
@@ -187,45 +188,93 @@ def serving_bench(args):
 
 
 def bonded_bench(args):
-    """The bonded-particle assembly at 1,048,576 discs (a 1024 x 1024 lattice at spacing
-    2.0, r ~ U[0.95, 1.05]; every pair closer than 1.05 (r_a + r_b) bonded by the device
-    bond builder, parallel bonds with Weibull(k = 4, 30) strengths; bottom and top rows
-    gripped and pulled apart): bond-builder time, then particle-steps/s of the bonded
-    step over --bonded-steps steps."""
+    """The bonded-assembly tensile test at 1,048,576 discs (scenes.bonded_assembly: a
+    seeded triangular lattice of r ~ U[0.8, 1.2] discs at area fraction 0.89 relaxed
+    on the device, every pair closer than 0.08 = 0.1 r_min bonded by the device bond
+    builder, parallel bonds with Weibull(4, 30) strengths, bottom and top rows gripped
+    and pulled apart at 1e-3 strain per 1,000 steps): --bonded-steps steps (50,000:
+    strain 0.05) with the stress-strain trace and the broken-bond count sampled from
+    the device every 1,000 steps (dem_group_sums on the two grips,
+    dem_broken_bond_count), then one 1,000-step window under HIP-event timing for the
+    per-kernel times and the roofline of the bond pass and the bonded step kernel."""
     from psim import scenes
     from psim.compute import ComputeProg
-    s, st, bottom, top = scenes.bonded_assembly(n_side=1024, pull_v=0.02)
-    n = st.p_count
-    prog = ComputeProg(s, st, bond_capacity=6 * n, record_data=False, reorder_interval=args.reorder)
     t = time.perf_counter()
-    built = prog.regen_bonds(s)
+    s, st, bottom, top, info = scenes.bonded_assembly()
     t_build = time.perf_counter() - t
-    built.bond_strengths = scenes.assembly_strengths(built.n_bonds)
-    prog.restore(built)
-    prog.step(10)
-    prog.sync()
-    t = time.perf_counter()
-    prog.step(args.bonded_steps)
-    prog.sync()
-    wall = time.perf_counter() - t
+    n, nb = st.p_count, info["bonds"]
+    prog = ComputeProg(s, st, bond_capacity=nb, record_data=False, reorder_interval=args.reorder)
+    every = 1000
+    trace = []
+
+    def sample(done):
+        gb, gt = prog.group_sums(bottom), prog.group_sums(top)
+        strain = (gt["pos"][1] / gt["count"] - gb["pos"][1] / gb["count"]) / info["L0"] - 1.0
+        stress = 0.5 * (gb["force"][1] - gt["force"][1]) / info["width"]
+        trace.append([done, strain, stress, prog.broken_bond_count()])
+
+    # the first 1,000 steps under HIP-event timing (the bonds still intact: the window
+    # of the per-kernel figures), the rest timed by the host clock
+    c_a = prog.counters()
     prog.timing(True)
-    prog.step(args.bonded_steps)
+    prog.step(every)
     prog.sync()
     tm = prog.timing(False)
-    c = prog.counters()
+    c_b = prog.counters()
+    sample(every)
+    done = every
+    prog.sync()
+    t0 = time.perf_counter()
+    while done < args.bonded_steps:
+        k = min(every, args.bonded_steps - done)
+        prog.step(k)
+        done += k
+        sample(done)
+        if done % 10000 == 0:
+            print("bench: bonded pull %d / %d steps" % (done, args.bonded_steps), file=sys.stderr, flush=True)
+    prog.sync()
+    wall = time.perf_counter() - t0
+    c0 = prog.counters()
     broken = prog.broken_bond_count()
     prog.close()
-    nc = c["contacts"] / n
-    nb_end = 2.0 * built.n_bonds / n
+    timed_steps = max(args.bonded_steps - every, 1)
+    nc = c_b["contacts"] / n
+    nrec = c_b["bond_endpoints"]
     k_ms = tm["step_ms"] / max(tm["n_step"], 1)
-    k_bytes = n * (60 + 16 * nc + 32 * nb_end)
-    return {"workload": "bonded assembly, %d discs, %d bonds (parallel bonds, Weibull(4, 30) strengths), "
-                        "rows gripped and pulled at 0.02 units/s, dt 1e-4" % (n, built.n_bonds),
-            "value": n * args.bonded_steps / wall, "unit": "particle-steps/s", "steps": args.bonded_steps,
-            "ms_per_step": 1e3 * wall / args.bonded_steps, "bond_builder_s": t_build,
-            "bond_endpoints_per_particle": nb_end, "mean_contacts_per_particle": nc,
-            "kernel_ms": {"step": k_ms}, "step_kernel_alg_gbs": k_bytes / (k_ms * 1e-3) / 1e9,
-            "broken_bonds": int(broken)}
+    b_ms = tm["bonds_ms"] / max(tm["n_bonds"], 1)
+    # algorithmic bytes: the bond pass reads and writes each live record's persistent
+    # state {ft, fbt, theta_b} (2 x 12 B) and reads its bond's rest length and strengths
+    # (8 B, shared by the endpoints); the step kernel reads and writes each disc's state
+    # (48 B read, 32 B written incl. the contact count) and its contacts (16 B each).
+    rec_bytes = 32.0 * nrec
+    step_bytes = n * (60.0 + 16.0 * nc)
+    peak = 8.0e12
+    peak_unit = 1e9
+    return {"workload": "bonded-assembly tensile test, %d discs r ~ U[0.8, 1.2] (lattice at area fraction 0.89 "
+                        "relaxed on the device), %d parallel bonds (gap < 0.08, Weibull(4, 30) strengths, "
+                        "kn = ks = 1500), grips pulled at 1e-3 strain per 1,000 steps, dt 1e-4, f32"
+                        % (n, nb),
+            "value": n * timed_steps / wall, "unit": "particle-steps/s", "steps": args.bonded_steps,
+            "ms_per_step": 1e3 * wall / timed_steps,
+            "timed": "steps 1,000 to the end by the host clock, with the trace sampling",
+            "scene_build_s": t_build,
+            "bonds": nb, "bonds_per_disc": 2.0 * nb / n, "grip_separation": info["L0"],
+            "specimen_width": info["width"], "v_grip": info["v_grip"],
+            "broken_bonds_end": int(broken), "list_rebuilds": c0["broad_phases"], "resorts": c0["reorders"],
+            "window": {"steps": every, "first_step": 0, "live_bond_endpoints": nrec,
+                       "mean_contacts_per_particle": nc,
+                       "list_rebuilds": c_b["broad_phases"] - c_a["broad_phases"],
+                       "kernel_ms_per_step": {"bond_pass": b_ms, "step": k_ms,
+                                              "rest": (tm["total_ms"] - tm["step_ms"] - tm["bonds_ms"]) / every}},
+            "roofline": {"bond_pass": {"bound": "hbm", "achieved": rec_bytes / (b_ms * 1e-3) / peak_unit,
+                                       "peak": peak / peak_unit, "unit": "GB/s",
+                                       "frac": rec_bytes / (b_ms * 1e-3) / peak,
+                                       "alg_bytes": rec_bytes},
+                         "step": {"bound": "hbm", "achieved": step_bytes / (k_ms * 1e-3) / peak_unit,
+                                  "peak": peak / peak_unit, "unit": "GB/s", "frac": step_bytes / (k_ms * 1e-3) / peak,
+                                  "alg_bytes": step_bytes}},
+            "stress_strain": {"columns": ["step", "strain", "grip_stress", "broken_bonds"], "every": every,
+                              "rows": trace}}
 
 
 def graph_bench(args):
