@@ -80,7 +80,7 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --block and --cell as given instead of tuning them on the workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
-    ap.add_argument("--serve-scenes", type=int, default=96)
+    ap.add_argument("--serve-scenes", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, scenes/s")
     ap.add_argument("--serve-substeps", type=int, default=50)
     ap.add_argument("--no-bonded", action="store_true", help="skip the 1M-disc bonded-assembly leg")
@@ -136,11 +136,14 @@ def copy_calibration(mib, reps=5):
 
 def serving_bench(args):
     """Continuous batching against plain one-scene-at-a-time serving on the same
-    stream (half granular boxes, half bonded chains; log-uniform sizes; seeded
-    per-scene friction and bond strengths).  Each mode runs twice: with Poisson
-    arrivals at --serve-rate (latency under that load) and with every scene queued
-    at t = 0 (service capacity).  "plain" is the same server admitting one scene at
-    a time, so the difference is the batching alone."""
+    stream, drawn from the serving configuration's distribution (scenes.serving_stream:
+    half granular boxes, half bonded chains struck by an r = 8 disc; n log-uniform in
+    [1k, 256k]; 2,000-20,000 steps; seeded per-scene friction, restitution and bond
+    strengths), --serve-scenes of them (the configuration's stream holds 2,000).  Each
+    mode runs twice: with Poisson arrivals at --serve-rate (latency under that load)
+    and with every scene queued at t = 0 (service capacity).  "plain" is the same
+    server admitting one scene at a time, so the difference is the batching alone;
+    both replay each tick's launches from HIP graphs."""
     from psim import scenes
     from psim.autotune import Tuner
     from psim.serve import Server
@@ -162,22 +165,25 @@ def serving_bench(args):
 
     def run(max_resident, saturated):
         srv = Server(slot_capacity=slots, cell_capacity=cells, bond_capacity=max(bonds, 1), scene_capacity=len(stream),
-                     substeps=args.serve_substeps, max_resident=max_resident, record_data=False)
+                     substeps=args.serve_substeps, max_resident=max_resident, record_data=False, graph=True)
         reqs = [(0.0 if saturated else t, s, st.copy(), n, g) for (t, s, st, n), g in zip(stream, grids)]
         _, lat, wall = srv.run_stream(reqs)
         stt = srv.stats()
         srv.close()
         lat = sorted(lat)
         return {"scenes_per_s": len(reqs) / wall, "p50_s": nearest_rank(lat, 0.50), "p99_s": nearest_rank(lat, 0.99),
-                "wall_s": wall,
+                "wall_s": wall, "failed": int(stt["failed"]),
                 "particle_steps_per_s": ps_total / wall, "ticks": int(stt["ticks"])}
 
     out = {"stream": {"scenes": len(stream), "arrival_rate": args.serve_rate, "particles_total":
                       sum(st.p_count for _, _, st, _ in stream), "particle_steps_total": ps_total,
                       "substeps_per_tick": args.serve_substeps,
                       "tuned_cell_factors": {"%s/2^%d" % k: v["cell_factor"] for k, v in tuner.cache.items()},
-                      "mix": "50/50 granular boxes (f32 contacts) / bonded chains (parallel bonds, Weibull strengths); "
-                             "n log-uniform [1024, 16384], steps U[200, 1000], mu U[0.2, 0.8], strength scale U[20, 80]"}}
+                      "graphs": True,
+                      "mix": "50/50 granular boxes (f32 contacts) / bonded chains of 256 discs struck by an r = 8 disc "
+                             "(parallel bonds, Weibull(6) strengths); n log-uniform [1024, 262144], steps U[2000, 20000], "
+                             "mu U[0.2, 0.8], restitution U[0.2, 0.9], strength scale U[20, 80]; %d scenes of the "
+                             "configuration's 2,000" % len(stream)}}
     for name, mr in (("batched", 0), ("plain", 1)):
         out[name] = {"poisson": run(mr, False), "saturated": run(mr, True)}
         print("bench: serving %s done" % name, file=sys.stderr, flush=True)

@@ -235,15 +235,18 @@ def chain_bonds(st: State, chains: int, per_chain: int, strengths: np.ndarray = 
     return st
 
 
-def serving_stream(n_scenes: int = 96, rate: float = 50.0, seed: int = 4, n_min: int = 1024, n_max: int = 16384,
-                   steps_min: int = 200, steps_max: int = 1000):
-    """A stream of independent scenes in the style of the serving configuration:
-    Poisson arrivals at `rate` scenes/s; half granular boxes (plain contacts, f32),
-    half bonded chains struck by a small dense disc (parallel bonds, Weibull(k = 6)
-    strengths); particle counts log-uniform in [n_min, n_max]; per-scene seeded
-    friction mu ~ U[0.2, 0.8] and bond strength scale ~ U[20, 80]; step counts
-    uniform in [steps_min, steps_max].  Scene i uses seed `seed * 100003 + i`.
-    Returns a list of (arrival offset s, settings, state, steps)."""
+def serving_stream(n_scenes: int = 64, rate: float = 50.0, seed: int = 4, n_min: int = 1024, n_max: int = 262144,
+                   steps_min: int = 2000, steps_max: int = 20000, striker_r: float = 8.0):
+    """A stream of independent scenes drawn from the serving configuration's
+    distribution: Poisson arrivals at `rate` scenes/s; half granular boxes in the style
+    of the granular-box configuration (plain contacts, f32), half bonded chains in the
+    style of the struck-chain configuration (chains of 256 r = 0.5 discs, parallel
+    bonds, Weibull(k = 6) strengths, ends pinned, struck by an r = `striker_r` disc of
+    density 10 at 40 units/s); particle counts log-uniform in [n_min, n_max]; per-scene
+    seeded friction mu ~ U[0.2, 0.8], contact restitution ~ U[0.2, 0.9] and bond
+    strength scale ~ U[20, 80]; step counts uniform in [steps_min, steps_max].  Scene i
+    uses seed `seed * 100003 + i`.  Returns a list of (arrival offset s, settings,
+    state, steps)."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, n_scenes))
     t -= t[0]
@@ -253,17 +256,19 @@ def serving_stream(n_scenes: int = 96, rate: float = 50.0, seed: int = 4, n_min:
         n = int(round(math.exp(r.uniform(math.log(n_min), math.log(n_max)))))
         mu = float(r.uniform(0.2, 0.8))
         steps = int(r.integers(steps_min, steps_max + 1))
+        e = float(r.uniform(0.2, 0.9))
         if i % 2 == 0:
             side = max(4, int(round(math.sqrt(n))))
-            s, st = granular_box(n_side=side, seed=int(r.integers(1 << 30)), spacing=2.0, mu=mu)
+            s, st = granular_box(n_side=side, seed=int(r.integers(1 << 30)), spacing=2.0, mu=mu, restitution=e)
         else:
-            per_chain = 64
-            chains = max(1, n // per_chain)
+            per_chain = 256
+            chains = max(1, int(round(n / per_chain)))
             scale = float(r.uniform(20.0, 80.0))
             s, st, strengths = bonded_chains(chains=chains, per_chain=per_chain, seed=int(r.integers(1 << 30)),
-                                             striker_r=1.0, striker_v=40.0, striker_density=50.0,
+                                             striker_r=striker_r, striker_v=40.0, striker_density=10.0,
                                              strength_scale=scale)
             s.physics.friction_coefficient = f32(mu)
+            s.physics.restitution = e
             chain_bonds(st, chains, per_chain, strengths)
         out.append((float(t[i]), s, st, steps))
     return out

@@ -1,12 +1,13 @@
 """Serving loop (continuous batching) on the GPU: every scene served in a mix of
 other scenes - admitted while others are mid-run, its steps split over ticks, its
-neighbours in the launches changing - reproduces its solo run bit for bit.  Needs an
-MI355X."""
+neighbours in the launches changing - reproduces its solo run and the oracle's run
+(the reference algorithm on the CPU) bit for bit.  Needs an MI355X."""
 import os
 
 import numpy as np
 import pytest
 
+import orc
 from psim import abi, scenes
 from psim.compute import ComputeProg
 from psim.serve import Server, footprint
@@ -55,6 +56,16 @@ def mixed_scenes():
     return out
 
 
+def oracle(s, st, n):
+    """The scene advanced n steps by the oracle (canonical slot order), as a State with
+    the oracle's contact table."""
+    eng = orc.OracleEngine(st.copy(), s, mode=orc.MODE_CANONICAL)
+    eng.step(n)
+    out = eng.st.copy()
+    out.contacts = eng.canonical_contacts().copy()
+    return out
+
+
 def solo(s, st, n):
     prog = ComputeProg(s, st.copy(), record_data=True)
     prog.step(n)
@@ -65,10 +76,12 @@ def solo(s, st, n):
 
 @pytest.mark.parametrize("graph", [False, True])
 def test_batched_equals_solo_with_continuous_arrivals(graph):
-    """Scenes arriving while others run reproduce their solo runs; with `graph` the
-    ticks' launches run as HIP graphs (captured once per launch pattern, replayed after)."""
+    """Scenes arriving while others run reproduce their solo runs and the oracle's runs;
+    with `graph` the ticks' launches run as HIP graphs (captured once per launch
+    pattern, replayed after)."""
     sc = mixed_scenes()
     want = [solo(s, st, n) for _, s, st, n in sc]
+    ref = [oracle(s, st, n) for _, s, st, n in sc]
     srv = Server(slot_capacity=8192, cell_capacity=1 << 16, bond_capacity=256, scene_capacity=16, substeps=7,
                  graph=graph)
     tickets, outs = [], []
@@ -90,7 +103,37 @@ def test_batched_equals_solo_with_continuous_arrivals(graph):
         got, lat = srv.take(t, outs[i])
         assert lat > 0.0
         assert_same(got, want[i], sc[i][0])
+        assert_same(got, ref[i], sc[i][0] + " vs oracle")
     srv.close()
+
+
+def test_stream_scenes_match_oracle():
+    """Eight scenes of the serving configuration's distribution (scenes.serving_stream:
+    granular boxes and r = 8-struck bonded chains with their own friction, restitution
+    and bond strengths), at reduced sizes (n log-uniform in [1k, 4k], 150-400 steps)
+    so that the oracle finishes in seconds, all submitted at once to pools that hold
+    about half of them, served with HIP graphs: each result equals the oracle's."""
+    stream = scenes.serving_stream(n_scenes=8, seed=11, n_max=4096, steps_min=150, steps_max=400)
+    slots = sum((st.p_count + 255) // 256 * 256 for _, _, st, _ in stream)
+    cells = sum(ComputeProg.default_grid(s, st).w * ComputeProg.default_grid(s, st).h + 1 for _, s, st, _ in stream)
+    bonds = sum(st.n_bonds for _, _, st, _ in stream)
+    srv = Server(slot_capacity=slots // 2 + 4096, cell_capacity=cells, bond_capacity=max(bonds, 1),
+                 scene_capacity=8, substeps=25, graph=True)
+    outs = [st.copy() for _, _, st, _ in stream]
+    tickets = [srv.submit(s, outs[i], n) for i, (_, s, st, n) in enumerate(stream)]
+    for _ in range(2000):
+        if all(srv.status(t) == abi.DEM_REQ_DONE for t in tickets):
+            break
+        srv.tick()
+    kinds = []
+    for i, t in enumerate(tickets):
+        _, s, st, n = stream[i]
+        got, _ = srv.take(t, outs[i])
+        assert_same(got, oracle(s, st, n), "stream scene %d (%d discs, %d steps)" % (i, st.p_count, n))
+        kinds.append((st.p_count, st.n_bonds, float(s.physics.restitution)))
+    srv.close()
+    print("served and matched:", kinds, flush=True)
+    assert sum(1 for k in kinds if k[1] > 0) == 4
 
 
 def test_admission_by_footprint():
