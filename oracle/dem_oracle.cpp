@@ -104,23 +104,46 @@ namespace {
 
 // Every overlapping pair (the narrow-phase predicate of 2D_Simulation.wgsl:159,
 // sqrt(dx*dx + dy*dy) < r_b + r_a) among the first n discs with r != 0 and finite
-// positions, found over a cell list of side >= 2 r_max built here (3x3 stencil):
-// found[a] = ascending partner ids.  O(N); the same set as the O(N^2) brute force.
+// positions, found over a cell list of side >= 2 r_max built here (3x3 stencil), the
+// few discs much larger than the rest by brute force: found[a] = ascending partner
+// ids.  O(N); the same set as the O(N^2) brute force.
 void cell_list_pairs(int n, const float* pos, const float* radii, std::vector<std::vector<int32_t>>& found) {
     found.assign(n, {});
     auto px = [&](int i) { return pos[2 * i]; };
     auto py = [&](int i) { return pos[2 * i + 1]; };
-    float rmax = 0.0f, xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    auto touch = [&](int a, int b) { return lenf(px(b) - px(a), py(b) - py(a)) < (radii[b] + radii[a]); };
+    float xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
     std::vector<int> act;
     for (int i = 0; i < n; i++) {
         if (radii[i] == 0.0f) continue;
         if (!(px(i) == px(i)) || !(py(i) == py(i))) continue;
         act.push_back(i);
-        rmax = std::max(rmax, radii[i]);
         xmin = std::min(xmin, px(i)); xmax = std::max(xmax, px(i));
         ymin = std::min(ymin, py(i)); ymax = std::max(ymax, py(i));
     }
     if (act.empty()) return;
+    // A few discs far larger than the rest (a striker among grains) would size the cells
+    // for everyone: with the radii in decreasing order, the first k <= 64 with
+    // r_{k-1} > 2 r_k are paired by brute force, the others over cells of 2 r_k.
+    std::vector<float> rs;
+    rs.reserve(act.size());
+    for (int i : act) rs.push_back(radii[i]);
+    const size_t top = std::min<size_t>(rs.size(), 65);
+    std::partial_sort(rs.begin(), rs.begin() + top, rs.end(), std::greater<float>());
+    float rmax = rs[0];
+    for (size_t k = 1; k < top; k++)
+        if (rs[k - 1] > 2.0f * rs[k]) { rmax = rs[k]; break; }
+    std::vector<int> big, small;
+    for (int i : act) (radii[i] > rmax ? big : small).push_back(i);
+    // large bodies against every disc (both directions)
+    std::vector<std::vector<int32_t>> big_of(n);
+    for (int a : big) {
+        for (int b : act)
+            if (b != a && touch(a, b)) {
+                found[a].push_back(b);
+                if (radii[b] <= rmax) big_of[b].push_back(a);
+            }
+    }
     double cs = std::max(2.0 * rmax, 1e-30);
     int64_t gw = std::min<int64_t>(std::max<int64_t>((int64_t)((xmax - xmin) / cs) + 1, 1), 1 << 14);
     int64_t gh = std::min<int64_t>(std::max<int64_t>((int64_t)((ymax - ymin) / cs) + 1, 1), 1 << 14);
@@ -130,14 +153,14 @@ void cell_list_pairs(int n, const float* pos, const float* radii, std::vector<st
         cy = std::min<int64_t>(std::max<int64_t>((int64_t)(((double)py(i) - ymin) / cs), 0), gh - 1);
     };
     std::vector<int64_t> start(gw * gh + 1, 0);
-    std::vector<int32_t> order(act.size());
-    for (int i : act) { int64_t cx, cy; cell_of(i, cx, cy); start[cy * gw + cx + 1]++; }
+    std::vector<int32_t> order(small.size());
+    for (int i : small) { int64_t cx, cy; cell_of(i, cx, cy); start[cy * gw + cx + 1]++; }
     for (int64_t c = 0; c < gw * gh; c++) start[c + 1] += start[c];
     std::vector<int64_t> fill(start.begin(), start.end() - 1);
-    for (int i : act) { int64_t cx, cy; cell_of(i, cx, cy); order[fill[cy * gw + cx]++] = i; }
+    for (int i : small) { int64_t cx, cy; cell_of(i, cx, cy); order[fill[cy * gw + cx]++] = i; }
 #pragma omp parallel for schedule(dynamic, 256)
-    for (size_t k = 0; k < act.size(); k++) {
-        int a = act[k];
+    for (size_t k = 0; k < small.size(); k++) {
+        int a = small[k];
         int64_t cx, cy;
         cell_of(a, cx, cy);
         std::vector<int32_t>& out = found[a];
@@ -146,12 +169,13 @@ void cell_list_pairs(int n, const float* pos, const float* radii, std::vector<st
                 int64_t c = yy * gw + xx;
                 for (int64_t e = start[c]; e < start[c + 1]; e++) {
                     int b = order[e];
-                    if (b == a) continue;
-                    if (lenf(px(b) - px(a), py(b) - py(a)) < (radii[b] + radii[a])) out.push_back(b);
+                    if (b != a && touch(a, b)) out.push_back(b);
                 }
             }
+        out.insert(out.end(), big_of[a].begin(), big_of[a].end());
         std::sort(out.begin(), out.end());
     }
+    for (int a : big) std::sort(found[a].begin(), found[a].end());
 }
 
 struct Ctx {
@@ -186,7 +210,17 @@ struct Ctx {
         if (s.bond_strengths && k >= 0 && k < s.n_bonds) return s.bond_strengths[2 * k + 1];
         return u.bond_shear_strength;
     }
-    void break_bond(int32_t k, int32_t code) { if (k >= 0 && k < s.n_bonds) s.bonds[3 * k] = code; }
+    // With `defer_breaks` (the canonical Sim pass run in parallel) a break is queued on
+    // the breaking particle and written after the pass in particle-id order, the order
+    // in which the serial pass writes them.
+    bool defer_breaks = false;
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> breaks;   // per particle {bond, code}
+    static int32_t& owner() { static thread_local int32_t id = -1; return id; }
+    void break_bond(int32_t k, int32_t code) {
+        if (!(k >= 0 && k < s.n_bonds)) return;
+        if (defer_breaks) breaks[owner()].push_back({k, code});
+        else s.bonds[3 * k] = code;
+    }
 
     // distance / distance2, 2D_Simulation.wgsl:264-284 ($ F32 and $ F64 variants)
     float distance(int a, int b) const {
@@ -589,6 +623,7 @@ struct Ctx {
     void find_all_pairs() { cell_list_pairs(u.particles, s.pos, s.radii, found); }
 
     void sim_canonical(int id) {
+        owner() = id;
         s.data[DATA_SIZE * id + 0] = 0.0f;
         s.data[DATA_SIZE * id + 1] = 0.0f;
         s.data[DATA_SIZE * id + 2] = 0.0f;
@@ -659,12 +694,31 @@ struct Ctx {
 
     void step() {
         const int n = u.particles;
-        for (int id = 0; id < n; id++) lom(id);
+        if (!canonical) {
+            for (int id = 0; id < n; id++) lom(id);
+        } else {
+            // invocation 0 first (it owns the coll_cont counters the others read), then
+            // the rest, each of which touches only its own particle
+            if (n > 0) lom(0);
+#pragma omp parallel for schedule(static)
+            for (int id = 1; id < n; id++) lom(id);
+        }
         bond_snapshot.resize(s.n_bonds);
         for (int k = 0; k < s.n_bonds; k++) bond_snapshot[k] = s.bonds[3 * k];
         if (canonical) {
             if (u.collisions == 1 && s.coll_cont[0] == 1) find_all_pairs();
+            // every invocation writes only its own particle's slots, data and
+            // accelerations (and invocation 0 the coll_cont words no other one reads in
+            // this pass); the shared bond words are queued and written in id order
+            breaks.resize(n);
+            defer_breaks = true;
+#pragma omp parallel for schedule(dynamic, 256)
             for (int id = 0; id < n; id++) sim_canonical(id);
+            defer_breaks = false;
+            for (int id = 0; id < n; id++) {
+                for (const auto& b : breaks[id]) s.bonds[3 * b.first] = b.second;
+                breaks[id].clear();
+            }
         } else {
             for (int id = 0; id < n; id++) sim_reference(id);
         }
@@ -702,6 +756,14 @@ int orc_run(const dem_uniform* u, const dem_material* mats, int32_t n_mats, cons
 
 // Sorts every particle's occupied slots by partner id and clears empty slots to
 // -1 words: the canonical table layout the HIP path downloads.
+// Threads of the oracle's OpenMP loops (the canonical mode's LOM, pair search and Sim
+// pass); results do not depend on it.  Returns the previous count.
+int orc_set_threads(int32_t threads) {
+    const int prev = omp_get_max_threads();
+    if (threads > 0) omp_set_num_threads(threads);
+    return prev;
+}
+
 int orc_canonicalize(float* contacts, int32_t n, int32_t slots) {
     Slot* t = reinterpret_cast<Slot*>(contacts);
     for (int i = 0; i < n; i++) Ctx::canonicalize(t + (int64_t)i * slots, slots);

@@ -57,6 +57,7 @@ def lib():
         L.orc_regen_bonds.argtypes = [C.c_int32, FP, FP, C.c_int32, C.c_int32, IP, C.c_int64, FP, C.c_float,
                                       C.c_float]
         L.orc_regen_bonds.restype = C.c_int64
+        L.orc_set_threads.argtypes = [C.c_int32]
         L.orc_kinetic_energy.argtypes = [C.c_int32, FP, FP, FP, IP, C.POINTER(abi.DemMaterial), C.c_int32]
         L.orc_kinetic_energy.restype = C.c_double
         L.orc_time_reference.argtypes = [C.POINTER(abi.DemUniform), C.POINTER(abi.DemMaterial), C.c_int32,

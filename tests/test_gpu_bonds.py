@@ -175,3 +175,44 @@ def test_struck_chains_r8_striker_match_oracle():
     striker = got.contacts.view(np.int32).reshape(-1, 14, 6)[n - 1]
     assert (striker[:, 1] >= 0).sum() > 0          # the striker reached the chains
     prog.close()
+
+
+def test_config1_full_size_matches_oracle():
+    """The struck-chain configuration at full size: 64 chains of 256 r = 0.5 discs
+    (16,320 parallel bonds, Weibull(6, 50) strengths, ends pinned) struck by an r = 8
+    disc of density 10 at 40 units/s, dt 5e-5, 20,000 steps.  The state, break codes,
+    contact table (the striker's 14-slot cap included) and kinetic energy equal the
+    oracle's at steps 5,000 and 20,000."""
+    s, st, strengths = scenes.bonded_chains()
+    scenes.chain_bonds(st, 64, 256, strengths)
+    n, nb = st.p_count, st.n_bonds
+    assert (n, nb) == (16385, 16320)
+    prog = ComputeProg(s, st.copy(), record_data=True, reorder_interval=64)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL)
+    done = 0
+    for k in (5000, 15000):
+        prog.step(k)
+        eng.step(k)
+        done += k
+        got = prog.update_state()
+        for name in FIELDS:
+            assert_bits(getattr(got, name)[: _n(name, n)], getattr(eng.st, name)[: _n(name, n)],
+                        "%s at step %d" % (name, done))
+        assert_bits(got.data[: 7 * n], eng.st.data[: 7 * n], "data at step %d" % done)
+        assert_bits(got.bonds[: 3 * nb], eng.st.bonds[: 3 * nb], "bond words at step %d" % done)
+        want = eng.canonical_contacts().view(np.int32).reshape(-1, 14, 6)[:n]
+        have = got.contacts.view(np.int32).reshape(-1, 14, 6)[:n]
+        assert_bits(have, want, "contact table at step %d" % done)
+        ke = prog.kinetic_energy()
+        o = eng.st
+        ke_o = orc.lib().orc_kinetic_energy(n, o.vel.ctypes.data_as(orc.FP), o.rot_vel.ctypes.data_as(orc.FP),
+                                            o.radii.ctypes.data_as(orc.FP), o.material_pointers.ctypes.data_as(orc.IP),
+                                            eng.mats, len(eng.mats))
+        assert abs(ke - ke_o) <= 1e-12 * ke_o
+        striker = int((have[n - 1, :, 1] >= 0).sum())
+        print("config 1 at step %d: %d of %d bonds broken, striker holds %d slots, kinetic energy %.6g"
+              % (done, int((got.bonds[0:3 * nb:3] < 0).sum()), nb, striker, ke), flush=True)
+    c = prog.counters()
+    prog.close()
+    assert int((got.bonds[0:3 * nb:3] < 0).sum()) > 0
+    print("counters:", c, flush=True)

@@ -349,3 +349,28 @@ def test_contact_slots_capped_at_14_existing_kept():
         assert c[0, :, 1].tolist() == want
         for b in range(1, 21):
             assert c[b, 0, 1] == 0 and c[b, 1, 1] == -1
+
+
+def test_parallel_oracle_equals_serial():
+    """The canonical mode runs LOM, the pair search and the Sim pass over OpenMP threads,
+    with bond breaks queued and written in particle-id order: struck chains with an
+    r = 8 striker (a large body in the pair search) through breakage, identical bit for
+    bit on 1 and 6 threads."""
+    s, st, strengths = scenes.bonded_chains(chains=8, per_chain=64, seed=5, striker_r=8.0, striker_v=40.0,
+                                            striker_density=10.0, strength_scale=30.0)
+    scenes.chain_bonds(st, 8, 64, strengths)
+    runs = []
+    prev = orc.lib().orc_set_threads(1)
+    try:
+        for threads in (1, 6):
+            orc.lib().orc_set_threads(threads)
+            e = orc.OracleEngine(st.copy(), s, mode=orc.MODE_CANONICAL)
+            e.step(600)
+            runs.append((e.st.copy(), e.canonical_contacts().copy()))
+    finally:
+        orc.lib().orc_set_threads(prev)
+    (a, ca), (b, cb) = runs
+    for name in ("pos", "vel", "acc", "rot", "rot_vel", "rot_acc", "data", "bonds"):
+        assert np.array_equal(getattr(a, name).view(np.int32), getattr(b, name).view(np.int32)), name
+    assert np.array_equal(ca.view(np.int32), cb.view(np.int32))
+    assert (a.bonds[0:3 * a.n_bonds:3] < 0).sum() > 5
