@@ -164,7 +164,7 @@ def _replay(engine_factory, scene):
     return {p.replace("\\", "/").split("/")[-1][:-4]: rows for p, rows in rep.run()}
 
 
-@pytest.mark.parametrize("scene", ["TEXPS", "PBEXPS", "Chain_TEXP"])
+@pytest.mark.parametrize("scene", ["TEXPS", "PBEXPS", "Chain_TEXP", "Chain_EXP", "CHTEXPDET"])
 def test_experiment_replay_matches_oracle_and_traces(scene):
     """The embedded experiment scripts replayed on the GPU (f64 distances, as the
     fixtures were recorded): every exported row equals the oracle's bit for bit, and
@@ -222,9 +222,9 @@ def test_neighbour_list_overflow_takes_stencil_fallback():
 def test_fast_motion_rebuilds_lists_often(reorder):
     """Discs at ~30 units/s (dt 1e-4): the lists are rebuilt every few dozen steps
     as particles cross 0.45 skin; every step stays bit-exact with the oracle.  With
-    re-sorts every 8 steps the rebuilds alternate between the launch chain of a
-    re-sort and the single cooperative launch; without re-sorts every rebuild after
-    the first is the cooperative one."""
+    re-sorts every 8 steps the rebuilds alternate between those that follow a re-sort
+    (the lists built over the moved storage) and those raised by the displacement
+    check alone; without re-sorts every rebuild is of the second kind."""
     s, st = scenes.granular_box(n_side=24, seed=12, spacing=1.9)
     n = st.p_count
     st.vel[: 2 * n] = np.random.default_rng(12).normal(0.0, 30.0, 2 * n).astype(np.float32)

@@ -50,6 +50,7 @@ def test_texp1_bit_exact(traces):
     ("PBEXPS", range(1, 7), 1e-4),        # kinematics (this export predates the bond data columns)
     ("Chain_EXP", range(1, 7), 1e-4),
     ("Chain_TEXP", range(1, 11), 1e-3),
+    ("CHTEXPDET", range(1, 11), 1e-3),
 ])
 def test_traces_follow_reference(traces, scene, cols, tol, mode):
     """All 13 experiments of each scene: the oracle's trace follows the
@@ -65,6 +66,21 @@ def test_traces_follow_reference(traces, scene, cols, tol, mode):
             scale = max(np.abs(ref[:n, c]).max(), 1e-30)
             err = np.abs(ref[:n, c] - rows[:n, c]).max() / scale
             assert err < tol, (stem, c, err)
+
+
+def test_chtexpdet_reproduces_determinism_runs(traces):
+    """CHTEXPDET.bin (the chain tension experiments, f64 distances): its first
+    experiment's position and velocity columns equal the reference's first recorded
+    run (data/Chain_DET1.csv) bit for bit, and lie as close to the second recorded run
+    (Chain_DET2.csv) as that run lies to the first."""
+    got = _replay("CHTEXPDET", orc.MODE_CANONICAL)["Chain_T_1"]
+    d1, d2 = traces["Chain_DET1"], traces["Chain_DET2"]
+    n = min(len(got), len(d1), len(d2))
+    assert n == 501
+    assert np.array_equal(got[:n, 1:7].astype(np.float32), d1[:n, 1:7].astype(np.float32))
+    for c in range(1, 7):
+        spread = np.abs(d2[:n, c] - d1[:n, c]).max()
+        assert np.abs(d2[:n, c] - got[:n, c]).max() <= spread
 
 
 def test_texp_closed_form(traces):
