@@ -80,7 +80,7 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --block and --cell as given instead of tuning them on the workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
-    ap.add_argument("--serve-scenes", type=int, default=32)
+    ap.add_argument("--serve-scenes", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, scenes/s")
     ap.add_argument("--serve-substeps", type=int, default=50)
     ap.add_argument("--no-bonded", action="store_true", help="skip the 1M-disc bonded-assembly leg")

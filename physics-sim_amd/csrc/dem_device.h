@@ -123,6 +123,9 @@ struct SceneDev {
 // in the radii sorted in decreasing order (a striker among small discs).  The grid is
 // sized for the others; large bodies pair with everything through the list.
 constexpr int LB_MAX = 64;
+// Small discs overlapping one large body, found by the large-body search each launch with
+// a narrow phase (more: the large body's thread walks the stencil itself).
+constexpr int LBH_MAX = 128;
 
 // A device array addressed by 32-bit element index: the byte offset is formed in
 // 32 bits, so loads and stores take the SGPR-base + VGPR-offset form and need no
@@ -174,6 +177,8 @@ struct DevArrays {
     Pool<const dem_material> MAT;
     // bond pass: owner of each record (global particle id, -1: none) and its results
     Pool<int32_t> LB;       // large-body particle ids, LB_MAX per scene
+    Pool<int32_t> LBH;      // [LB pool][LBH_MAX] slots of the small discs overlapping each large body
+    Pool<int32_t> LBHC;     // [LB pool] their count, or -1 (more than LBH_MAX)
     Pool<int32_t> RO;
     Pool<int32_t> RS, RP;   // slots of each record's owner (-1: none) and partner
     Pool<int2> BRS;         // [cap] per slot: its record range (moves with the particle)

@@ -87,11 +87,11 @@ struct SceneHost {
 // One planned launch of a step sequence.  Operations are built zero-filled (padding
 // included) so that two plans compare bytewise: a repeated plan replays its graph.
 enum OpKind { OP_STEP, OP_RB_STORE, OP_RB_BIN, OP_SCAN, OP_RB_SCATTER, OP_BUILD_NL, OP_PARTITION, OP_MOVE, OP_BONDS,
-              OP_RSLOTS };
+              OP_RSLOTS, OP_LARGE };
 static const char* const kOpNames[] = {"step kernel",      "rebuild: flag",  "rebuild: bin",
                                        "rebuild: scan",    "rebuild: scatter", "rebuild: lists",
                                        "re-sort: partition", "re-sort: move", "bond pass",
-                                       "re-sort: record slots"};
+                                       "re-sort: record slots", "large-body search"};
 struct LaunchOp {
     int32_t kind, timed;
     int32_t grid, block, nlb, ncells, value, f64, bonds;
@@ -213,6 +213,8 @@ private:
     int32_t *KEY_ = nullptr, *RANK_ = nullptr, *CNT_ = nullptr, *START_ = nullptr, *SIDX_ = nullptr, *INV_ = nullptr;
     int32_t* RB_ = nullptr;
     int32_t* LB_ = nullptr;   // large-body ids, LB_MAX per scene
+    int32_t* LBH_ = nullptr;  // small discs overlapping each large body (LBH_MAX per large body)
+    int32_t* LBHC_ = nullptr; // their counts
     int32_t* RO_ = nullptr;   // bond pass: record owners, their slots and results
     int32_t *RS_ = nullptr, *RP_ = nullptr;
     float4 *RF_ = nullptr, *RD_ = nullptr;

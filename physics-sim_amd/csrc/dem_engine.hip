@@ -38,6 +38,8 @@ hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, i
 hipError_t launch_count_contacts(const int32_t* m, int p0, int np, unsigned long long* out, hipStream_t st);
 hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
                              float gap, int32_t* out_cnt, int32_t* out_j, hipStream_t st);
+hipError_t launch_large_search(const StepArgs& args, int nblocks, hipStream_t st, hipEvent_t ev0 = nullptr,
+                               hipEvent_t ev1 = nullptr);
 hipError_t launch_group_sums(const DevArrays& a, int cur, int m0, int nm, int p0, const int32_t* ids, int n_ids,
                              int with_data, double* partials, double* out, hipStream_t st);
 hipError_t launch_ke_reduce(double* buf, int n, double* host_out, hipStream_t st);
@@ -117,7 +119,7 @@ Engine::~Engine() {
     hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BLS_); hipFree(BRB_); hipFree(MAT_);
     hipFree(KEY_); hipFree(RANK_); hipFree(CNT_); hipFree(START_); hipFree(SIDX_); hipFree(INV_);
     hipFree(RB_); hipFree(ORDER_); hipFree(NEWSLOT_); hipFree(RO_); hipFree(RF_); hipFree(RD_);
-    hipFree(RS_); hipFree(RP_); hipFree(LB_);
+    hipFree(RS_); hipFree(RP_); hipFree(LB_); hipFree(LBH_); hipFree(LBHC_);
     hipFree(STAT_); hipFree(TSUM_); hipFree(d_scenes_); hipFree(d_blk_scene_); hipFree(d_lf_);
     hipFree(d_ke_); hipFree(d_cnt_);
     if (d_group_) hipFree(d_group_);
@@ -193,6 +195,8 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = e ? e : dalloc(&TSUM_, ntiles_ + 1);
     e = e ? e : dalloc(&d_scenes_, scene_cap_);
     e = e ? e : dalloc(&LB_, (int64_t)scene_cap_ * LB_MAX);
+    e = e ? e : dalloc(&LBH_, (int64_t)scene_cap_ * LB_MAX * LBH_MAX);
+    e = e ? e : dalloc(&LBHC_, (int64_t)scene_cap_ * LB_MAX);
     const int64_t nblk = launch_grid_blocks();
     e = e ? e : dalloc(&d_blk_scene_, nblk);
     lf_cap_ = (int64_t)scene_cap_ * 4096;
@@ -525,6 +529,7 @@ DevArrays Engine::arrays(int set) const {
     a.KEY = KEY_; a.RANK = RANK_; a.CNT = CNT_; a.START = START_; a.SIDX = SIDX_; a.INV = INV_;
     a.NL = s.NL; a.XR = s.XR; a.RB = RB_;
     a.RO = RO_; a.RF = RF_; a.RD = RD_; a.RS = RS_; a.RP = RP_; a.BRS = s.BRS; a.LB = LB_;
+    a.LBH = LBH_; a.LBHC = LBHC_;
     a.STAT = STAT_; a.TSUM = TSUM_; a.ntiles = ntiles_;
     return a;
 }
@@ -873,6 +878,7 @@ hipError_t Engine::run_op(const LaunchOp& op, hipEvent_t ev0, hipEvent_t ev1) {
         case OP_MOVE: return launch_reorder_move(op.ra, op.nlb, op.block, stream_, ev0, ev1);
         case OP_BONDS: return launch_bond_forces(op.sa, op.ncells, op.block, op.f64 != 0, stream_, ev0, ev1);
         case OP_RSLOTS: return launch_record_slots(op.sa, op.ncells, op.block, stream_, ev0, ev1);
+        case OP_LARGE: return launch_large_search(op.sa, op.grid, stream_, ev0, ev1);
         default: return hipErrorInvalidValue;
     }
 }
@@ -927,7 +933,7 @@ int Engine::run_ops(const std::vector<LaunchOp>& ops) {
                 float m = 0;
                 hipEventElapsedTime(&m, x.a, x.b);
                 switch (x.timed) {
-                    case 0: t_step += m; n_step++; break;
+                    case 0: t_step += m; n_step += x.kind == OP_STEP ? 1 : 0; break;
                     case 1: t_scan += m; n_scan++; break;
                     case 2: t_scatter += m; n_scatter++; break;
                     case 4: t_bonds += m; n_bonds++; break;
@@ -1036,10 +1042,14 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
     if ((rc = check(e, "upload launch flags"))) return rc;
 
     bool any_f64 = false, any_bonds = false, any_large = false;
+    int lb_grid = 0;   // one wave per large-body entry of the scenes up to the last with large bodies
     for (int32_t sid : sids) {
         any_f64 |= scenes_[sid].params.use_f64 != 0;
         any_bonds |= scenes_[sid].nrec > 0;
-        any_large |= !scenes_[sid].large.empty();
+        if (!scenes_[sid].large.empty()) {
+            any_large = true;
+            lb_grid = std::max(lb_grid, (sid + 1) * LB_MAX);
+        }
     }
     const int nblk = active_blocks();
     // the neighbour-list rebuild kernels stride over the slot blocks with one round of
@@ -1070,6 +1080,13 @@ int Engine::step(const std::vector<int32_t>& sids, const std::vector<int32_t>& n
             bp.ncells = (int)hw_recs_;
             bp.block = block_;
             bp.f64 = any_f64 ? 1 : 0;
+        }
+        if (any_large && sim) {
+            // each large body's overlapping small discs, one wave per large body, for the
+            // contact search of its thread in the step kernel (timed with the step kernel)
+            LaunchOp& lo = new_op(ops, OP_LARGE, 0);
+            lo.sa = make_args(lfk);
+            lo.grid = lb_grid;
         }
         LaunchOp& st = new_op(ops, OP_STEP, 0);
         st.sa = make_args(lfk);

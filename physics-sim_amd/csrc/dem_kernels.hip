@@ -759,11 +759,26 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                         test(j, a.P[cur][j], true);
                     }
                 } else {
-                    // more candidates than a list holds, or a large body: the stencil around
-                    // the particle's cell at the last build, over the last cell list (cells
-                    // are wider than 2 r_small + skin, so the 3x3 stencil holds every small
-                    // disc that can touch a small disc until the rebuild; a large body
-                    // reaches r + r_small + skin), then the scene's large bodies from their list
+                    // a large body: the small discs that overlap it, found by this launch's
+                    // large-body search (all 64 lanes of a wave on its stencil)
+                    int lbh = -1, lbg = -1;
+                    if (LARGE && (pf & PF_LARGE) && S.nlb > 0) {
+                        const int me_id = a.ID[s];
+                        for (int k = 0; k < S.nlb; k++)
+                            if (a.LB[S.lb0 + k] == me_id) lbg = S.lb0 + k;
+                        if (lbg >= 0) lbh = a.LBHC[lbg];
+                    }
+                    if (lbh >= 0) {
+                        for (int k = 0; k < lbh; k++) {
+                            const int j = a.LBH[lbg * LBH_MAX + k];
+                            test(j, a.P[cur][j], false);
+                        }
+                    } else {
+                    // more candidates than a list holds, or a large body whose search
+                    // overflowed: the stencil around the particle's cell at the last build,
+                    // over the last cell list (cells are wider than 2 r_small + skin, so the
+                    // 3x3 stencil holds every small disc that can touch a small disc until
+                    // the rebuild; a large body reaches r + r_small + skin)
                     const float2 xr0 = a.XR[s];
                     int cx, cy;
                     cell_of(S, xr0.x, xr0.y, cx, cy);
@@ -784,6 +799,8 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                             if (jb >= 0 && (!LARGE || S.nlb == 0 || qb.z <= S.rsmax)) test(jb, qb, false);
                         }
                     }
+                    }
+                    // then the scene's large bodies, from their list
                     if (LARGE) {
                         for (int k = 0; k < S.nlb; k++) {
                             const int jl = a.INV[S.p0 + a.LB[S.lb0 + k]];
@@ -1135,8 +1152,9 @@ __device__ __forceinline__ bool block_scene_f64(const StepArgs& args) {
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_step_plain(StepArgs args) {
     step_slot<false, false, false>(args);
 }
-// 5 waves per SIMD: 94 VGPRs without spills.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_step_plain_large(StepArgs args) {
+// 4 waves per SIMD: 97 VGPRs without spills (the budget of 5 waves spills the
+// large body's hit-list walk).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_step_plain_large(StepArgs args) {
     step_slot<false, false, true>(args);
 }
 __global__ void __launch_bounds__(256) k_step_plain_f64(StepArgs args) {
@@ -1707,6 +1725,52 @@ __global__ void __launch_bounds__(256) k_count_contacts(const int32_t* m, int p0
     }
 }
 
+// ---- large-body search: one wave per large body ----
+// Runs before the step kernel of every launch with a force half in a scene with large
+// bodies.  The small discs that overlap large body g (the contact search's predicate
+// over the stencil of the last cell list around its build position, with the reach
+// ceil((R + r_small + skin) / cs) + 1 of the step kernel's own stencil walk), found by the
+// wave's 64 lanes together, in any order, into LBH[g]; their count into LBHC[g] (-1
+// beyond LBH_MAX).  The large body's thread then tests these instead of walking the
+// stencil alone: the same candidates pass the same predicate, and its contact search
+// sorts what it admits, so the result does not depend on the order found here.
+__global__ void __launch_bounds__(64) k_large_search(StepArgs args) {
+    const DevArrays& a = args.a;
+    const int g = (int)blockIdx.x;
+    const int sid = g / LB_MAX, k = g % LB_MAX;
+    const SceneDev& S = args.scenes[sid];
+    const uint32_t fl = (uint32_t)args.lf[sid].x;
+    if (!(fl & SF_ACTIVE) || !(fl & SF_SIM) || !(fl & SF_NARROW) || k >= S.nlb) return;
+    __shared__ int cnt;
+    __shared__ int hits[LBH_MAX];
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const int s = a.INV[S.p0 + a.LB[S.lb0 + k]];
+    const float4 p = a.P[args.cur][s];
+    const float2 xr0 = a.XR[s];
+    int cx, cy;
+    cell_of(S, xr0.x, xr0.y, cx, cy);
+    const int pos0 = S.p0 - a.START[S.c0];
+    const int reach = max(1, (int)ceilf((p.z + S.rsmax + S.skin) * S.inv_cs) + 1);
+    const int xa = max(cx - reach, 0), xb = min(cx + reach, S.w - 1);
+    for (int ry = max(cy - reach, 0); ry <= min(cy + reach, S.h - 1); ry++) {
+        const int e0 = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
+        const int e1 = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += 64) {
+            const int j = a.SIDX[e];
+            const float4 q = a.P[args.cur][j];
+            if (j != s && q.z <= S.rsmax && overlaps(q, p)) {
+                const int w = atomicAdd(&cnt, 1);
+                if (w < LBH_MAX) hits[w] = j;
+            }
+        }
+    }
+    __syncthreads();
+    const int c = cnt;
+    for (int i = (int)threadIdx.x; i < min(c, LBH_MAX); i += 64) a.LBH[g * LBH_MAX + i] = hits[i];
+    if (threadIdx.x == 0) a.LBHC[g] = c <= LBH_MAX ? c : -1;
+}
+
 // ---- bond builder (State::regen_bonds rule on the cell list) ----
 // For particle id i: every j > i with sqrt((xj-xi)^2 + (yj-yi)^2) < (ri + rj) * factor + gap,
 // the `maxb` smallest j kept.  Stencil of (2*reach+1)^2 cells.
@@ -1877,6 +1941,11 @@ hipError_t launch_count_contacts(const int32_t* m, int p0, int np, unsigned long
     const int b = 256;
     const int nb = (np + b - 1) / b;
     hipLaunchKernelGGL(k_count_contacts, dim3(nb < COUNT_BLOCKS ? nb : COUNT_BLOCKS), dim3(b), 0, st, m, p0, np, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_large_search(const StepArgs& args, int nblocks, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+    launch_k(k_large_search, dim3(nblocks), dim3(64), st, ev0, ev1, args);
     return hipGetLastError();
 }
 
