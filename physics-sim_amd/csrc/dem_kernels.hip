@@ -1564,19 +1564,25 @@ __global__ void k_reorder_move(ReorderArgs r) {
 
 // 0. the destination of every slot.  Units of PART_BLOCKS consecutive blocks of list
 //    positions (cut at scene boundaries: the scene table's blocks never straddle
-//    scenes) keep their positions, partitioned stably into particles with neighbour-
-//    list candidates first.  The few candidates of a unit then fill whole waves at its
-//    start and every other wave skips the narrow phase, while particles stay within
-//    the unit (16384 slots at blocks of 256) and so near their cell-order neighbours.
-//    The cell list is renamed to the destination slots in place.  One workgroup per
-//    unit, blockDim.x = the logical block size.
+//    scenes) keep their positions, partitioned stably into three classes: particles
+//    with more neighbour-list candidates than one round of the step kernel's contact
+//    loop takes (PART_HEAVY or more, or the stencil fallback), then those with fewer,
+//    then those with none.  Waves then hold particles that take the same number of
+//    candidate rounds (a wave runs as many as its busiest lane), waves without
+//    candidates skip the narrow phase, and particles stay within their unit (16384
+//    slots at blocks of 256), near their cell-order neighbours.  The cell list is
+//    renamed to the destination slots in place.  One workgroup per unit,
+//    blockDim.x = the logical block size.
 constexpr int PART_BLOCKS = 64;
+constexpr int PART_HEAVY = 5;   // the step kernel's list path tests candidates four at a time
 static_assert(PART_BLOCKS <= 64, "one flag bit per block in a 64-bit mask");
 
-__device__ __forceinline__ bool part_flag(const ReorderArgs& r, const SceneDev& S, int j, int& s) {
+// class of the particle at list position j: 0 heavy, 1 light, 2 no candidates
+__device__ __forceinline__ int part_class(const ReorderArgs& r, const SceneDev& S, int j, int& s) {
     s = r.src.SIDX[j];
     if (s < S.p0 || s >= S.p0 + S.np) s = j;     // a damaged list keeps the slot in place
-    return nl_count(r.src.NL[s]) != 0;
+    const int nc = nl_count(r.src.NL[s]);
+    return nc == 0 ? 2 : ((nc < 0 || nc >= PART_HEAVY) ? 0 : 1);
 }
 
 __global__ void __launch_bounds__(256) k_partition(ReorderArgs r, int nlb) {
@@ -1586,36 +1592,42 @@ __global__ void __launch_bounds__(256) k_partition(ReorderArgs r, int nlb) {
     const int tid = (int)threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6, nw = lbs >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
-    __shared__ uint64_t s_bal[PART_BLOCKS][4];   // flagged lanes of each wave of each block
-    __shared__ int s_pre[PART_BLOCKS];           // flagged slots of the segment before each block
-    __shared__ int s_tot;
+    __shared__ uint64_t s_bal[2][PART_BLOCKS][4];   // lanes of class 0 / 1 of each wave of each block
+    __shared__ int s_pre[2][PART_BLOCKS];           // class 0 / 1 slots of the segment before each block
+    __shared__ int s_tot[2];
     int bs = b0;
-    while (bs < b1) {                            // segments: runs of blocks of one scene
+    while (bs < b1) {                               // segments: runs of blocks of one scene
         const int sid = r.blk_scene[bs];
         int be = bs + 1;
         while (be < b1 && r.blk_scene[be] == sid) be++;
         if (sid >= 0) {
             const SceneDev& S = r.scenes[sid];
-            // this thread's slot of every block of the segment: one flag bit per block,
+            // this thread's slot of every block of the segment: two class bits per block,
             // the loads of several blocks in flight together
-            uint64_t mask = 0;
+            uint64_t m0 = 0, m1 = 0;
 #pragma unroll 8
             for (int b = bs; b < be; b++) {
                 int s;
-                if (part_flag(r, S, b * lbs + tid, s)) mask |= 1ull << (b - bs);
+                const int c = part_class(r, S, b * lbs + tid, s);
+                if (c == 0) m0 |= 1ull << (b - bs);
+                if (c == 1) m1 |= 1ull << (b - bs);
             }
             for (int b = bs; b < be; b++) {
-                const uint64_t bf = __ballot((mask >> (b - bs)) & 1ull);
-                if (lane == 0) s_bal[b - bs][wv] = bf;
+                const uint64_t f0 = __ballot((m0 >> (b - bs)) & 1ull);
+                const uint64_t f1 = __ballot((m1 >> (b - bs)) & 1ull);
+                if (lane == 0) {
+                    s_bal[0][b - bs][wv] = f0;
+                    s_bal[1][b - bs][wv] = f1;
+                }
             }
             __syncthreads();
-            if (tid == 0) {
+            if (tid < 2) {
                 int run = 0;
                 for (int q = 0; q < be - bs; q++) {
-                    s_pre[q] = run;
-                    for (int w = 0; w < nw; w++) run += __popcll(s_bal[q][w]);
+                    s_pre[tid][q] = run;
+                    for (int w = 0; w < nw; w++) run += __popcll(s_bal[tid][q][w]);
                 }
-                s_tot = run;
+                s_tot[tid] = run;
             }
             __syncthreads();
             const int seg0 = bs * lbs;
@@ -1625,11 +1637,17 @@ __global__ void __launch_bounds__(256) k_partition(ReorderArgs r, int nlb) {
                 const int j = b * lbs + tid;
                 int s = r.src.SIDX[j];
                 if (s < S.p0 || s >= S.p0 + S.np) s = j;
-                const bool f = (mask >> k) & 1ull;
-                int rf = __popcll(s_bal[k][wv] & below);                      // flagged lanes before this one
-                for (int w = 0; w < wv; w++) rf += __popcll(s_bal[k][w]);
-                const int cb = s_pre[k];                                       // flagged slots before block b
-                const int dst = seg0 + (f ? cb + rf : s_tot + (k * lbs - cb) + (tid - rf));
+                int r0 = __popcll(s_bal[0][k][wv] & below);    // lanes of each class before this one
+                int r1 = __popcll(s_bal[1][k][wv] & below);
+                for (int w = 0; w < wv; w++) {
+                    r0 += __popcll(s_bal[0][k][w]);
+                    r1 += __popcll(s_bal[1][k][w]);
+                }
+                const int c0 = s_pre[0][k], c1 = s_pre[1][k];   // class 0 / 1 slots before block b
+                int dst;
+                if ((m0 >> k) & 1ull) dst = seg0 + c0 + r0;
+                else if ((m1 >> k) & 1ull) dst = seg0 + s_tot[0] + c1 + r1;
+                else dst = seg0 + s_tot[0] + s_tot[1] + (k * lbs - c0 - c1) + (tid - r0 - r1);
                 r.order[dst] = s;
                 r.newslot[s] = dst;
                 r.src.SIDX[j] = dst;
