@@ -32,6 +32,9 @@ __host__ __device__ inline int nl_count(int w) {
 __host__ __device__ inline int nl_first(int w) { return w & ((1 << NL_SHIFT) - 1); }
 constexpr int DATA_WORDS = 7;
 constexpr int REC_DEAD = INT32_MIN;   // bonded word of a dead bond record in BRB
+// last word of a bond-pass result (RF.w, int bits): the partner id of a live record, or
+constexpr int REC_RF_DEAD = -1;       // a dead record (no force)
+constexpr int REC_RF_DIED = -2;       // a record that died in this launch (its partner joins the owner's list)
 
 // Break word of a bond: the bits of BLS.w, so that the bond pass reads a bond's break
 // state in the same 16-byte load as its rest length and strengths.  0 while intact;
@@ -182,7 +185,7 @@ struct DevArrays {
     Pool<int32_t> RO;
     Pool<int32_t> RS, RP;   // slots of each record's owner (-1: none) and partner
     Pool<int2> BRS;         // [cap] per slot: its record range (moves with the particle)
-    Pool<float4> RF;        // {fx, fy, moment, alive}
+    Pool<float4> RF;        // {fx, fy, moment, partner id bits (REC_RF_DEAD / REC_RF_DIED: none)}
     Pool<float4> RD;        // [2 per record] data[0..2] updates (with data recording)
     // broad phase
     Pool<int32_t> KEY;      // cell of each slot (global cell index)

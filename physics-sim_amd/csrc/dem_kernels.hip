@@ -398,22 +398,23 @@ __device__ __forceinline__ bool overlaps(const float4& pj, const float4& p) {
     return sqrtf(d2) < R;
 }
 
+// a live bond record of this launch's bond pass with partner oj among [b_lo, b_hi)
 template <bool BONDS>
 __device__ __forceinline__ bool bonded_partner(const DevArrays& a, int b_lo, int b_hi, int oj) {
     if (!BONDS) return false;
     bool hit = false;
-    for (int q = b_lo; q < b_hi; q++) {
-        const int4 br = a.BR[q];
-        if (br.w != 0 && br.x == oj) hit = true;
-    }
+    for (int q = b_lo; q < b_hi; q++)
+        if (__float_as_int(a.RF[q].w) == oj) hit = true;
     return hit;
 }
 
 // One bond endpoint record of the particle `me` (position p, scene S; partner in slot j) in a launch
 // whose force half runs: the narrow-phase check of the bonded partner, the break
 // state seen from the partner, the bond or contact law; writes the record (and the
-// bond's break state) back.  Returns {fx, fy, moment, 1}, all 0 when the record is
-// dead, or {0, 0, 0, -1} when it dies here (2D_Simulation.wgsl:185-204, 230-262).
+// bond's break state) back.  Returns {fx, fy, moment, partner id} (the id's bits in the
+// last word: the owner merges its records with its contacts in partner-id order from
+// these results alone), {0, 0, 0, -1} when the record is dead, or {0, 0, 0, -2} when it
+// dies here (2D_Simulation.wgsl:185-204, 230-262).
 template <bool F64, class DD>
 __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const SceneDev& S, const Self& me,
                                                    const float4& p, bool narrow, int step, int cur, int q, int j,
@@ -422,9 +423,10 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
     // {bond id, bonded word}: the 8-byte copy of the record word (BR keeps the partner
     // id for the step kernel and the lists; both are written together)
     const int2 rb = a.BRB[q];
-    if (rb.y == REC_DEAD) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (rb.y == REC_DEAD) return make_float4(0.0f, 0.0f, 0.0f, __int_as_float(REC_RF_DEAD));
     float4 bf = a.BF[q];
     const float4 pb = a.P[cur][j];
+    const int pid = a.ID[j];
     const float4 db = del_of(a, j, u.dT);
     int bonded = rb.y;
     if (narrow) {
@@ -435,7 +437,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
             // the partner may come back into contact: the owner's step kernel of this
             // launch puts it into the owner's neighbour list (which leaves out live bond
             // partners), told by the alive word -1
-            return make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+            return make_float4(0.0f, 0.0f, 0.0f, __int_as_float(REC_RF_DIED));
         } else if (!found && bonded > 0 && u.bonds <= 1) {
             bf.x = 0.0f;
         }
@@ -482,7 +484,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
         a.BRB[q].y = bonded;
     }
     a.BF[q] = bf;
-    return make_float4(f.x, f.y, f.z, 1.0f);
+    return make_float4(f.x, f.y, f.z, __int_as_float(pid));
 }
 
 template <bool F64, bool BONDS, bool LARGE>
@@ -538,15 +540,17 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         b_lo = br.x;
         b_hi = br.y;
     }
+    // the partner ids come from this launch's bond-pass results (their last word: the
+    // partner id of a live record, negative otherwise); a dead record's -1 sorts first,
+    // where it adds nothing
     auto load_records = [&]() {
         if (!BONDS || rloaded) return;
         rloaded = true;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (b_lo + k < b_hi) {
-                const int4 r = a.BR[b_lo + k];
-                rx[k] = r.x;
-                if (r.w != 0) ralive |= 1 << k;
+                rx[k] = __float_as_int(a.RF[b_lo + k].w);
+                if (rx[k] >= 0) ralive |= 1 << k;
             }
         }
     };
@@ -630,8 +634,9 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         };
         auto eval_record = [&](int q) {
             const float4 rf = a.RF[q];
-            if (!(rf.w > 0.0f)) {
-                if (rf.w < 0.0f) died(q);
+            const int rw = __float_as_int(rf.w);
+            if (rw < 0) {
+                if (rw == REC_RF_DIED) died(q);
                 return;
             }
             nfx += rf.x;
@@ -645,7 +650,8 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         // partner id of record q (selects, not an indexed register array, which would
         // live in scratch memory)
         auto rec_id = [&](int q) {
-            if (!rec_regs) return a.BR[q].x;
+            if (!rec_regs) return __float_as_int(a.RF[q].w);
+            load_records();
             const int k = q - b_lo;
             return k == 0 ? rx[0] : (k == 1 ? rx[1] : (k == 2 ? rx[2] : rx[3]));
         };
@@ -722,7 +728,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                         load_records();
                         alive = __popc(ralive);
                     } else {
-                        for (int q = b_lo; q < b_hi; q++) alive += a.BR[q].w != 0;
+                        for (int q = b_lo; q < b_hi; q++) alive += __float_as_int(a.RF[q].w) >= 0;
                     }
                 }
                 const int cap_c = max(KMAX - alive, 0);
@@ -840,7 +846,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             for (int q = 0; q < n; q++) {
                 const int ob = a.CPn[q * cap + s];
                 if (BONDS) {
-                    while (bq < b_hi && a.BR[bq].x < ob) eval_record(bq++);
+                    while (bq < b_hi && rec_id(bq) < ob) eval_record(bq++);
                 }
                 float ft = 0.0f;
                 if (narrow) {
@@ -869,7 +875,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                                       m > 3 ? a.RF[bq + 3] : z};
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    if (k < m && rf[k].w > 0.0f) {
+                    if (k < m && __float_as_int(rf[k].w) >= 0) {
                         nfx += rf[k].x;
                         nfy += rf[k].y;
                         nm += rf[k].z;
@@ -877,7 +883,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                             replay(dd, a.RD[2 * (bq + k)]);
                             replay(dd, a.RD[2 * (bq + k) + 1]);
                         }
-                    } else if (k < m && rf[k].w < 0.0f) {
+                    } else if (k < m && __float_as_int(rf[k].w) == REC_RF_DIED) {
                         died(bq + k);
                     }
                 }
@@ -1374,9 +1380,8 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
     // live bond partners stay out of the list: their records carry the pair until the
     // record dies, when the step kernel adds the partner to the list
     const int2 brs = a.BRS[s];
-    auto bonded = [&](int j) {
+    auto bonded = [&](int oj) {
         if (brs.y <= brs.x) return false;
-        const int oj = a.ID[j];
         for (int q = brs.x; q < brs.y; q++) {
             const int4 br = a.BR[q];
             if (br.w != 0 && br.x == oj) return true;
@@ -1389,8 +1394,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
     // round, so that six slot-id loads and then six position loads are in flight at once.
     int ls[NLMAX], li[NLMAX];   // slots and ids, ascending id
     int cnt = 0;
-    auto add = [&](int j) {
-        const int id = a.ID[j];
+    auto add = [&](int j, int id) {
         if (cnt < NLMAX) {
             int pos = 0;
 #pragma unroll
@@ -1430,9 +1434,14 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
             j[2 * r] = e[r] < e1[r] ? a.SIDX[e[r]] : -1;
             j[2 * r + 1] = e[r] + 1 < e1[r] ? a.SIDX[e[r] + 1] : -1;
         }
+        // positions and ids of the six candidates in one round of loads
         float4 q[6];
+        int id[6];
 #pragma unroll
-        for (int k = 0; k < 6; k++) q[k] = j[k] >= 0 ? a.P[cur][j[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int k = 0; k < 6; k++) {
+            q[k] = j[k] >= 0 ? a.P[cur][j[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            id[k] = j[k] >= 0 ? a.ID[j[k]] : 0;
+        }
 #pragma unroll
         for (int k = 0; k < 6; k++) {
             if (j[k] < 0 || j[k] == s) continue;
@@ -1440,7 +1449,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
             const float xx = dx * dx;
             const float yy = dy * dy;
             const float L = (lr + q[k].z) * 1.0001f;
-            if (xx + yy < L * L && (S.nlb == 0 || q[k].z <= S.rsmax) && !bonded(j[k])) add(j[k]);
+            if (xx + yy < L * L && (S.nlb == 0 || q[k].z <= S.rsmax) && !bonded(id[k])) add(j[k], id[k]);
         }
 #pragma unroll
         for (int r = 0; r < 3; r++) e[r] += 2;
@@ -1453,7 +1462,10 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
         const float xx = dx * dx;
         const float yy = dy * dy;
         const float L = (lr + q.z) * 1.0001f;
-        if (xx + yy < L * L && !bonded(jl)) add(jl);
+        if (xx + yy < L * L) {
+            const int idl = a.ID[jl];
+            if (!bonded(idl)) add(jl, idl);
+        }
     }
     if (cnt <= NLMAX) {
 #pragma unroll
