@@ -190,8 +190,13 @@ __device__ __forceinline__ float3 linear_model(float mu, const Self& a, const fl
                                                const float3& kb, float L, float& ft, DD& dd) {
     float nd = fminf(0.0f, dist2<F64>(a, pb.x, pb.y, L));
     if (nd == 0.0f) return make_float3(0.0f, 0.0f, 0.0f);
-    float kn = a.one ? a.kne : 1.0f / (1.0f / a.kn + 1.0f / kb.x);
-    float ks = a.one ? a.kse : 1.0f / (1.0f / a.ks + 1.0f / kb.y);
+    // one-material scenes take the series stiffnesses formed once for the scene (a
+    // branch uniform over the wave: the divisions are not even issued)
+    float kn = a.kne, ks = a.kse;
+    if (!a.one) {
+        kn = 1.0f / (1.0f / a.kn + 1.0f / kb.x);
+        ks = 1.0f / (1.0f / a.ks + 1.0f / kb.y);
+    }
     float nf = -nd * kn;
     float dx = a.x - pb.x, dy = a.y - pb.y;
     float l = lenf(dx, dy);
