@@ -152,24 +152,25 @@ __device__ __forceinline__ void replay(Data3& dd, const float4& o) {
 
 // ---- force laws (2D_Simulation.wgsl:264-432); `a` is the calling particle ----
 
-// distance2 (:273-284): |p_a - p_b| - L
+// distance2 (:273-284): |p_a - p_b| - L.  l = the f32 length lenf(a.x - bx, a.y - by),
+// which the f32 variant uses as is.
 template <bool F64>
-__device__ __forceinline__ float dist2(const Self& a, float bx, float by, float L) {
+__device__ __forceinline__ float dist2(const Self& a, float bx, float by, float L, float l) {
     if (F64) {
         double dx = (double)a.x - (double)bx, dy = (double)a.y - (double)by;
         return (float)(lend(dx, dy) - (double)L);
     }
-    return lenf(a.x - bx, a.y - by) - L;
+    return l - L;
 }
 
-// distance (:264-271): |p_a - p_b| - (r_a + r_b)
+// distance (:264-271): |p_a - p_b| - (r_a + r_b); l as for dist2
 template <bool F64>
-__device__ __forceinline__ float dist1(const Self& a, const float4& pb) {
+__device__ __forceinline__ float dist1(const Self& a, const float4& pb, float l) {
     if (F64) {
         double dx = (double)a.x - (double)pb.x, dy = (double)a.y - (double)pb.y;
         return (float)(lend(dx, dy) - ((double)a.r + (double)pb.z));
     }
-    return lenf(a.x - pb.x, a.y - pb.y) - (a.r + pb.z);
+    return l - (a.r + pb.z);
 }
 
 // relative tangential displacement, :352-360; db = the partner's {del_x, del_y, del_rot}
@@ -182,13 +183,16 @@ __device__ __forceinline__ float rel_tangent(const Self& a, const float4& pb, co
 }
 
 // linear_model, :338-373.  L = r_a + r_b (contact) or the bond length.  kb = the
-// partner's {kn, ks, density}.  With a.cn2 != 0 the normal force carries a dashpot
-// (an extension; the reference's law has none): + 2 zeta sqrt(m_eff kn) times the
-// approach speed along the normal, the relative displacement of the last drift over dT.
+// partner's {kn, ks, density}.  l = |p_a - p_b| in f32, lenf(a.x - pb.x, a.y - pb.y):
+// the f32 distance (:273-284) and the normal share it, and a caller that has formed the
+// same squares for the overlap test passes its root in (x - y and y - x square to the
+// same bits, so the value is identical).  With a.cn2 != 0 the normal force carries a
+// dashpot (an extension; the reference's law has none): + 2 zeta sqrt(m_eff kn) times
+// the approach speed along the normal, the relative displacement of the last drift over dT.
 template <bool F64, class DD>
 __device__ __forceinline__ float3 linear_model(float mu, const Self& a, const float4& pb, const float4& db,
-                                               const float3& kb, float L, float& ft, DD& dd) {
-    float nd = fminf(0.0f, dist2<F64>(a, pb.x, pb.y, L));
+                                               const float3& kb, float L, float l, float& ft, DD& dd) {
+    float nd = fminf(0.0f, dist2<F64>(a, pb.x, pb.y, L, l));
     if (nd == 0.0f) return make_float3(0.0f, 0.0f, 0.0f);
     // one-material scenes take the series stiffnesses formed once for the scene (a
     // branch uniform over the wave: the divisions are not even issued)
@@ -199,7 +203,6 @@ __device__ __forceinline__ float3 linear_model(float mu, const Self& a, const fl
     }
     float nf = -nd * kn;
     float dx = a.x - pb.x, dy = a.y - pb.y;
-    float l = lenf(dx, dy);
     float nx = dx / l, ny = dy / l;
     if (a.cn2 != 0.0f) {
         float ma = a.rho * PI_F * a.r * a.r;
@@ -231,7 +234,8 @@ struct BondCtx {
 // Sets `code` to the break code when the bond tears.
 template <bool F64, class DD>
 __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, const float4& pb, const float4& db,
-                                           const float3& kb, const BondCtx& bc, float4& bf, DD& dd, int& code) {
+                                           const float3& kb, const BondCtx& bc, float l, float4& bf, DD& dd,
+                                           int& code) {
     code = 0;
     if (u.bonds == 3) {
         float kn = u.bond_normal_stiffness;
@@ -239,10 +243,9 @@ __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, 
         float R = 0.5f * fminf(a.r, pb.z);
         float A = 2.0f * R * 1.0f;
         float I = 2.0f / 3.0f * R * R * R;
-        float nd = dist2<F64>(a, pb.x, pb.y, bc.L);
+        float nd = dist2<F64>(a, pb.x, pb.y, bc.L, l);
         float nf = -kn * nd * A;
         float dx = a.x - pb.x, dy = a.y - pb.y;
-        float l = lenf(dx, dy);
         float nx = dx / l, ny = dy / l;
         float tx = -ny, ty = nx;
         float rt = rel_tangent(a, pb, db, tx, ty);
@@ -262,15 +265,14 @@ __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, 
             m = 0.0f;
         }
         dd.add(-nf, bf.y, m);
-        float3 lm = linear_model<F64>(u.friction_coefficient, a, pb, db, kb, bc.L, bf.x, dd);
+        float3 lm = linear_model<F64>(u.friction_coefficient, a, pb, db, kb, bc.L, l, bf.x, dd);
         return make_float3(fx + lm.x, fy + lm.y, m + lm.z);
     } else if (u.bonds == 2) {
-        float nd = dist1<F64>(a, pb);
+        float nd = dist1<F64>(a, pb, l);
         float kn = 1.0f / (1.0f / u.bond_normal_stiffness + 1.0f / u.bond_normal_stiffness);
         float ks = 1.0f / (1.0f / u.bond_shear_stiffness + 1.0f / u.bond_shear_stiffness);
         float nf = -nd * kn;
         float dx = a.x - pb.x, dy = a.y - pb.y;
-        float l = lenf(dx, dy);
         float nx = dx / l, ny = dy / l;
         float tx = -ny, ty = nx;
         float rt = rel_tangent(a, pb, db, tx, ty);
@@ -289,17 +291,16 @@ __device__ __forceinline__ float3 bond_law(const dem_uniform& u, const Self& a, 
         dd.assign(-nf, bf.x, m);
         return make_float3(fx, fy, m);
     } else if (u.bonds == 1) {
-        float disp = dist1<F64>(a, pb);
+        float disp = dist1<F64>(a, pb, l);
         float sf = u.bond_normal_stiffness * disp;
         float dx = pb.x - a.x, dy = pb.y - a.y;
-        float l = lenf(dx, dy);
         float fx = 0.0f + sf * (dx / l), fy = 0.0f + sf * (dy / l);
         if (u.bonds_tear == 1 && sf >= bc.sn) {
             code = -10 + sign_i(disp);
             fx = 0.0f;
             fy = 0.0f;
         }
-        float3 lm = linear_model<F64>(u.friction_coefficient, a, pb, db, kb, bc.L, bf.x, dd);
+        float3 lm = linear_model<F64>(u.friction_coefficient, a, pb, db, kb, bc.L, l, bf.x, dd);
         return make_float3(lm.x + fx, lm.y + fy, lm.z + 0.0f);
     }
     return make_float3(0.0f, 0.0f, 0.0f);
@@ -402,6 +403,17 @@ __device__ __forceinline__ bool overlaps(const float4& pj, const float4& p) {
     if (d2 >= R * R * 1.0001f) return false;
     return sqrtf(d2) < R;
 }
+// the same predicate, also giving the root it took: lenf(p.x - pj.x, p.y - pj.y)
+__device__ __forceinline__ bool overlaps_l(const float4& pj, const float4& p, float& l) {
+    const float dx = pj.x - p.x, dy = pj.y - p.y;
+    const float xx = dx * dx;
+    const float yy = dy * dy;
+    const float d2 = xx + yy;
+    const float R = pj.z + p.z;
+    if (d2 >= R * R * 1.0001f) return false;
+    l = sqrtf(d2);
+    return l < R;
+}
 
 // a live bond record of this launch's bond pass with partner oj among [b_lo, b_hi)
 template <bool BONDS>
@@ -434,8 +446,10 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
     const int pid = a.ID[j];
     const float4 db = del_of(a, j, u.dT);
     int bonded = rb.y;
+    // |p - pb| in f32: the narrow-phase test and every law below use this one value
+    const float l = lenf(pb.x - p.x, pb.y - p.y);
     if (narrow) {
-        const bool found = lenf(pb.x - p.x, pb.y - p.y) < (pb.z + p.z);
+        const bool found = l < (pb.z + p.z);
         if (!found && (bonded < 0 || u.bonds == 0)) {
             a.BR[q].w = 0;
             a.BRB[q].y = REC_DEAD;
@@ -461,7 +475,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
     const float3 kb = partner_k(a, S, j);
     float3 f;
     if (bonded < 0 || u.bonds == 0) {
-        f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb, me.r + pb.z, bf.x, dd);
+        f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb, me.r + pb.z, l, bf.x, dd);
     } else if (u.bonds >= 1 && u.bonds <= 3) {
         BondCtx bc;
         bc.L = bl.x;
@@ -473,7 +487,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
             bc.ss = u.bond_shear_strength;
         }
         int code = 0;
-        f = bond_law<F64>(u, me, pb, db, kb, bc, bf, dd, code);
+        f = bond_law<F64>(u, me, pb, db, kb, bc, l, bf, dd, code);
         if (code < 0) {
             bonded = code;
             if (bid_ok) {
@@ -497,10 +511,13 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const DevArrays& a = args.a;
     const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int s = blk * args.lbs + (int)threadIdx.x;
-    // one live scene: its slot range below decides, with no table load on the way
-    const int sid = args.one_scene >= 0 ? args.one_scene : args.blk_scene[blk];
+    // one live scene: its slot range below decides, with no table load on the way.  The
+    // scene is the block's (uniform over the wave), and its record is read into registers
+    // here, before any store: read through the table later, every field would be loaded
+    // again after each store to the lists (the compiler cannot rule out aliasing).
+    const int sid = __builtin_amdgcn_readfirstlane(args.one_scene >= 0 ? args.one_scene : args.blk_scene[blk]);
     if (sid < 0) return;
-    const SceneDev& S = args.scenes[sid];
+    const SceneDev S = args.scenes[sid];
     if (s < S.p0 || s >= S.p0 + S.np) return;   // whole blocks (np is a multiple of the block)
     const int2 lfw = args.lf[sid];
     const uint32_t fl = (uint32_t)lfw.x;
@@ -679,25 +696,33 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             // ids in one round of loads (before the overlap tests, so that a hit waits on
             // nothing)
             for (int k0 = 0; k0 < nc; k0 += 4) {
+                // Loads without branches, so that all of them are in flight together: list
+                // rows past the count (rows < NLMAX exist for every slot) and, for a missing
+                // candidate, this slot's own records stand in and the test below drops them.
                 int jj[4];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     const int k = k0 + t;
-                    jj[t] = k < nc ? (k == 0 ? nl_first(w0) : (k0 == 0 ? nlr[t - 1] : a.NL[k * cap + s])) : -1;
+                    const int w = k == 0 ? nl_first(w0) : (k0 == 0 ? nlr[t - 1] : a.NL[min(k, NLMAX - 1) * cap + s]);
+                    jj[t] = k < nc ? w : -1;
                 }
                 float4 qq[4], vq[4];
                 int iq[4];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    const bool ok = jj[t] >= 0;
-                    qq[t] = ok ? a.P[cur][jj[t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    vq[t] = ok ? a.V[jj[t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    iq[t] = ok ? a.ID[jj[t]] : INT32_MAX;
+                    const int jl = jj[t] >= 0 ? jj[t] : s;
+                    qq[t] = a.P[cur][jl];
+                    vq[t] = a.V[jl];
+                    iq[t] = a.ID[jl];
                 }
+                // keeps the compiler from sinking any of these loads into the branches
+                // below, which would wait on each candidate in turn
+                asm volatile("" ::: "memory");
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     const int j = jj[t];
-                    if (j < 0 || j == s || !overlaps(qq[t], p)) continue;
+                    float lj = 0.0f;
+                    if (j < 0 || j == s || !overlaps_l(qq[t], p, lj)) continue;
                     const int oj = iq[t];
                     if (BONDS) {
                         while (bq < b_hi && rec_id(bq) < oj) eval_record(bq++);
@@ -713,7 +738,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                     float ft = ow[0] == oj ? fw[0] : 0.0f;
                     const float4 db = del_from(a, j, vq[t], dt);
                     const float3 kb = partner_k(a, S, j);
-                    const float3 f = linear_model<F64>(u.friction_coefficient, me, qq[t], db, kb, me.r + qq[t].z, ft, dd);
+                    const float3 f = linear_model<F64>(u.friction_coefficient, me, qq[t], db, kb, me.r + qq[t].z, lj, ft, dd);
                     nfx += f.x;
                     nfy += f.y;
                     nm += f.z;
@@ -864,7 +889,8 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 const float4 pb = a.P[cur][j];
                 const float4 db = del_of(a, j, dt);
                 const float3 kb = partner_k(a, S, j);
-                const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb, me.r + pb.z, ft, dd);
+                const float f3l = lenf(me.x - pb.x, me.y - pb.y);
+                const float3 f = linear_model<F64>(u.friction_coefficient, me, pb, db, kb, me.r + pb.z, f3l, ft, dd);
                 nfx += f.x;
                 nfy += f.y;
                 nm += f.z;
