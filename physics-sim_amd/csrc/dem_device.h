@@ -150,12 +150,13 @@ struct DevArrays {
     // double-buffered neighbour-visible state.  Positions: P[cur] read, P[cur ^ 1]
     // written by a drift.  Velocities and explicit displacements flip after every launch
     // with the contact lists: V / DX read, Vn / DXn written.
-    Pool<float4> P[2];      // {x, y, r, rot}
+    Pool<float4> P[2];      // {x, y, r, id bits}: the id travels with the position its neighbours read
     Pool<float4> V;         // {vx, vy, w, word: VW_EXPLICIT | path}  (after a LOM half: the mid-step velocity)
     Pool<float4> Vn;
     Pool<float4> DX;        // {del_x, del_y, del_rot, 0}, valid where V.w has VW_EXPLICIT
     Pool<float4> DXn;
     Pool<float4> A;         // {ax, ay, alpha, 0}
+    Pool<float> TH;         // rotation, read and written by the particle's own thread only
     Pool<int32_t> M;        // material | flags | contact count (the word the step kernel reads)
     Pool<int32_t> ID;       // particle id within the scene (read for partners only)
     Pool<float4> EXT;       // external {fx, fy, moment, 0}

@@ -33,6 +33,7 @@ struct DevSet {
     float4* V[2] = {nullptr, nullptr};    // by list parity (lcur_), as DX
     float4* DX[2] = {nullptr, nullptr};
     float4* A = nullptr;
+    float* TH = nullptr;     // rotation (read and written by the particle's own thread only)
     int32_t* M = nullptr;    // material | flags | contact count
     int32_t* ID = nullptr;   // particle id within the scene
     float4* EXT = nullptr;

@@ -112,7 +112,7 @@ Engine::~Engine() {
     clear_graphs();   // before the buffers their nodes refer to
     for (auto& s : sets_) {
         hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.V[0]); hipFree(s.V[1]); hipFree(s.DX[0]); hipFree(s.DX[1]);
-        hipFree(s.A); hipFree(s.M); hipFree(s.ID); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
+        hipFree(s.A); hipFree(s.TH); hipFree(s.M); hipFree(s.ID); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
     }
     for (auto& s : sets_) { hipFree(s.NL); hipFree(s.XR); hipFree(s.BRS); }
     for (auto& l : lists_) { hipFree(l.CP); hipFree(l.CF); }
@@ -168,7 +168,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
         e = e ? e : dalloc(&s.P[0], S); e = e ? e : dalloc(&s.P[1], S);
         e = e ? e : dalloc(&s.V[0], S); e = e ? e : dalloc(&s.V[1], S);
         e = e ? e : dalloc(&s.DX[0], S); e = e ? e : dalloc(&s.DX[1], S);
-        e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.M, S); e = e ? e : dalloc(&s.ID, S);
+        e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.TH, S); e = e ? e : dalloc(&s.M, S); e = e ? e : dalloc(&s.ID, S);
         e = e ? e : dalloc(&s.EXT, S);
         e = e ? e : dalloc(&s.NL, S * NLMAX); e = e ? e : dalloc(&s.XR, S); e = e ? e : dalloc(&s.BRS, S);
         if (with_data) { e = e ? e : dalloc(&s.DATA0, S); e = e ? e : dalloc(&s.DATA1, S); }
@@ -519,7 +519,7 @@ DevArrays Engine::arrays(int set) const {
     DevArrays a{};
     a.P[0] = s.P[0]; a.P[1] = s.P[1];
     a.V = s.V[lcur_]; a.Vn = s.V[lcur_ ^ 1]; a.DX = s.DX[lcur_]; a.DXn = s.DX[lcur_ ^ 1];
-    a.A = s.A; a.M = s.M; a.ID = s.ID; a.EXT = s.EXT; a.DATA0 = s.DATA0; a.DATA1 = s.DATA1;
+    a.A = s.A; a.TH = s.TH; a.M = s.M; a.ID = s.ID; a.EXT = s.EXT; a.DATA0 = s.DATA0; a.DATA1 = s.DATA1;
     const ContactLists& lc = lists_[lcur_];
     const ContactLists& ln = lists_[lcur_ ^ 1];
     a.CP = lc.CP; a.CF = lc.CF;
@@ -578,7 +578,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<int32_t>(np);
         c.take<int32_t>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<int32_t>(np);
         c.take<int32_t>((size_t)np * KMAX); c.take<float>((size_t)np * KMAX); c.take<int32_t>(np);
-        c.take<int32_t>(np); c.take<int2>(np);
+        c.take<int32_t>(np); c.take<int2>(np); c.take<float>(np);
     };
     carve(cv);
     cv.base = staging(cv.off);
@@ -599,6 +599,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     int32_t* INV = cv.take<int32_t>(np);
     int32_t* SIDX = cv.take<int32_t>(np);
     int2* BRANGE = cv.take<int2>(np);
+    float* TH = cv.take<float>(np);
     std::fill(CC, CC + np, 0);
     int32_t rows = 0;   // contact-list rows in use
     sh.fixity.assign(v.fixity, v.fixity + 6 * (size_t)n);
@@ -616,6 +617,8 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         SIDX[i] = (int32_t)sh.p0 + i;
         if (i >= n) {
             P[i] = DX[i] = V[i] = A[i] = EXT[i] = D0[i] = D1[i] = make_float4(0, 0, 0, 0);
+            P[i].w = bitsf(i);
+            TH[i] = 0.0f;
             M[i] = (int32_t)PF_INACTIVE;
             ID[i] = i;
             continue;
@@ -623,7 +626,10 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
         const float r = v.radii[i];
         // WGPUComputeProg::restore uploads vel into del_pos and rot_vel into del_rot
         // (wgpu_prog.rs:1247-1248): an explicit displacement record
-        P[i] = make_float4(v.pos[2 * i], v.pos[2 * i + 1], r, v.rot[i]);
+        // the position record carries the particle id (its neighbours read it with the
+        // position); the rotation has its own array
+        P[i] = make_float4(v.pos[2 * i], v.pos[2 * i + 1], r, bitsf(i));
+        TH[i] = v.rot[i];
         DX[i] = make_float4(v.vel[2 * i], v.vel[2 * i + 1], v.rot_vel[i], 0.0f);
         float vw;
         std::memcpy(&vw, &VW_EXPLICIT, sizeof(vw));
@@ -749,6 +755,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     h2d(S.DX[lcur_] + p0, DX, sizeof(float4) * np);
     h2d(S.V[lcur_] + p0, V, sizeof(float4) * np);
     h2d(S.A + p0, A, sizeof(float4) * np);
+    h2d(S.TH + p0, TH, sizeof(float) * np);
     h2d(S.M + p0, M, sizeof(int32_t) * np);
     h2d(S.ID + p0, ID, sizeof(int32_t) * np);
     h2d(S.EXT + p0, EXT, sizeof(float4) * np);
@@ -1173,7 +1180,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
         c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np);
         c.take<int32_t>(np); c.take<int32_t>(np); c.take<int32_t>((size_t)np * KMAX);
         c.take<float>((size_t)np * KMAX); c.take<int2>(np); c.take<int4>(nrec); c.take<float4>(nrec);
-        c.take<int2>(nbs);
+        c.take<int2>(nbs); c.take<float>(np);
     };
     carve(cv);
     cv.base = staging(cv.off);
@@ -1192,6 +1199,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     int4* BR = cv.take<int4>(nrec);
     float4* BF = cv.take<float4>(nrec);
     int2* BS = cv.take<int2>(nbs);
+    float* TH = cv.take<float>(np);
     hipError_t e = hipSuccess;
     auto d2h = [&](void* dst, const void* src, size_t bytes) {
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_);
@@ -1199,6 +1207,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     d2h(P, S.P[cur_] + p0, sizeof(float4) * np);
     d2h(V, S.V[lcur_] + p0, sizeof(float4) * np);
     d2h(A, S.A + p0, sizeof(float4) * np);
+    d2h(TH, S.TH + p0, sizeof(float) * np);
     d2h(M, S.M + p0, sizeof(int32_t) * np);
     d2h(INV, INV_ + p0, sizeof(int32_t) * np);
     const bool rec = with_data_ && sh.params.record_data && v.data;
@@ -1231,7 +1240,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
         const float4 p = P[s], vv = V[s], aa = A[s];
         if (v.pos) { v.pos[2 * i] = p.x; v.pos[2 * i + 1] = p.y; }
         if (v.vel) { v.vel[2 * i] = vv.x; v.vel[2 * i + 1] = vv.y; }
-        if (v.rot) v.rot[i] = p.w;
+        if (v.rot) v.rot[i] = TH[s];
         if (v.rot_vel) v.rot_vel[i] = vv.z;
         if (v.acc) { v.acc[2 * i] = aa.x; v.acc[2 * i + 1] = aa.y; }
         if (v.rot_acc) v.rot_acc[i] = aa.z;

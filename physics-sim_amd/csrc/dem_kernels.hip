@@ -12,7 +12,7 @@
 //              partner-id order, store_forces, then the next drift, which raises the
 //              rebuild flag when the particle has moved more than 0.45 skin since the
 //              rebuild (a path bound in the velocity word gates the exact test)
-//              (stages 2a, 4, 5).  Per disc it streams P (x, y, r, rotation) and V
+//              (stages 2a, 4, 5).  Per disc it streams P (x, y, r, id), TH (rotation) and V
 //              (velocity, path word) in and out and reads the flag word and the list
 //              word; the drift's displacement is re-formed as velocity x dT unless an
 //              explicit record DX holds it.  The first launch of a dem_step call runs
@@ -533,6 +533,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const uint32_t pf = (uint32_t)a.M[s];
     const bool act = !(pf & PF_INACTIVE);
     float4 p = a.P[cur][s];
+    const float th = a.TH[s];   // rotation, advanced by the drift
     const int nlw = a.NL[s];     // issued with the other own-state loads, used if the narrow phase runs
     const int cap = (int)a.cap;  // KMAX * cap < 2^31 (checked at pool creation)
     // Issued with the own state as well, before the counts that say how many of them are
@@ -707,13 +708,13 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                     jj[t] = k < nc ? w : -1;
                 }
                 float4 qq[4], vq[4];
-                int iq[4];
+                int iq[4];   // partner ids: the last word of the position record
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     const int jl = jj[t] >= 0 ? jj[t] : s;
                     qq[t] = a.P[cur][jl];
                     vq[t] = a.V[jl];
-                    iq[t] = a.ID[jl];
+                    iq[t] = __float_as_int(qq[t].w);
                 }
                 // keeps the compiler from sinking any of these loads into the branches
                 // below, which would wait on each candidate in turn
@@ -1081,8 +1082,9 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     }
     px += dpx;
     py += dpy;
-    const float rot = p.w + dr;
-    a.P[nxt][s] = make_float4(px, py, rr, rot);
+    const float rot = th + dr;
+    a.TH[s] = rot;
+    a.P[nxt][s] = make_float4(px, py, rr, p.w);
     if (expl) a.DXn[s] = make_float4(dpx, dpy, dr, 0.0f);
     // Moved more than 0.45 skin since the neighbour lists were built: rebuild them after
     // this launch (every writer stores 1).  The path word bounds the distance from above
@@ -1465,13 +1467,13 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
             j[2 * r] = e[r] < e1[r] ? a.SIDX[e[r]] : -1;
             j[2 * r + 1] = e[r] + 1 < e1[r] ? a.SIDX[e[r] + 1] : -1;
         }
-        // positions and ids of the six candidates in one round of loads
+        // positions of the six candidates (ids in their last words) in one round of loads
         float4 q[6];
         int id[6];
 #pragma unroll
         for (int k = 0; k < 6; k++) {
             q[k] = j[k] >= 0 ? a.P[cur][j[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            id[k] = j[k] >= 0 ? a.ID[j[k]] : 0;
+            id[k] = __float_as_int(q[k].w);
         }
 #pragma unroll
         for (int k = 0; k < 6; k++) {
@@ -1584,6 +1586,7 @@ __global__ void k_reorder_move(ReorderArgs r) {
     r.dst.M[j] = m.x;
     r.dst.ID[j] = m.y;
     r.dst.A[j] = r.src.A[s];
+    r.dst.TH[j] = r.src.TH[s];
     if (r.with_ext) r.dst.EXT[j] = r.src.EXT[s];
     if (r.with_data) {
         r.dst.DATA0[j] = r.src.DATA0[s];
