@@ -134,7 +134,8 @@ constexpr int LBH_MAX = 128;
 // 32 bits, so loads and stores take the SGPR-base + VGPR-offset form and need no
 // 64-bit address arithmetic per access.  Every pool is below 4 GiB (the engine
 // checks its sizes at allocation).
-template <class T>
+// STRIDE: the bytes from one element to the next (records interleaved with others).
+template <class T, uint32_t STRIDE = (uint32_t)sizeof(T)>
 struct Pool {
     T* p = nullptr;
     Pool() = default;
@@ -142,17 +143,22 @@ struct Pool {
     template <class I>
     __device__ __forceinline__ T& operator[](I i) const {
         return *reinterpret_cast<T*>(reinterpret_cast<char*>(const_cast<typename std::remove_const<T>::type*>(p)) +
-                                     (uint32_t)i * (uint32_t)sizeof(T));
+                                     (uint32_t)i * STRIDE);
     }
 };
+
+// A slot's position record and velocity record of the same parity sit side by side in
+// one 32-byte record {x, y, r, id | vx, vy, w, word}: a neighbour's gather of both (and
+// of its id) reads one cache line.
+constexpr uint32_t PV_STRIDE = 32;
 
 struct DevArrays {
     // double-buffered neighbour-visible state.  Positions: P[cur] read, P[cur ^ 1]
     // written by a drift.  Velocities and explicit displacements flip after every launch
     // with the contact lists: V / DX read, Vn / DXn written.
-    Pool<float4> P[2];      // {x, y, r, id bits}: the id travels with the position its neighbours read
-    Pool<float4> V;         // {vx, vy, w, word: VW_EXPLICIT | path}  (after a LOM half: the mid-step velocity)
-    Pool<float4> Vn;
+    Pool<float4, PV_STRIDE> P[2];   // {x, y, r, id bits}: the id travels with the position its neighbours read
+    Pool<float4, PV_STRIDE> V;      // {vx, vy, w, word: VW_EXPLICIT | path}  (after a LOM half: the mid-step velocity)
+    Pool<float4, PV_STRIDE> Vn;     // V[k] and P[k] of one parity interleave (PV_STRIDE)
     Pool<float4> DX;        // {del_x, del_y, del_rot, 0}, valid where V.w has VW_EXPLICIT
     Pool<float4> DXn;
     Pool<float4> A;         // {ax, ay, alpha, 0}

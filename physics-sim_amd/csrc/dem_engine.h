@@ -29,8 +29,9 @@ int check_cell_size(const dem_grid& g, float rsmax);
 float dashpot_factor(float e);
 
 struct DevSet {
-    float4* P[2] = {nullptr, nullptr};    // by position parity (cur_)
-    float4* V[2] = {nullptr, nullptr};    // by list parity (lcur_), as DX
+    float4* PV[2] = {nullptr, nullptr};   // by parity: per slot {position, velocity record}, 32 B
+    float4* P[2] = {nullptr, nullptr};    // PV[k]: positions (stride 32 B), by position parity (cur_)
+    float4* V[2] = {nullptr, nullptr};    // PV[k] + 16 B: velocity records, by list parity (lcur_ == cur_)
     float4* DX[2] = {nullptr, nullptr};
     float4* A = nullptr;
     float* TH = nullptr;     // rotation (read and written by the particle's own thread only)

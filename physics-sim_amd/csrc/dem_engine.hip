@@ -48,6 +48,12 @@ double* static_scratch(int device);
 namespace {
 inline int32_t fbits(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
 inline float bitsf(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
+// Host view of one half of interleaved {position, velocity} records (staging buffers
+// laid out as on the device, so that each copy is one contiguous transfer).
+struct PVHalf {
+    float4* b;
+    float4& operator[](size_t i) const { return b[2 * i]; }
+};
 inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
@@ -111,7 +117,7 @@ Engine::~Engine() {
     if (stream_) hipStreamSynchronize(stream_);
     clear_graphs();   // before the buffers their nodes refer to
     for (auto& s : sets_) {
-        hipFree(s.P[0]); hipFree(s.P[1]); hipFree(s.V[0]); hipFree(s.V[1]); hipFree(s.DX[0]); hipFree(s.DX[1]);
+        hipFree(s.PV[0]); hipFree(s.PV[1]); hipFree(s.DX[0]); hipFree(s.DX[1]);
         hipFree(s.A); hipFree(s.TH); hipFree(s.M); hipFree(s.ID); hipFree(s.EXT); hipFree(s.DATA0); hipFree(s.DATA1);
     }
     for (auto& s : sets_) { hipFree(s.NL); hipFree(s.XR); hipFree(s.BRS); }
@@ -165,8 +171,11 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     const int64_t S = slot_cap_;
     for (auto& s : sets_) {
         hipError_t e = hipSuccess;
-        e = e ? e : dalloc(&s.P[0], S); e = e ? e : dalloc(&s.P[1], S);
-        e = e ? e : dalloc(&s.V[0], S); e = e ? e : dalloc(&s.V[1], S);
+        e = e ? e : dalloc(&s.PV[0], 2 * S); e = e ? e : dalloc(&s.PV[1], 2 * S);
+        for (int k = 0; k < 2; k++) {
+            s.P[k] = s.PV[k];
+            s.V[k] = s.PV[k] ? s.PV[k] + 1 : nullptr;
+        }
         e = e ? e : dalloc(&s.DX[0], S); e = e ? e : dalloc(&s.DX[1], S);
         e = e ? e : dalloc(&s.A, S); e = e ? e : dalloc(&s.TH, S); e = e ? e : dalloc(&s.M, S); e = e ? e : dalloc(&s.ID, S);
         e = e ? e : dalloc(&s.EXT, S);
@@ -224,7 +233,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     for (auto& s : sets_) e = e ? e : hipMemsetAsync(s.ID, 0xff, sizeof(int32_t) * S, stream_);
     for (auto& s : sets_)
         for (int b = 0; b < 2; b++) {
-            e = e ? e : hipMemsetAsync(s.V[b], 0, sizeof(float4) * S, stream_);
+            e = e ? e : hipMemsetAsync(s.PV[b], 0, 2 * sizeof(float4) * S, stream_);
             e = e ? e : hipMemsetAsync(s.DX[b], 0, sizeof(float4) * S, stream_);
         }
     std::vector<int32_t> blk(nblk, -1);
@@ -556,6 +565,9 @@ StepArgs Engine::make_args(const int2* lf) const {
 // Upload: reference layouts (state.rs:69-93) -> device layout, storage order = id order.
 int Engine::upload(int32_t sid, const dem_state_view& v) {
     if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    // a slot's position and velocity records of one parity share a record: the two
+    // parities flip together
+    if (cur_ != lcur_) { set_error("position and list parities differ"); return DEM_ESTATE; }
     SceneHost& sh = scenes_[sid];
     const int32_t n = v.n;
     if (n < 0 || n > sh.slot_cap) { set_error("particle count exceeds the scene capacity"); return DEM_ECAPACITY; }
@@ -575,7 +587,7 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     // the device layout is built in pinned staging memory and copied from there
     Carve cv;
     auto carve = [&](Carve& c) {
-        c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<int32_t>(np);
+        c.take<float4>(2 * (size_t)np); c.take<float4>(np); c.take<float4>(np); c.take<int32_t>(np);
         c.take<int32_t>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<int32_t>(np);
         c.take<int32_t>((size_t)np * KMAX); c.take<float>((size_t)np * KMAX); c.take<int32_t>(np);
         c.take<int32_t>(np); c.take<int2>(np); c.take<float>(np);
@@ -584,9 +596,9 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     cv.base = staging(cv.off);
     if (!cv.base) { set_error("upload: pinned staging allocation failed"); return DEM_ENOMEM; }
     cv.off = 0;
-    float4* P = cv.take<float4>(np);
+    float4* PV = cv.take<float4>(2 * (size_t)np);   // {position, velocity record} per slot
+    const PVHalf P{PV}, V{PV + 1};
     float4* DX = cv.take<float4>(np);
-    float4* V = cv.take<float4>(np);
     float4* A = cv.take<float4>(np);
     int32_t* M = cv.take<int32_t>(np);    // flag words
     int32_t* ID = cv.take<int32_t>(np);
@@ -751,9 +763,9 @@ int Engine::upload(int32_t sid, const dem_state_view& v) {
     auto h2d = [&](void* dst, const void* src, size_t bytes) {
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_);
     };
-    h2d(S.P[cur_] + p0, P, sizeof(float4) * np);
+    // positions and velocity records of one parity share their records (cur_ == lcur_)
+    h2d(S.PV[cur_] + 2 * p0, PV, 2 * sizeof(float4) * np);
     h2d(S.DX[lcur_] + p0, DX, sizeof(float4) * np);
-    h2d(S.V[lcur_] + p0, V, sizeof(float4) * np);
     h2d(S.A + p0, A, sizeof(float4) * np);
     h2d(S.TH + p0, TH, sizeof(float) * np);
     h2d(S.M + p0, M, sizeof(int32_t) * np);
@@ -1168,6 +1180,7 @@ int Engine::sync() { return check(hipStreamSynchronize(stream_), "hipStreamSynch
 // ---------------------------------------------------------------------------
 int Engine::download(int32_t sid, dem_state_view& v) {
     if (sid < 0 || sid >= (int32_t)scenes_.size() || !scenes_[sid].live) { set_error("no such scene"); return DEM_EINVAL; }
+    if (cur_ != lcur_) { set_error("position and list parities differ"); return DEM_ESTATE; }
     const SceneHost& sh = scenes_[sid];
     const int32_t n = sh.n, np = sh.np;
     if (v.n != n) { set_error("view.n does not match the scene's particle count"); return DEM_EINVAL; }
@@ -1177,7 +1190,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     const size_t nrec = (size_t)std::max(sh.nrec, 1), nbs = (size_t)std::max(sh.nb, 1);
     Carve cv;
     auto carve = [&](Carve& c) {
-        c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np);
+        c.take<float4>(2 * (size_t)np); c.take<float4>(np); c.take<float4>(np); c.take<float4>(np);
         c.take<int32_t>(np); c.take<int32_t>(np); c.take<int32_t>((size_t)np * KMAX);
         c.take<float>((size_t)np * KMAX); c.take<int2>(np); c.take<int4>(nrec); c.take<float4>(nrec);
         c.take<int2>(nbs); c.take<float>(np);
@@ -1186,8 +1199,8 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     cv.base = staging(cv.off);
     if (!cv.base) { set_error("download: pinned staging allocation failed"); return DEM_ENOMEM; }
     cv.off = 0;
-    float4* P = cv.take<float4>(np);
-    float4* V = cv.take<float4>(np);
+    float4* PV = cv.take<float4>(2 * (size_t)np);   // {position, velocity record} per slot
+    const PVHalf P{PV}, V{PV + 1};
     float4* A = cv.take<float4>(np);
     float4* D0 = cv.take<float4>(np);
     float4* D1 = cv.take<float4>(np);
@@ -1204,8 +1217,7 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     auto d2h = [&](void* dst, const void* src, size_t bytes) {
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_);
     };
-    d2h(P, S.P[cur_] + p0, sizeof(float4) * np);
-    d2h(V, S.V[lcur_] + p0, sizeof(float4) * np);
+    d2h(PV, S.PV[cur_] + 2 * p0, 2 * sizeof(float4) * np);   // cur_ == lcur_
     d2h(A, S.A + p0, sizeof(float4) * np);
     d2h(TH, S.TH + p0, sizeof(float) * np);
     d2h(M, S.M + p0, sizeof(int32_t) * np);
@@ -1532,12 +1544,13 @@ int Engine::bond_pairs(int32_t sid, int32_t max_bonds, float factor, float gap, 
     // bin the current positions (also refreshes the cell lists)
     int rc = debug_broad_phase(sid, nullptr, nullptr, nullptr);
     if (rc) return rc;
-    std::vector<float4> P(sh.np);
-    hipError_t e = hipMemcpyAsync(P.data(), sets_[set_].P[cur_] + sh.p0, sizeof(float4) * sh.np, hipMemcpyDeviceToHost, stream_);
+    std::vector<float4> PV(2 * (size_t)sh.np);   // {position, velocity record} per slot
+    hipError_t e = hipMemcpyAsync(PV.data(), sets_[set_].PV[cur_] + 2 * sh.p0, 2 * sizeof(float4) * sh.np,
+                                  hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     if ((rc = check(e, "bond pairs"))) return rc;
     float rmax = 0.0f;
-    for (int i = 0; i < sh.np; i++) rmax = std::max(rmax, P[i].z);
+    for (int i = 0; i < sh.np; i++) rmax = std::max(rmax, PV[2 * (size_t)i].z);
     const int reach = std::max(1, (int)std::ceil(((double)factor * 2.0 * rmax + gap) / sh.params.grid.cell_size));
     int32_t* d_cnt = nullptr;
     int32_t* d_j = nullptr;
