@@ -1381,38 +1381,54 @@ __device__ __forceinline__ void rb_scatter_slot(const StepArgs& args, int s, int
     else atomicAdd(&a.STAT[3], 1ull);
 }
 
-// Neighbour list of slot s: the slots within r_a + r_b + skin (x 1.0001) in the 3x3
-// stencil of its cell, sorted by partner id; more than NLMAX marks the particle for the
-// stencil fallback (count NL_OVERFLOW).  Records the build position.
-__device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int lbs) {
+// Neighbour list of slot s (a slot of the scene whose record is S): the slots within
+// r_a + r_b + skin (x 1.0001) in the 3x3 stencil of its cell, sorted by partner id; more
+// than NLMAX marks the particle for the stencil fallback (count NL_OVERFLOW).  Records
+// the build position.  Every load comes in branch-free rounds (indices clamped into the
+// scene's own ranges, entries past a row's end dropped by the test), so that a round's
+// loads are in flight together.
+constexpr int NL_ROW_STEP = 2;   // cell-list entries per stencil row and round
+__device__ __forceinline__ void build_nl_slot(const StepArgs& args, const SceneDev& S, int pos0, int s) {
     const DevArrays& a = args.a;
-    const int sid = args.blk_scene[s / lbs];
-    if (sid < 0) return;
-    const SceneDev& S = args.scenes[sid];
     const int cap = (int)a.cap;
     const int cur = args.cur;
-    const int2 m = make_int2(a.M[s], a.ID[s]);
+    const int mw = a.M[s];
     const float4 p = a.P[cur][s];
+    const int2 brs = a.BRS[s];
+    const float4 vs = a.V[s];
+    int cx, cy;
+    cell_of(S, p.x, p.y, cx, cy);
+    const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
+    // the three stencil rows' ranges of the cell list, rows off the grid empty
+    int e[3], e1[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const int ry = cy - 1 + r;
+        const int ryc = min(max(ry, 0), S.h - 1);
+        e[r] = a.START[S.c0 + ryc * S.w + xa];
+        e1[r] = a.START[S.c0 + ryc * S.w + xb + 1];
+    }
     a.XR[s] = make_float2(p.x, p.y);
-    a.V[s].w = __uint_as_float(__float_as_uint(a.V[s].w) & VW_EXPLICIT);   // path since the build: 0
-    if (m.x & PF_INACTIVE) {
+    a.V[s].w = __uint_as_float(__float_as_uint(vs.w) & VW_EXPLICIT);   // path since the build: 0
+    if (mw & PF_INACTIVE) {
         a.NL[s] = nl_word(0, 0);
         return;
     }
-    if (m.x & PF_LARGE) {
+    if (mw & PF_LARGE) {
         // a large body searches the stencil of the cell list every step
         a.NL[s] = nl_word(-1, 0);
         atomicAdd(&a.STAT[5], 1ull);
         return;
     }
-    int cx, cy;
-    cell_of(S, p.x, p.y, cx, cy);
-    const int xa = max(cx - 1, 0), xb = min(cx + 1, S.w - 1);
-    const int pos0 = S.p0 - a.START[S.c0];
-    const int rlo = max(cy - 1, 0), rhi = min(cy + 1, S.h - 1);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const int ry = cy - 1 + r;
+        const bool row = ry >= 0 && ry < S.h;
+        e[r] = row ? max(e[r] + pos0, S.p0) : 0;
+        e1[r] = row ? min(e1[r] + pos0, S.p0 + S.np) : 0;
+    }
     // live bond partners stay out of the list: their records carry the pair until the
     // record dies, when the step kernel adds the partner to the list
-    const int2 brs = a.BRS[s];
     auto bonded = [&](int oj) {
         if (brs.y <= brs.x) return false;
         for (int q = brs.x; q < brs.y; q++) {
@@ -1423,8 +1439,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
     };
     // The list is kept sorted by partner id (the step kernel forces its hits in that
     // order): candidates go into a register array by insertion, with compile-time
-    // indices only.  The three stencil rows advance together, two candidates each per
-    // round, so that six slot-id loads and then six position loads are in flight at once.
+    // indices only.
     int ls[NLMAX], li[NLMAX];   // slots and ids, ascending id
     int cnt = 0;
     auto add = [&](int j, int id) {
@@ -1447,45 +1462,35 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
         }
         cnt++;
     };
-    int e[3], e1[3];
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        const int ry = rlo + r;
-        if (ry <= rhi) {
-            e[r] = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
-            e1[r] = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
-        } else {
-            e[r] = 0;
-            e1[r] = 0;
-        }
-    }
     const float lr = p.z + S.skin;
+    const int lo = S.p0, hi = S.p0 + S.np - 1;   // the scene's cell-list positions
     while (e[0] < e1[0] || e[1] < e1[1] || e[2] < e1[2]) {
-        int j[6];
+        // NL_ROW_STEP entries of each row: their slots, then their positions (ids in the
+        // last words), each in one round of loads
+        constexpr int K = 3 * NL_ROW_STEP;
+        int j[K];
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
-            j[2 * r] = e[r] < e1[r] ? a.SIDX[e[r]] : -1;
-            j[2 * r + 1] = e[r] + 1 < e1[r] ? a.SIDX[e[r] + 1] : -1;
-        }
-        // positions of the six candidates (ids in their last words) in one round of loads
-        float4 q[6];
-        int id[6];
+        for (int r = 0; r < 3; r++)
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            q[k] = j[k] >= 0 ? a.P[cur][j[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            id[k] = __float_as_int(q[k].w);
-        }
+            for (int u = 0; u < NL_ROW_STEP; u++) j[r * NL_ROW_STEP + u] = a.SIDX[min(max(e[r] + u, lo), hi)];
+        asm volatile("" ::: "memory");
+        float4 q[K];
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            if (j[k] < 0 || j[k] == s) continue;
+        for (int k = 0; k < K; k++) q[k] = a.P[cur][j[k]];
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int r = k / NL_ROW_STEP, u = k % NL_ROW_STEP;
+            if (e[r] + u >= e1[r] || j[k] == s) continue;
             const float dx = q[k].x - p.x, dy = q[k].y - p.y;
             const float xx = dx * dx;
             const float yy = dy * dy;
             const float L = (lr + q[k].z) * 1.0001f;
-            if (xx + yy < L * L && (S.nlb == 0 || q[k].z <= S.rsmax) && !bonded(id[k])) add(j[k], id[k]);
+            const int idk = __float_as_int(q[k].w);
+            if (xx + yy < L * L && (S.nlb == 0 || q[k].z <= S.rsmax) && !bonded(idk)) add(j[k], idk);
         }
 #pragma unroll
-        for (int r = 0; r < 3; r++) e[r] += 2;
+        for (int r = 0; r < 3; r++) e[r] += NL_ROW_STEP;
     }
     // the scene's large bodies, from their list (never through the cells)
     for (int k = 0; k < S.nlb; k++) {
@@ -1496,7 +1501,7 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, int s, int l
         const float yy = dy * dy;
         const float L = (lr + q.z) * 1.0001f;
         if (xx + yy < L * L) {
-            const int idl = a.ID[jl];
+            const int idl = __float_as_int(q.w);
             if (!bonded(idl)) add(jl, idl);
         }
     }
@@ -1532,15 +1537,29 @@ __global__ void k_rb_scatter(StepArgs args, int nlb) {
 // candidates to the front of their units, out of cell order.
 __global__ void k_build_nl(StepArgs args, int nlb) {
     if (args.a.RB[args.cur] == 0) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&args.a.STAT[4], 1ull);   // rebuilds
     const int lbs = (int)blockDim.x;
-    for (int lb = (int)blockIdx.x; lb < nlb; lb += (int)gridDim.x) {
-        const int sid = args.blk_scene[lb];
-        if (sid < 0) continue;
-        const SceneDev& S = args.scenes[sid];
-        const int s = args.a.SIDX[lb * lbs + (int)threadIdx.x];   // the scene's list is a permutation of its slots
-        if (s >= S.p0 && s < S.p0 + S.np) build_nl_slot(args, s, lbs);
+    // the scene's list is a permutation of its slots: list position and slot share the
+    // scene.  One live scene: its record is read into registers before any store (read
+    // after one, every field would be loaded again).
+    auto block = [&](const SceneDev& S, int lb) {
+        const int e = lb * lbs + (int)threadIdx.x;   // the scene's list positions are its slot range
+        if (e < S.p0 || e >= S.p0 + S.np) return;
+        const int pos0 = S.p0 - args.a.START[S.c0];   // cell-list position -> slot-list index
+        const int s = args.a.SIDX[e];
+        if (s >= S.p0 && s < S.p0 + S.np) build_nl_slot(args, S, pos0, s);
+    };
+    if (args.one_scene >= 0) {
+        const SceneDev S = args.scenes[args.one_scene];
+        for (int lb = (int)blockIdx.x; lb < nlb; lb += (int)gridDim.x) block(S, lb);
+    } else {
+        for (int lb = (int)blockIdx.x; lb < nlb; lb += (int)gridDim.x) {
+            const int sid = __builtin_amdgcn_readfirstlane(args.blk_scene[lb]);
+            if (sid < 0) continue;
+            const SceneDev S = args.scenes[sid];
+            block(S, lb);
+        }
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&args.a.STAT[4], 1ull);   // rebuilds
 }
 
 // ---- (cell, id) order of the cell list (the debug view of stages 1-3) ----
