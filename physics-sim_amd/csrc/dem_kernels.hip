@@ -506,6 +506,10 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
     return make_float4(f.x, f.y, f.z, __int_as_float(pid));
 }
 
+// Candidates of the list path per round of loads (their slots, position and velocity
+// records in flight together).
+constexpr int STEP_ROUND = 4;
+
 template <bool F64, bool BONDS, bool LARGE>
 __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const DevArrays& a = args.a;
@@ -696,21 +700,27 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             // four candidates at a time: slots, then their positions, velocity records and
             // ids in one round of loads (before the overlap tests, so that a hit waits on
             // nothing)
-            for (int k0 = 0; k0 < nc; k0 += 4) {
+            for (int k0 = 0; k0 < nc; k0 += STEP_ROUND) {
                 // Loads without branches, so that all of them are in flight together: list
                 // rows past the count (rows < NLMAX exist for every slot) and, for a missing
                 // candidate, this slot's own records stand in and the test below drops them.
-                int jj[4];
+                // Rows 0-3 came with the own state.
+                int jj[STEP_ROUND];
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
+                for (int t = 0; t < STEP_ROUND; t++) {
                     const int k = k0 + t;
-                    const int w = k == 0 ? nl_first(w0) : (k0 == 0 ? nlr[t - 1] : a.NL[min(k, NLMAX - 1) * cap + s]);
+                    int w;
+                    if (k == 0) w = nl_first(w0);
+                    else if (k == 1) w = nlr[0];
+                    else if (k == 2) w = nlr[1];
+                    else if (k == 3) w = nlr[2];
+                    else w = a.NL[min(k, NLMAX - 1) * cap + s];
                     jj[t] = k < nc ? w : -1;
                 }
-                float4 qq[4], vq[4];
-                int iq[4];   // partner ids: the last word of the position record
+                float4 qq[STEP_ROUND], vq[STEP_ROUND];
+                int iq[STEP_ROUND];   // partner ids: the last word of the position record
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
+                for (int t = 0; t < STEP_ROUND; t++) {
                     const int jl = jj[t] >= 0 ? jj[t] : s;
                     qq[t] = a.P[cur][jl];
                     vq[t] = a.V[jl];
@@ -720,7 +730,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 // below, which would wait on each candidate in turn
                 asm volatile("" ::: "memory");
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
+                for (int t = 0; t < STEP_ROUND; t++) {
                     const int j = jj[t];
                     float lj = 0.0f;
                     if (j < 0 || j == s || !overlaps_l(qq[t], p, lj)) continue;
