@@ -425,6 +425,25 @@ __device__ __forceinline__ bool bonded_partner(const DevArrays& a, int b_lo, int
     return hit;
 }
 
+// The loaded words of one bond endpoint record: the record word {bond id, bonded} (the
+// bond pass's 8-byte copy; BR keeps the partner id for the step kernel and the lists)
+// and, for a live record, its {ft, fbt, theta_b}, the partner's position and velocity
+// records and the bond's {rest length, strengths, break word} (bond id clamped into the
+// scene's bonds; used only where the id is valid).
+struct RecIn {
+    int2 rb;
+    float4 bf, pb, vb, bl;
+};
+__device__ __forceinline__ void load_record(const DevArrays& a, const SceneDev& S, int cur, int q, int j, RecIn& in) {
+    in.rb = a.BRB[q];
+    if (in.rb.y != REC_DEAD) {
+        in.bf = a.BF[q];
+        in.pb = a.P[cur][j];
+        in.vb = a.V[j];
+        in.bl = a.BLS[S.b0 + min(max(in.rb.x, 0), max(S.nb - 1, 0))];
+    }
+}
+
 // One bond endpoint record of the particle `me` (position p, scene S; partner in slot j) in a launch
 // whose force half runs: the narrow-phase check of the bonded partner, the break
 // state seen from the partner, the bond or contact law; writes the record (and the
@@ -434,17 +453,15 @@ __device__ __forceinline__ bool bonded_partner(const DevArrays& a, int b_lo, int
 // dies here (2D_Simulation.wgsl:185-204, 230-262).
 template <bool F64, class DD>
 __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const SceneDev& S, const Self& me,
-                                                   const float4& p, bool narrow, int step, int cur, int q, int j,
-                                                   DD& dd) {
+                                                   const float4& p, bool narrow, int step, int q, int j,
+                                                   const RecIn& in, DD& dd) {
     const dem_uniform& u = S.u;
-    // {bond id, bonded word}: the 8-byte copy of the record word (BR keeps the partner
-    // id for the step kernel and the lists; both are written together)
-    const int2 rb = a.BRB[q];
+    const int2 rb = in.rb;
     if (rb.y == REC_DEAD) return make_float4(0.0f, 0.0f, 0.0f, __int_as_float(REC_RF_DEAD));
-    float4 bf = a.BF[q];
-    const float4 pb = a.P[cur][j];
-    const int pid = a.ID[j];
-    const float4 db = del_of(a, j, u.dT);
+    float4 bf = in.bf;
+    const float4 pb = in.pb;
+    const int pid = __float_as_int(pb.w);   // the partner's id travels with its position
+    const float4 db = del_from(a, j, in.vb, u.dT);
     int bonded = rb.y;
     // |p - pb| in f32: the narrow-phase test and every law below use this one value
     const float l = lenf(pb.x - p.x, pb.y - p.y);
@@ -467,7 +484,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
     // constants
     float4 bl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (bonded >= 0 && bid_ok) {
-        bl = a.BLS[S.b0 + bid];
+        bl = in.bl;
         // a break recorded by the partner in an earlier step is seen here
         const uint32_t bw = __float_as_uint(bl.w);
         if (brk_before(bw, (uint32_t)step)) bonded = (bw & 3u) ? (int)(bw & 3u) - 12 : a.BS[S.b0 + bid].x;
@@ -1120,24 +1137,29 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
 // block of slots.  Results: RF[q] = {fx, fy, moment, alive}, RD[2q..2q+1] = the
 // record's data[0..2] updates (with data recording).
 template <bool F64>
-__device__ __forceinline__ void bond_pass_body(const StepArgs& args, const SceneDev& S, int q, int i, int j, int2 lfw) {
+__device__ __forceinline__ void bond_pass_body(const StepArgs& args, const SceneDev& S, int q, int i, int j, int2 lfw,
+                                               const float4& p, const float4& vi, uint32_t mi, const RecIn& in) {
     const DevArrays& a = args.a;
-    const int cur = args.cur;
-    const float4 p = a.P[cur][i];
-    const float4 d = del_of(a, i, S.u.dT);
-    const int mat = (int)((uint32_t)a.M[i] & PF_MAT_MASK);
-    const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
+    const float4 d = del_from(a, i, vi, S.u.dT);
+    const int mat = (int)(mi & PF_MAT_MASK);
+    float rho = S.rho1, kn = S.kn1, ks = S.ks1;
+    if (S.nm > 1) {
+        const dem_material MA = a.MAT[S.m0 + (mat < S.nm ? mat : 0)];
+        rho = MA.density;
+        kn = MA.normal_stiffness;
+        ks = MA.shear_stiffness;
+    }
     Self me;
     me.x = p.x; me.y = p.y; me.r = p.z; me.drot = d.z; me.dx = d.x; me.dy = d.y;
-    me.kn = MA.normal_stiffness; me.ks = MA.shear_stiffness;
-    me.rho = MA.density; me.cn2 = S.cn2; me.dt = S.u.dT;
+    me.kn = kn; me.ks = ks;
+    me.rho = rho; me.cn2 = S.cn2; me.dt = S.u.dT;
     me.one = S.nm <= 1; me.kne = S.kne1; me.kse = S.kse1;
     const uint32_t fl = (uint32_t)lfw.x;
     const bool narrow = (fl & SF_SIM) && (fl & SF_NARROW);
     DataLog dl;
     dl.n = 0;
     dl.op[0] = dl.op[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    a.RF[q] = eval_bond_record<F64>(a, S, me, p, narrow, lfw.y, cur, q, j, dl);
+    a.RF[q] = eval_bond_record<F64>(a, S, me, p, narrow, lfw.y, q, j, in, dl);
     if (S.record) {
         a.RD[2 * q] = dl.op[0];
         a.RD[2 * q + 1] = dl.op[1];
@@ -1159,9 +1181,41 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
     const int2 lfw = args.lf[sid];
     const uint32_t fl = (uint32_t)lfw.x;
     if (!(fl & SF_ACTIVE) || !(fl & SF_SIM)) return;
-    if ((uint32_t)a.M[i] & PF_INACTIVE) return;   // an inactive owner evaluates nothing
-    if (F64 && S.use_f64) bond_pass_body<true>(args, S, q, i, j, lfw);
-    else bond_pass_body<false>(args, S, q, i, j, lfw);
+    const uint32_t mi = (uint32_t)a.M[i];
+    if (mi & PF_INACTIVE) return;   // an inactive owner evaluates nothing
+    const float4 p = a.P[args.cur][i];
+    const float4 vi = a.V[i];
+    RecIn in;
+    load_record(a, S, args.cur, q, j, in);
+    if (F64 && S.use_f64) bond_pass_body<true>(args, S, q, i, j, lfw, p, vi, mi, in);
+    else bond_pass_body<false>(args, S, q, i, j, lfw, p, vi, mi, in);
+}
+
+// The bond pass of a launch of one f32 scene: the scene record and flags are read into
+// registers once, and each record's loads come in two rounds (its slots and record
+// word; then both endpoints' position and velocity records, the owner's flag word, the
+// force record and the bond record), each issued together.
+__global__ void __launch_bounds__(256) k_bond_forces1(StepArgs args, int nrec) {
+    const DevArrays& a = args.a;
+    const int sid = args.one_scene;
+    const SceneDev S = args.scenes[sid];
+    const int2 lfw = args.lf[sid];
+    const uint32_t fl = (uint32_t)lfw.x;
+    if (!(fl & SF_ACTIVE) || !(fl & SF_SIM)) return;
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= nrec) return;
+    const int i = a.RS[q];   // owner slot (-1: no record)
+    const int j = a.RP[q];   // partner slot
+    if (i < 0) return;
+    const int cur = args.cur;
+    const uint32_t mi = (uint32_t)a.M[i];
+    const float4 p = a.P[cur][i];
+    const float4 vi = a.V[i];
+    RecIn in;
+    load_record(a, S, cur, q, j, in);
+    asm volatile("" ::: "memory");   // the loads above go out together
+    if (mi & PF_INACTIVE) return;   // an inactive owner evaluates nothing
+    bond_pass_body<false>(args, S, q, i, j, lfw, p, vi, mi, in);
 }
 
 // Owner and partner slots of every record (after an upload and after each re-sort):
@@ -1948,6 +2002,7 @@ hipError_t launch_bond_forces(const StepArgs& args, int nrec, int lbs, bool f64,
     const int b = 256;
     const dim3 g((unsigned)((nrec + b - 1) / b));
     if (f64) launch_k(k_bond_forces<true>, g, dim3(b), st, ev0, ev1, args, nrec, lbs);
+    else if (args.one_scene >= 0) launch_k(k_bond_forces1, g, dim3(b), st, ev0, ev1, args, nrec);
     else launch_k(k_bond_forces<false>, g, dim3(b), st, ev0, ev1, args, nrec, lbs);
     return hipGetLastError();
 }
