@@ -746,10 +746,18 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 // keeps the compiler from sinking any of these loads into the branches
                 // below, which would wait on each candidate in turn
                 asm volatile("" ::: "memory");
+                // this round's new list entries, stored after its bodies: a store issued
+                // inside them would hold up every later wait on a load (the vector memory
+                // counter counts loads and stores in one queue)
+                int hid[STEP_ROUND];
+                float hft[STEP_ROUND];
+                uint32_t hits = 0u;
 #pragma unroll
                 for (int t = 0; t < STEP_ROUND; t++) {
                     const int j = jj[t];
                     float lj = 0.0f;
+                    hid[t] = 0;
+                    hft[t] = 0.0f;
                     if (j < 0 || j == s || !overlaps_l(qq[t], p, lj)) continue;
                     const int oj = iq[t];
                     if (BONDS) {
@@ -770,8 +778,15 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                     nfx += f.x;
                     nfy += f.y;
                     nm += f.z;
-                    a.CPn[n * cap + s] = oj;
-                    a.CFn[n * cap + s] = ft;
+                    hid[t] = oj;
+                    hft[t] = ft;
+                    hits |= 1u << t;
+                }
+#pragma unroll
+                for (int t = 0; t < STEP_ROUND; t++) {
+                    if (!((hits >> t) & 1u)) continue;
+                    a.CPn[n * cap + s] = hid[t];
+                    a.CFn[n * cap + s] = hft[t];
                     n++;
                 }
             }
@@ -1246,13 +1261,13 @@ __device__ __forceinline__ bool block_scene_f64(const StepArgs& args) {
     return sid >= 0 && args.scenes[sid].use_f64 != 0;
 }
 
-// Plain f32 scenes: 5 waves per SIMD, the most at which the list path (four candidates'
-// positions, velocity records and ids in flight) fits its registers (94 VGPRs) without
-// spilling; launches with a large body take
+// Plain f32 scenes: 4 waves per SIMD, at which the list path (four candidates' position
+// and velocity records in flight, a round's new list entries held until its end) fits
+// its registers without spilling; launches with a large body take
 // k_step_plain_large, whose stencil fallback also covers them.  Launches holding an
 // f64 scene (k_step_plain_f64) and bonded scenes carry more live state and keep the
 // default register budget; both handle large bodies.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_step_plain(StepArgs args) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_step_plain(StepArgs args) {
     step_slot<false, false, false>(args);
 }
 // 4 waves per SIMD: 97 VGPRs without spills (the budget of 5 waves spills the
