@@ -84,6 +84,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, scenes/s")
     ap.add_argument("--serve-substeps", type=int, default=50)
     ap.add_argument("--no-bonded", action="store_true", help="skip the 1M-disc bonded-assembly leg")
+    ap.add_argument("--bonded-window", type=int, default=1000,
+                    help="steps of the bonded leg's HIP-event window, also its sampling interval")
     ap.add_argument("--bonded-steps", type=int, default=50000,
                     help="steps of the bonded-assembly pull (50,000: strain 0.05)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -200,17 +202,18 @@ def bonded_bench(args):
     builder, parallel bonds with Weibull(4, 30) strengths, bottom and top rows gripped
     and pulled apart at 1e-3 strain per 1,000 steps): --bonded-steps steps (50,000:
     strain 0.05) with the stress-strain trace and the broken-bond count sampled from
-    the device every 1,000 steps (dem_group_sums on the two grips,
-    dem_broken_bond_count), then one 1,000-step window under HIP-event timing for the
-    per-kernel times and the roofline of the bond pass and the bonded step kernel."""
+    the device every --bonded-window steps (1,000: dem_group_sums on the two grips,
+    dem_broken_bond_count), the first window under HIP-event timing for the per-kernel
+    times and the roofline of the bond pass and the bonded step kernel."""
     from psim import scenes
     from psim.compute import ComputeProg
     t = time.perf_counter()
-    s, st, bottom, top, info = scenes.bonded_assembly()
+    s, st, bottom, top, info = scenes.bonded_assembly(
+        progress=lambda d, tot: print("bench: bonded relaxation %d / %d steps" % (d, tot), file=sys.stderr, flush=True))
     t_build = time.perf_counter() - t
     n, nb = st.p_count, info["bonds"]
     prog = ComputeProg(s, st, bond_capacity=nb, record_data=False, reorder_interval=args.reorder)
-    every = 1000
+    every = max(1, args.bonded_window)
     trace = []
 
     def sample(done):
@@ -219,7 +222,7 @@ def bonded_bench(args):
         stress = 0.5 * (gb["force"][1] - gt["force"][1]) / info["width"]
         trace.append([done, strain, stress, prog.broken_bond_count()])
 
-    # the first 1,000 steps under HIP-event timing (the bonds still intact: the window
+    # the first window under HIP-event timing (the bonds still intact: the window
     # of the per-kernel figures), the rest timed by the host clock
     c_a = prog.counters()
     prog.timing(True)
@@ -227,6 +230,7 @@ def bonded_bench(args):
     prog.sync()
     tm = prog.timing(False)
     c_b = prog.counters()
+    print("bench: bonded event window done (%d steps)" % every, file=sys.stderr, flush=True)
     sample(every)
     done = every
     prog.sync()
@@ -236,7 +240,7 @@ def bonded_bench(args):
         prog.step(k)
         done += k
         sample(done)
-        if done % 10000 == 0:
+        if args.bonded_steps <= 10000 or done % 10000 == 0:
             print("bench: bonded pull %d / %d steps" % (done, args.bonded_steps), file=sys.stderr, flush=True)
     prog.sync()
     wall = time.perf_counter() - t0

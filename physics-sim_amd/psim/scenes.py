@@ -305,8 +305,9 @@ def bonded_assembly(rows: int = 1024, per_row: int = 1024, seed: int = 3, rmin: 
                     phi: float = 0.89, relax_steps: int = 3000, quench_every: int = 500, gap_frac: float = 0.1,
                     kn: float = 1500.0, ks: float = 1500.0, mu: float = 0.5, dt: float = 1e-4,
                     strain_per_1000: float = 1e-3, strength_k: float = 4.0, strength_scale: float = 30.0,
-                    margin: float = 0.25):
+                    margin: float = 0.25, progress=None):
     """Bonded-particle assembly for a tensile test (1,048,576 discs at the default size).
+    `progress(done, total)`, if given, is called after each quench of the relaxation.
 
     1. assembly_packing: a seeded triangular lattice of r ~ U[0.8, 1.2] discs at area
        fraction 0.86, relaxed on the device (ComputeProg: frictionless, no gravity,
@@ -349,6 +350,8 @@ def bonded_assembly(rows: int = 1024, per_row: int = 1024, seed: int = 3, rmin: 
             cur.vel[:] = 0.0
             cur.rot_vel[:] = 0.0
             prog.restore(cur)
+            if progress is not None:
+                progress(done, relax_steps)
         built = prog.regen_bonds(s0, factor=1.0, gap=gap_frac * rmin)
     finally:
         prog.close()
