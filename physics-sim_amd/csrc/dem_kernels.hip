@@ -1915,15 +1915,55 @@ __global__ void __launch_bounds__(64) k_large_search(StepArgs args) {
     const int pos0 = S.p0 - a.START[S.c0];
     const int reach = max(1, (int)ceilf((p.z + S.rsmax + S.skin) * S.inv_cs) + 1);
     const int xa = max(cx - reach, 0), xb = min(cx + reach, S.w - 1);
-    for (int ry = max(cy - reach, 0); ry <= min(cy + reach, S.h - 1); ry++) {
-        const int e0 = max(a.START[S.c0 + ry * S.w + xa] + pos0, S.p0);
-        const int e1 = min(a.START[S.c0 + ry * S.w + xb + 1] + pos0, S.p0 + S.np);
-        for (int e = e0 + (int)threadIdx.x; e < e1; e += 64) {
-            const int j = a.SIDX[e];
-            const float4 q = a.P[args.cur][j];
-            if (j != s && q.z <= S.rsmax && overlaps(q, p)) {
-                const int w = atomicAdd(&cnt, 1);
-                if (w < LBH_MAX) hits[w] = j;
+    const int ry0 = max(cy - reach, 0), nrows = min(cy + reach, S.h - 1) - ry0 + 1;
+    const int lane = (int)threadIdx.x;
+    // Up to 64 stencil rows at a time: each lane takes one row's range of the cell list,
+    // a prefix over the lanes lays the rows' entries end to end, and every lane then takes
+    // LS_K entries per round (its row found by a binary search over the prefix), their
+    // slots and then their positions each in one round of loads.
+    constexpr int LS_K = 4;
+    for (int rb = 0; rb < nrows; rb += 64) {
+        int e0 = 0, len = 0;
+        if (rb + lane < nrows) {
+            const int ry = ry0 + rb + lane;
+            const int s0 = a.START[S.c0 + ry * S.w + xa];
+            const int s1 = a.START[S.c0 + ry * S.w + xb + 1];
+            e0 = max(s0 + pos0, S.p0);
+            len = max(min(s1 + pos0, S.p0 + S.np) - e0, 0);
+        }
+        const int incl = wave_incl_scan(len);
+        const int total = __shfl(incl, 63, 64);
+        for (int t0 = 0; t0 < total; t0 += 64 * LS_K) {
+            int jj[LS_K];
+#pragma unroll
+            for (int u = 0; u < LS_K; u++) {
+                const int t = t0 + u * 64 + lane;
+                const int tc = min(t, total - 1);
+                // the first row whose inclusive prefix passes tc (all 64 lanes take part)
+                int lo = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    const int probe = __shfl(incl, lo + step - 1, 64);
+                    if (probe <= tc) lo += step;
+                }
+                const int row_incl = __shfl(incl, lo, 64);
+                const int row_len = __shfl(len, lo, 64);
+                const int row_e0 = __shfl(e0, lo, 64);
+                const int e = row_e0 + tc - (row_incl - row_len);
+                jj[u] = t < total ? a.SIDX[e] : -1;
+            }
+            asm volatile("" ::: "memory");
+            float4 q[LS_K];
+#pragma unroll
+            for (int u = 0; u < LS_K; u++) q[u] = a.P[args.cur][jj[u] >= 0 ? jj[u] : s];
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int u = 0; u < LS_K; u++) {
+                const int j = jj[u];
+                if (j >= 0 && j != s && q[u].z <= S.rsmax && overlaps(q[u], p)) {
+                    const int w = atomicAdd(&cnt, 1);
+                    if (w < LBH_MAX) hits[w] = j;
+                }
             }
         }
     }
