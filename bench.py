@@ -150,8 +150,11 @@ def serving_bench(args):
     from psim.autotune import Tuner
     from psim.serve import Server
     stream = scenes.serving_stream(n_scenes=args.serve_scenes, rate=args.serve_rate)
-    # cell sizes per scene class (kind, size bucket) at the server's block size
-    tuner = Tuner(blocks=(256,), steps=60, warmup=5)   # long enough to include list rebuilds
+    # cell sizes per scene class (kind, size bucket) at the server's block size, from the
+    # default cell up: in a batch a list rebuild covers every resident scene, so the thin
+    # skin of a smaller cell, which a scene alone may afford, makes the whole batch
+    # rebuild more often
+    tuner = Tuner(blocks=(256,), factors=(1.0, 1.25), steps=60, warmup=5)   # long enough to include list rebuilds
     grids = [tuner.grid_for("box" if i % 2 == 0 else "chain", s, st, block=256)
              for i, (_, s, st, _) in enumerate(stream)]
     slots = sum((st.p_count + 255) // 256 * 256 for _, _, st, _ in stream)

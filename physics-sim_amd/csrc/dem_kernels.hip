@@ -1181,18 +1181,10 @@ __device__ __forceinline__ void bond_pass_body(const StepArgs& args, const Scene
     }
 }
 
-// 8 waves per SIMD: both variants fit 64 VGPRs without spills.
+// One record q (owner slot i, partner slot j) of scene sid whose record is S.
 template <bool F64>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_bond_forces(StepArgs args, int nrec, int lbs) {
+__device__ __forceinline__ void bond_record(const StepArgs& args, const SceneDev& S, int sid, int q, int i, int j) {
     const DevArrays& a = args.a;
-    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (q >= nrec) return;
-    const int i = a.RS[q];   // owner slot (-1: no record)
-    if (i < 0) return;
-    const int j = a.RP[q];   // partner slot
-    const int sid = args.blk_scene[i / lbs];
-    if (sid < 0) return;
-    const SceneDev& S = args.scenes[sid];
     const int2 lfw = args.lf[sid];
     const uint32_t fl = (uint32_t)lfw.x;
     if (!(fl & SF_ACTIVE) || !(fl & SF_SIM)) return;
@@ -1204,6 +1196,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
     load_record(a, S, args.cur, q, j, in);
     if (F64 && S.use_f64) bond_pass_body<true>(args, S, q, i, j, lfw, p, vi, mi, in);
     else bond_pass_body<false>(args, S, q, i, j, lfw, p, vi, mi, in);
+}
+
+// A scene's records are contiguous, so a wave's records nearly always share one scene:
+// its record is then read into registers once (a wave that spans scenes reads each
+// lane's through the table).  5 waves per SIMD: both variants fit without spills.
+template <bool F64>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_bond_forces(StepArgs args, int nrec, int lbs) {
+    const DevArrays& a = args.a;
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= nrec) return;
+    const int i = a.RS[q];   // owner slot (-1: no record)
+    if (i < 0) return;
+    const int j = a.RP[q];   // partner slot
+    const int sid = args.blk_scene[i / lbs];
+    if (sid < 0) return;
+    const int s0 = __builtin_amdgcn_readfirstlane(sid);
+    if (__ballot(sid != s0) == 0) {
+        const SceneDev S = args.scenes[s0];
+        bond_record<F64>(args, S, s0, q, i, j);
+    } else {
+        bond_record<F64>(args, args.scenes[sid], sid, q, i, j);
+    }
 }
 
 // The bond pass of a launch of one f32 scene: the scene record and flags are read into
