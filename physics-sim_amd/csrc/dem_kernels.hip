@@ -1481,7 +1481,7 @@ __device__ __forceinline__ void rb_scatter_slot(const StepArgs& args, int s, int
 // scene's own ranges, entries past a row's end dropped by the test), so that a round's
 // loads are in flight together.
 constexpr int NL_ROW_STEP = 2;   // cell-list entries per stencil row and round
-__device__ __forceinline__ void build_nl_slot(const StepArgs& args, const SceneDev& S, int pos0, int s) {
+__device__ __forceinline__ void build_nl_slot(const StepArgs& args, const SceneDev& S, int pos0, int s, int2* nl_lds) {
     const DevArrays& a = args.a;
     const int cap = (int)a.cap;
     const int cur = args.cur;
@@ -1530,29 +1530,16 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, const SceneD
         }
         return false;
     };
-    // The list is kept sorted by partner id (the step kernel forces its hits in that
-    // order): candidates go into a register array by insertion, with compile-time
-    // indices only.
-    int ls[NLMAX], li[NLMAX];   // slots and ids, ascending id
+    // The list is sorted by partner id (the step kernel forces its hits in that order):
+    // candidates are appended to this thread's column of the block's LDS area (entry k
+    // of thread t at k * blockDim.x + t, so that a wave's lanes hit distinct banks) and
+    // sorted once at the end.  Appending costs one LDS store, so a wave's candidate
+    // rounds, which run whenever any lane still has candidates, stay cheap.
+    int2* const lst = nl_lds + threadIdx.x;   // {partner id, slot}, stride blockDim.x
+    const int lstride = (int)blockDim.x;
     int cnt = 0;
     auto add = [&](int j, int id) {
-        if (cnt < NLMAX) {
-            int pos = 0;
-#pragma unroll
-            for (int t = 0; t < NLMAX; t++) pos += (t < cnt && li[t] < id) ? 1 : 0;
-#pragma unroll
-            for (int t = NLMAX - 1; t > 0; t--)
-                if (t > pos && t <= cnt) {
-                    ls[t] = ls[t - 1];
-                    li[t] = li[t - 1];
-                }
-#pragma unroll
-            for (int t = 0; t < NLMAX; t++)
-                if (t == pos) {
-                    ls[t] = j;
-                    li[t] = id;
-                }
-        }
+        if (cnt < NLMAX) lst[cnt * lstride] = make_int2(id, j);
         cnt++;
     };
     const float lr = p.z + S.skin;
@@ -1598,12 +1585,23 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, const SceneD
             if (!bonded(idl)) add(jl, idl);
         }
     }
+    int first = 0;
     if (cnt <= NLMAX) {
-#pragma unroll
-        for (int t = 1; t < NLMAX; t++)
-            if (t < cnt) a.NL[t * cap + s] = ls[t];
+        // insertion sort of this thread's entries by partner id (ids are unique)
+        for (int k = 1; k < cnt; k++) {
+            const int2 x = lst[k * lstride];
+            int m = k - 1;
+            for (; m >= 0; m--) {
+                const int2 y = lst[m * lstride];
+                if (y.x < x.x) break;
+                lst[(m + 1) * lstride] = y;
+            }
+            lst[(m + 1) * lstride] = x;
+        }
+        for (int k = 1; k < cnt; k++) a.NL[k * cap + s] = lst[k * lstride].y;
+        if (cnt > 0) first = lst[0].y;
     }
-    a.NL[s] = nl_word(cnt <= NLMAX ? cnt : -1, cnt > 0 && cnt <= NLMAX ? ls[0] : 0);
+    a.NL[s] = nl_word(cnt <= NLMAX ? cnt : -1, first);
     if (cnt > NLMAX) atomicAdd(&a.STAT[5], 1ull);   // particles on the stencil fallback
 }
 
@@ -1628,8 +1626,12 @@ __global__ void k_rb_scatter(StepArgs args, int nlb) {
 // The lists are built in cell-list order (thread e takes the slot at list position e),
 // so that a wave's stencils overlap whatever the storage order: the re-sort moves
 // candidates to the front of their units, out of cell order.
-__global__ void k_build_nl(StepArgs args, int nlb) {
+// The list build's LDS: NLMAX {id, slot} entries per thread of a block of at most 256.
+constexpr int NL_LDS_THREADS = 256;
+
+__global__ void __launch_bounds__(NL_LDS_THREADS) k_build_nl(StepArgs args, int nlb) {
     if (args.a.RB[args.cur] == 0) return;
+    __shared__ int2 nl_lds[NLMAX * NL_LDS_THREADS];
     const int lbs = (int)blockDim.x;
     // the scene's list is a permutation of its slots: list position and slot share the
     // scene.  One live scene: its record is read into registers before any store (read
@@ -1639,7 +1641,7 @@ __global__ void k_build_nl(StepArgs args, int nlb) {
         if (e < S.p0 || e >= S.p0 + S.np) return;
         const int pos0 = S.p0 - args.a.START[S.c0];   // cell-list position -> slot-list index
         const int s = args.a.SIDX[e];
-        if (s >= S.p0 && s < S.p0 + S.np) build_nl_slot(args, S, pos0, s);
+        if (s >= S.p0 && s < S.p0 + S.np) build_nl_slot(args, S, pos0, s, nl_lds);
     };
     if (args.one_scene >= 0) {
         const SceneDev S = args.scenes[args.one_scene];
