@@ -60,13 +60,19 @@ __host__ __device__ inline bool brk_before(uint32_t w, uint32_t step) {
 //               changed the velocity after it was formed.  Set after a wall bounce,
 //               when the drift kept the loaded displacement (coll_cont[3] != 0), after
 //               a force half without a drift (the velocity is kicked again) and at upload.
-//  VW_PATH      distance travelled since the neighbour lists were built, an upper bound
-//               in units of 0.45 skin / 2^24 (sum of |dx| + |dy| per drift, rounded up,
-//               saturating): the position at the build is read only once it passes 2^24.
+//  VW_MASK      (bits 16..29) where M has PF_ALIGNED: bit k set when neighbour-list
+//               candidate k is a contact, its tangential force in FT row k.
+//  VW_PATH      (bits 0..15) distance travelled since the neighbour lists were built, an
+//               upper bound in units of 0.45 skin / 2^14 (sum of |dx| + |dy| per drift,
+//               rounded up, saturating): the position at the build is read only once it
+//               passes 2^14 (a drift adds at most one unit of rounding).
 constexpr uint32_t VW_EXPLICIT = 1u << 31;
-constexpr uint32_t VW_PATH = (1u << 31) - 1u;
-constexpr uint32_t VW_PATH_TRIG = 1u << 24;
-constexpr uint32_t VW_PATH_SAT = 1u << 30;
+constexpr int VW_MASK_SHIFT = 16;
+constexpr uint32_t VW_MASK = ((1u << KMAX) - 1u) << VW_MASK_SHIFT;
+constexpr uint32_t VW_PATH = (1u << 16) - 1u;
+constexpr uint32_t VW_PATH_TRIG = 1u << 14;
+constexpr uint32_t VW_PATH_SAT = 1u << 15;
+static_assert(VW_MASK_SHIFT + KMAX <= 30, "the contact mask sits below VW_EXPLICIT");
 
 // per-particle flag bits (word M bits 16..24; bits 0..15 hold the material id)
 enum : uint32_t {
@@ -87,6 +93,12 @@ enum : uint32_t {
 constexpr int PF_CC_SHIFT = 25;
 constexpr uint32_t PF_CC_MASK = 0x1fu << PF_CC_SHIFT;
 __host__ __device__ inline int cc_of(uint32_t mx) { return (int)((mx & PF_CC_MASK) >> PF_CC_SHIFT); }
+// Bit 30 of M: the contact list is held beside the neighbour list (the contact mask in
+// the velocity word, VW_MASK, and the tangential forces in FT rows of the candidates)
+// instead of in CP / CF.  Only the list path of the step kernel writes this form; the
+// list build, the re-sort, the other paths and reads from the host turn it back into
+// the partner-id list (contacts_from_aligned).
+constexpr uint32_t PF_ALIGNED = 1u << 30;
 
 // per-step flags (one word per scene and step)
 enum : uint32_t {
@@ -175,6 +187,9 @@ struct DevArrays {
     Pool<float> CF;         // tangential force
     Pool<int32_t> CPn;
     Pool<float> CFn;
+    // [KMAX][cap] tangential force of neighbour-list candidate k where M has PF_ALIGNED,
+    // read and written in place by the particle's own thread (valid where VW_MASK says)
+    Pool<float> FT;
     int64_t cap;       // particle pool size (slot stride of CP / CF)
     // bond endpoint records: ranges indexed by (scene p0 + particle id), each list sorted
     // by partner id; records never move

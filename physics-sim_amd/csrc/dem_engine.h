@@ -214,6 +214,7 @@ private:
     dem_material* MAT_ = nullptr;
     int32_t *KEY_ = nullptr, *RANK_ = nullptr, *CNT_ = nullptr, *START_ = nullptr, *SIDX_ = nullptr, *INV_ = nullptr;
     int32_t* RB_ = nullptr;
+    float* FT_ = nullptr;     // [KMAX][cap] tangential forces of list candidates (PF_ALIGNED slots)
     int32_t* LB_ = nullptr;   // large-body ids, LB_MAX per scene
     int32_t* LBH_ = nullptr;  // small discs overlapping each large body (LBH_MAX per large body)
     int32_t* LBHC_ = nullptr; // their counts

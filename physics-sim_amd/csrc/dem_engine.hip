@@ -36,6 +36,7 @@ hipError_t launch_record_slots(const StepArgs& args, int nrec, int lbs, hipStrea
                                hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, int np, double* out, hipStream_t st);
 hipError_t launch_count_contacts(const int32_t* m, int p0, int np, unsigned long long* out, hipStream_t st);
+hipError_t launch_contacts_by_id(const StepArgs& args, int p0, int np, hipStream_t st);
 hipError_t launch_bond_pairs(const StepArgs& args, int nblocks, int block, int reach, int maxb, float factor,
                              float gap, int32_t* out_cnt, int32_t* out_j, hipStream_t st);
 hipError_t launch_large_search(const StepArgs& args, int nblocks, hipStream_t st, hipEvent_t ev0 = nullptr,
@@ -122,6 +123,7 @@ Engine::~Engine() {
     }
     for (auto& s : sets_) { hipFree(s.NL); hipFree(s.XR); hipFree(s.BRS); }
     for (auto& l : lists_) { hipFree(l.CP); hipFree(l.CF); }
+    hipFree(FT_);
     hipFree(BOFF_); hipFree(BR_); hipFree(BF_); hipFree(BS_); hipFree(BLS_); hipFree(BRB_); hipFree(MAT_);
     hipFree(KEY_); hipFree(RANK_); hipFree(CNT_); hipFree(START_); hipFree(SIDX_); hipFree(INV_);
     hipFree(RB_); hipFree(ORDER_); hipFree(NEWSLOT_); hipFree(RO_); hipFree(RF_); hipFree(RD_);
@@ -188,6 +190,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
         e = e ? e : dalloc(&l.CP, S * KMAX); e = e ? e : dalloc(&l.CF, S * KMAX);
         if ((rc = check(e, "hipMalloc contact lists"))) return rc;
     }
+    if ((rc = check(dalloc(&FT_, S * KMAX), "hipMalloc contact lists"))) return rc;
     hipError_t e = hipSuccess;
     e = e ? e : dalloc(&BOFF_, S); e = e ? e : dalloc(&BR_, rec_cap_); e = e ? e : dalloc(&BF_, rec_cap_);
     e = e ? e : dalloc(&RO_, rec_cap_); e = e ? e : dalloc(&RF_, rec_cap_);
@@ -223,6 +226,7 @@ int Engine::init(int64_t slot_cap, int64_t cell_cap, int64_t bond_cap, int64_t r
     e = e ? e : hipMemsetAsync(ORDER_, 0, sizeof(int32_t) * S, stream_);
     e = e ? e : hipMemsetAsync(NEWSLOT_, 0, sizeof(int32_t) * S, stream_);
     e = e ? e : hipMemsetAsync(RB_, 0, 2 * sizeof(int32_t), stream_);
+    e = e ? e : hipMemsetAsync(FT_, 0, sizeof(float) * S * KMAX, stream_);
     e = e ? e : hipMemsetAsync(RO_, 0xff, sizeof(int32_t) * rec_cap_, stream_);   // no records
     e = e ? e : hipMemsetAsync(RS_, 0xff, sizeof(int32_t) * rec_cap_, stream_);
     e = e ? e : hipMemsetAsync(RP_, 0, sizeof(int32_t) * rec_cap_, stream_);
@@ -533,6 +537,7 @@ DevArrays Engine::arrays(int set) const {
     const ContactLists& ln = lists_[lcur_ ^ 1];
     a.CP = lc.CP; a.CF = lc.CF;
     a.CPn = ln.CP; a.CFn = ln.CF;
+    a.FT = FT_;
     a.cap = slot_cap_;
     a.BOFF = BOFF_; a.BR = BR_; a.BF = BF_; a.BS = BS_; a.BLS = BLS_; a.BRB = BRB_; a.MAT = MAT_;
     a.KEY = KEY_; a.RANK = RANK_; a.CNT = CNT_; a.START = START_; a.SIDX = SIDX_; a.INV = INV_;
@@ -1213,7 +1218,8 @@ int Engine::download(int32_t sid, dem_state_view& v) {
     float4* BF = cv.take<float4>(nrec);
     int2* BS = cv.take<int2>(nbs);
     float* TH = cv.take<float>(np);
-    hipError_t e = hipSuccess;
+    // contact lists kept beside the neighbour lists become partner-id lists first
+    hipError_t e = launch_contacts_by_id(make_args(d_lf_), (int)p0, np, stream_);
     auto d2h = [&](void* dst, const void* src, size_t bytes) {
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_);
     };
@@ -1510,7 +1516,9 @@ int Engine::debug_contacts(int32_t sid, int32_t max_per, int32_t* counts, int32_
     const int np = sh.np;
     std::vector<int32_t> M(np);
     std::vector<int32_t> CP((size_t)np * KMAX), INV(np);
-    hipError_t e = hipMemcpyAsync(M.data(), sets_[set_].M + sh.p0, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
+    hipError_t e = launch_contacts_by_id(make_args(d_lf_), (int)sh.p0, np, stream_);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(M.data(), sets_[set_].M + sh.p0, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);
     for (int q = 0; q < KMAX && e == hipSuccess; q++)
         e = hipMemcpyAsync(CP.data() + (size_t)q * np, lists_[lcur_].CP + (int64_t)q * slot_cap_ + sh.p0,
                            sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream_);

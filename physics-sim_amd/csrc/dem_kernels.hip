@@ -368,12 +368,38 @@ __device__ __forceinline__ void clear_tile_sums(const DevArrays& a) {
     for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < a.ntiles; i += stride) a.TSUM[i] = 0;
 }
 
-// Carries the contact list of slot s (c entries) over to the next-list buffers.
-__device__ __forceinline__ void copy_list(const DevArrays& a, int s, int c) {
+// Carries the contact list of slot s (c entries) over to the next-list buffers; a list
+// held beside the neighbour list (PF_ALIGNED in flag word mw) stays where it is, its
+// mask travelling in the velocity word.
+__device__ __forceinline__ void copy_list(const DevArrays& a, int s, uint32_t mw) {
+    if (mw & PF_ALIGNED) return;
+    const int c = cc_of(mw);
     const int cap = (int)a.cap;
     for (int q = 0; q < c && q < KMAX; q++) {
         a.CPn[q * cap + s] = a.CP[q * cap + s];
         a.CFn[q * cap + s] = a.CF[q * cap + s];
+    }
+}
+
+__device__ __forceinline__ uint32_t contact_mask(const float4& v) {
+    return (__float_as_uint(v.w) & VW_MASK) >> VW_MASK_SHIFT;
+}
+
+// The contact list of slot s held beside its neighbour list (contact mask m over the
+// candidates, tangential forces in FT) written as a partner-id list into CPd / CFd at
+// column j: the candidates are sorted by partner id, so the entries come out ascending,
+// popc(m) of them (the count in M).
+__device__ __forceinline__ void contacts_from_aligned(const DevArrays& a, int s, uint32_t m, Pool<int32_t> CPd,
+                                                      Pool<float> CFd, int j) {
+    const int cap = (int)a.cap;
+    const int w0 = a.NL[s];
+    int n = 0;
+    for (uint32_t mk = m; mk != 0u; mk &= mk - 1u) {
+        const int k = __ffs(mk) - 1;
+        const int js = k == 0 ? nl_first(w0) : a.NL[k * cap + s];
+        CPd[n * cap + j] = a.ID[js];
+        CFd[n * cap + j] = a.FT[k * cap + s];
+        n++;
     }
 }
 
@@ -544,7 +570,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const uint32_t fl = (uint32_t)lfw.x;
     const int cur = args.cur, nxt = cur ^ 1;
     if (!(fl & SF_ACTIVE)) {
-        copy_list(a, s, cc_of((uint32_t)a.M[s]));  // the lists and velocities flip after every launch
+        copy_list(a, s, (uint32_t)a.M[s]);  // the lists and velocities flip after every launch
         carry_velocity(a, s);
         if (args.flip) a.P[nxt][s] = a.P[cur][s];
         return;
@@ -559,17 +585,13 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const int cap = (int)a.cap;  // KMAX * cap < 2^31 (checked at pool creation)
     // Issued with the own state as well, before the counts that say how many of them are
     // valid are known (rows past a count hold stale words of this slot's column, in
-    // bounds): neighbour-list rows 1-3 and the old contact list's first four entries.
+    // bounds): neighbour-list rows 1-3 and the tangential forces of candidates 0-3.
     int nlr[3];
-    int ow[4];
-    float fw[4];
+    float ftr[4];
 #pragma unroll
     for (int t = 0; t < 3; t++) nlr[t] = a.NL[(t + 1) * cap + s];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-        ow[t] = a.CP[t * cap + s];
-        fw[t] = a.CF[t * cap + s];
-    }
+    for (int t = 0; t < 4; t++) ftr[t] = a.FT[t * cap + s];
     int b_lo = 0, b_hi = 0;
     // bond records: at most four are held in registers (partner ids, alive bits) for
     // the bonded-partner test of the contact search and the merge with the contacts
@@ -627,7 +649,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
 
     if (!act) {
         // never binned into a real cell, never a partner; state kept in both buffers
-        copy_list(a, s, cc_of(pf));
+        copy_list(a, s, pf);
         if ((fl & SF_SIM) && S.record) {
             const float4 d0 = a.DATA0[s];
             a.DATA0[s] = make_float4(0.0f, 0.0f, 0.0f, d0.w);
@@ -649,6 +671,14 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     // overflow) the general path collects the partners first and applies the
     // reference's keep order before forcing them.
     const bool stream = narrow && nc > 0 && nc + (b_hi - b_lo) <= KMAX;
+    // The list path keeps its contacts beside the neighbour list (PF_ALIGNED): bit k of a
+    // mask in the velocity word for candidate k, its tangential force in FT row k, read
+    // and rewritten in place, so that no partner ids are stored and no merge by id runs.
+    // The other paths take and give partner-id lists (CP -> CPn), converting a list
+    // that arrives in the other form first.
+    const bool aligned = (pf & PF_ALIGNED) != 0;
+    bool out_aligned = aligned;     // the form this launch leaves
+    uint32_t vbits = vword(v) & VW_MASK;   // and the mask it leaves in the velocity word
 
     float ax, ay, al;
 
@@ -702,27 +732,33 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
 
         if (stream) {
             if (BONDS) load_records();
-            // The old list (ascending partner ids) through a window of four entries in
-            // registers: ow[0] is the entry under the merge pointer, the window shifts as
-            // it advances and loads the entry four ahead; a hit takes its tangential force
-            // from there.
+            // A partner-id list (the first launch after a list build or after another
+            // path) is read through a window of four entries in registers: ow[0] is the
+            // entry under the merge pointer, the window shifts as it advances and loads
+            // the entry four ahead.  A list beside the neighbour list needs no search.
+            int ow[4], pn = 4;   // pn: old-list index of the next entry to load into the window
+            float fw[4];
+            const uint32_t omask = aligned ? contact_mask(v) : 0u;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                if (t >= oc) {
-                    ow[t] = INT32_MAX;
-                    fw[t] = 0.0f;
+                ow[t] = INT32_MAX;
+                fw[t] = 0.0f;
+                if (!aligned && t < oc) {
+                    ow[t] = a.CP[t * cap + s];
+                    fw[t] = a.CF[t * cap + s];
                 }
             }
-            int pn = 4;   // old-list index of the next entry to load into the window
-            // four candidates at a time: slots, then their positions, velocity records and
-            // ids in one round of loads (before the overlap tests, so that a hit waits on
-            // nothing)
+            uint32_t nmask = 0u;
+            // four candidates at a time: slots, then their positions, velocity records,
+            // ids and tangential forces in one round of loads (before the overlap tests, so
+            // that a hit waits on nothing)
             for (int k0 = 0; k0 < nc; k0 += STEP_ROUND) {
                 // Loads without branches, so that all of them are in flight together: list
                 // rows past the count (rows < NLMAX exist for every slot) and, for a missing
                 // candidate, this slot's own records stand in and the test below drops them.
                 // Rows 0-3 came with the own state.
                 int jj[STEP_ROUND];
+                float fq[STEP_ROUND];
 #pragma unroll
                 for (int t = 0; t < STEP_ROUND; t++) {
                     const int k = k0 + t;
@@ -733,6 +769,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                     else if (k == 3) w = nlr[2];
                     else w = a.NL[min(k, NLMAX - 1) * cap + s];
                     jj[t] = k < nc ? w : -1;
+                    fq[t] = k0 == 0 ? ftr[t] : a.FT[min(k, KMAX - 1) * cap + s];
                 }
                 float4 qq[STEP_ROUND], vq[STEP_ROUND];
                 int iq[STEP_ROUND];   // partner ids: the last word of the position record
@@ -746,51 +783,60 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 // keeps the compiler from sinking any of these loads into the branches
                 // below, which would wait on each candidate in turn
                 asm volatile("" ::: "memory");
-                // this round's new list entries, stored after its bodies: a store issued
+                // this round's tangential forces, stored after its bodies: a store issued
                 // inside them would hold up every later wait on a load (the vector memory
                 // counter counts loads and stores in one queue)
-                int hid[STEP_ROUND];
                 float hft[STEP_ROUND];
                 uint32_t hits = 0u;
 #pragma unroll
                 for (int t = 0; t < STEP_ROUND; t++) {
                     const int j = jj[t];
                     float lj = 0.0f;
-                    hid[t] = 0;
                     hft[t] = 0.0f;
                     if (j < 0 || j == s || !overlaps_l(qq[t], p, lj)) continue;
                     const int oj = iq[t];
                     if (BONDS) {
                         while (bq < b_hi && rec_id(bq) < oj) eval_record(bq++);
                     }
-                    while (ow[0] < oj) {
-                        ow[0] = ow[1]; fw[0] = fw[1];
-                        ow[1] = ow[2]; fw[1] = fw[2];
-                        ow[2] = ow[3]; fw[2] = fw[3];
-                        ow[3] = pn < oc ? a.CP[pn * cap + s] : INT32_MAX;
-                        fw[3] = pn < oc ? a.CF[pn * cap + s] : 0.0f;
-                        pn++;
+                    float ft;
+                    if (aligned) {
+                        ft = ((omask >> (k0 + t)) & 1u) ? fq[t] : 0.0f;
+                    } else {
+                        while (ow[0] < oj) {
+                            ow[0] = ow[1]; fw[0] = fw[1];
+                            ow[1] = ow[2]; fw[1] = fw[2];
+                            ow[2] = ow[3]; fw[2] = fw[3];
+                            ow[3] = pn < oc ? a.CP[pn * cap + s] : INT32_MAX;
+                            fw[3] = pn < oc ? a.CF[pn * cap + s] : 0.0f;
+                            pn++;
+                        }
+                        ft = ow[0] == oj ? fw[0] : 0.0f;
                     }
-                    float ft = ow[0] == oj ? fw[0] : 0.0f;
                     const float4 db = del_from(a, j, vq[t], dt);
                     const float3 kb = partner_k(a, S, j);
                     const float3 f = linear_model<F64>(u.friction_coefficient, me, qq[t], db, kb, me.r + qq[t].z, lj, ft, dd);
                     nfx += f.x;
                     nfy += f.y;
                     nm += f.z;
-                    hid[t] = oj;
                     hft[t] = ft;
                     hits |= 1u << t;
                 }
+                // candidates below nc <= KMAX: rows of FT and bits of the mask exist for them
 #pragma unroll
                 for (int t = 0; t < STEP_ROUND; t++) {
                     if (!((hits >> t) & 1u)) continue;
-                    a.CPn[n * cap + s] = hid[t];
-                    a.CFn[n * cap + s] = hft[t];
+                    a.FT[(k0 + t) * cap + s] = hft[t];
+                    nmask |= 1u << (k0 + t);
                     n++;
                 }
             }
+            out_aligned = true;
+            vbits = nmask << VW_MASK_SHIFT;
         } else {
+            // the other paths read a partner-id list
+            if (aligned) contacts_from_aligned(a, s, contact_mask(v), a.CP, a.CF, s);
+            out_aligned = false;
+            vbits = 0u;
             if (narrow && nc != 0) {
                 // general path: every overlapping partner, from the list or (nc < 0) from the
                 // stencil of the last cell list plus the scene's large bodies
@@ -971,12 +1017,19 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             // launch's drift, or after the next launch's if this one has none); a particle
             // on the stencil fallback (nc < 0) finds the partner in the cells.
             if (dead != 0u || dead_far) {
-                if (dead_far || nc + __popc(dead) > NLMAX) {
+                if (dead_far || nl_count(nlw) + __popc(dead) > NLMAX) {
                     a.RB[0] = 1;
                     a.RB[1] = 1;
-                } else if (nc >= 0) {
-                    int cnt = nc;
-                    int first = nl_first(w0);
+                } else if (nl_count(nlw) >= 0) {
+                    // the insertion renumbers the candidates: contacts kept beside the list
+                    // leave as a partner-id list
+                    if (out_aligned) {
+                        contacts_from_aligned(a, s, vbits >> VW_MASK_SHIFT, a.CPn, a.CFn, s);
+                        out_aligned = false;
+                        vbits = 0u;
+                    }
+                    int cnt = nl_count(nlw);
+                    int first = nl_first(nlw);
                     for (uint32_t mk = dead; mk != 0u; mk &= mk - 1u) {
                         const int oj = a.BR[b_lo + __ffs(mk) - 1].x;
                         const int js = a.INV[S.p0 + oj];
@@ -995,7 +1048,9 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             }
         }
         // single-buffered: written only when it changes
-        if (n != cc_old) a.M[s] = (int)((pf & ~PF_CC_MASK) | ((uint32_t)n << PF_CC_SHIFT));
+        const uint32_t mnew = (pf & ~(PF_CC_MASK | PF_ALIGNED)) | ((uint32_t)n << PF_CC_SHIFT) |
+                              (out_aligned ? PF_ALIGNED : 0u);
+        if (mnew != pf) a.M[s] = (int)mnew;
 
         // 3. store_forces, 2D_Simulation.wgsl:434-478
         const float density = ma_rho;
@@ -1043,7 +1098,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         if (!(fl & SF_LOM)) {
             // no drift follows: the kicked velocity no longer gives the displacement,
             // which is kept explicitly
-            a.Vn[s] = make_float4(v.x, v.y, v.z, __uint_as_float(VW_EXPLICIT | (vword(v) & VW_PATH)));
+            a.Vn[s] = make_float4(v.x, v.y, v.z, __uint_as_float(VW_EXPLICIT | vbits | (vword(v) & VW_PATH)));
             a.DXn[s] = dme;
             a.A[s] = make_float4(ax, ay, al, 0.0f);
             // force pass only, while other scenes drift: positions carry over
@@ -1051,7 +1106,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             return;
         }
     } else {
-        copy_list(a, s, cc_old);
+        copy_list(a, s, pf);
         const float4 av = a.A[s];
         ax = av.x;
         ay = av.y;
@@ -1143,7 +1198,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
         if (mxx + myy > S.trig2) a.RB[nxt] = 1;
         else q = min((uint32_t)ceilf(fminf((fabsf(mx) + fabsf(my)) * S.pscale * 1.0001f, 1.0e9f)), VW_PATH_SAT);
     }
-    a.Vn[s] = make_float4(ivx, ivy, iw, __uint_as_float((expl ? VW_EXPLICIT : 0u) | q));
+    a.Vn[s] = make_float4(ivx, ivy, iw, __uint_as_float((expl ? VW_EXPLICIT : 0u) | vbits | q));
 }
 
 // ---- bond pass: one thread per bond endpoint record ----
@@ -1501,6 +1556,11 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, const SceneD
         e[r] = a.START[S.c0 + ryc * S.w + xa];
         e1[r] = a.START[S.c0 + ryc * S.w + xb + 1];
     }
+    // contacts kept beside the old list leave as a partner-id list before it is replaced
+    if ((uint32_t)mw & PF_ALIGNED) {
+        contacts_from_aligned(a, s, contact_mask(vs), a.CP, a.CF, s);
+        a.M[s] = (int)((uint32_t)mw & ~PF_ALIGNED);
+    }
     a.XR[s] = make_float2(p.x, p.y);
     a.V[s].w = __uint_as_float(__float_as_uint(vs.w) & VW_EXPLICIT);   // path since the build: 0
     if (mw & PF_INACTIVE) {
@@ -1697,7 +1757,7 @@ __global__ void k_reorder_move(ReorderArgs r) {
     r.dst.V[j] = v;
     if (vword(v) & VW_EXPLICIT) r.dst.DX[j] = r.src.DX[s];
     const int2 m = make_int2(r.src.M[s], r.src.ID[s]);
-    r.dst.M[j] = m.x;
+    r.dst.M[j] = (int)((uint32_t)m.x & ~PF_ALIGNED);
     r.dst.ID[j] = m.y;
     r.dst.A[j] = r.src.A[s];
     r.dst.TH[j] = r.src.TH[s];
@@ -1708,9 +1768,15 @@ __global__ void k_reorder_move(ReorderArgs r) {
     }
     const int c = cc_of((uint32_t)m.x);
     const int cap = (int)r.src.cap;
-    for (int q = 0; q < c && q < KMAX; q++) {
-        r.dst.CP[q * cap + j] = r.src.CP[q * cap + s];
-        r.dst.CF[q * cap + j] = r.src.CF[q * cap + s];
+    // the lists arrive as partner-id lists (the re-sort follows a list build); one kept
+    // beside the neighbour list would leave as one
+    if ((uint32_t)m.x & PF_ALIGNED) {
+        contacts_from_aligned(r.src, s, contact_mask(v), r.dst.CP, r.dst.CF, j);
+    } else {
+        for (int q = 0; q < c && q < KMAX; q++) {
+            r.dst.CP[q * cap + j] = r.src.CP[q * cap + s];
+            r.dst.CF[q * cap + j] = r.src.CF[q * cap + s];
+        }
     }
     // the neighbour list, its candidates renamed to their destination slots
     const int w0 = r.src.NL[s];
@@ -1889,6 +1955,19 @@ __global__ void k_group_total(const double* partials, int nblocks, double* out) 
 // order independent.  A grid of COUNT_BLOCKS strides over the scene, so only that
 // many same-address atomics meet in L2.
 constexpr int COUNT_BLOCKS = 256;
+// Slots [p0, p0 + np): every contact list kept beside its neighbour list becomes a
+// partner-id list in CP / CF (for reads from the host; the next step kernel takes
+// either form).
+__global__ void __launch_bounds__(256) k_contacts_by_id(StepArgs args, int p0, int np) {
+    const DevArrays& a = args.a;
+    const int s = p0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= p0 + np) return;
+    const uint32_t mw = (uint32_t)a.M[s];
+    if (!(mw & PF_ALIGNED)) return;
+    contacts_from_aligned(a, s, contact_mask(a.V[s]), a.CP, a.CF, s);
+    a.M[s] = (int)(mw & ~PF_ALIGNED);
+}
+
 __global__ void __launch_bounds__(256) k_count_contacts(const int32_t* m, int p0, int np, unsigned long long* out) {
     __shared__ int s_w[4];
     int c = 0;
@@ -2153,6 +2232,12 @@ hipError_t launch_partition(const ReorderArgs& r, int nblocks, int block, hipStr
 hipError_t launch_kinetic(const DevArrays& a, int cur, int m0, int nm, int p0, int np, double* out, hipStream_t st) {
     const int b = 256;
     hipLaunchKernelGGL(k_kinetic, dim3((np + b - 1) / b), dim3(b), 0, st, a, cur, m0, nm, p0, np, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_contacts_by_id(const StepArgs& args, int p0, int np, hipStream_t st) {
+    const int b = 256;
+    if (np > 0) hipLaunchKernelGGL(k_contacts_by_id, dim3((np + b - 1) / b), dim3(b), 0, st, args, p0, np);
     return hipGetLastError();
 }
 
