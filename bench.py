@@ -41,22 +41,27 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_step_plain_column_collapse_4
 
 
 def measured_traffic(args, cell_size: float):
-    """Bytes per launch of the step kernel from the committed counter summary, for the
-    configuration that summary covers (4,194,304 discs, the settle, block and cell
-    side it records); None otherwise."""
+    """Bytes per launch of the step kernel from the committed counter summary, for a
+    configuration that summary covers (4,194,304 discs, its settle, and one of its
+    rows of block size and cell side); None otherwise."""
     if not (args.particles == 4194304 and args.per_row == 2048 and args.collision_interval == 1):
         return None
     if not os.path.exists(TRAFFIC_PROFILE):
         return None
-    meta = {}
+    settle, rows = None, {}
     for line in open(TRAFFIC_PROFILE):
-        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "block", "cell_size", "settle"):
-            k, v = line.split(":", 1)
-            meta[k] = float(v)
-    if meta.get("block") != args.block or abs(meta.get("cell_size", -1.0) - cell_size) > 1e-6 or \
-            meta.get("settle") != args.settle:
+        w = line.split()
+        if len(w) == 2 and w[0] == "settle:":
+            settle = int(w[1])
+        # "traffic block B cell_size C: bytes"
+        if len(w) == 6 and w[0] == "traffic" and w[1] == "block" and w[3] == "cell_size":
+            rows[(int(w[2]), float(w[4].rstrip(":")))] = float(w[5])
+    if settle != args.settle:
         return None
-    return meta.get("traffic_bytes_per_launch")
+    for (b, c), v in rows.items():
+        if b == args.block and abs(c - cell_size) <= 1e-6:
+            return v
+    return None
 
 
 def parse():

@@ -1556,21 +1556,23 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, const SceneD
         e[r] = a.START[S.c0 + ryc * S.w + xa];
         e1[r] = a.START[S.c0 + ryc * S.w + xb + 1];
     }
-    // contacts kept beside the old list leave as a partner-id list before it is replaced
-    if ((uint32_t)mw & PF_ALIGNED) {
-        contacts_from_aligned(a, s, contact_mask(vs), a.CP, a.CF, s);
+    // contacts kept beside the old list (PF_ALIGNED): moved beside the new one below, or
+    // left as a partner-id list where that cannot hold them
+    const bool al = ((uint32_t)mw & PF_ALIGNED) != 0;
+    const uint32_t om = al ? contact_mask(vs) : 0u;
+    const int w0old = a.NL[s];
+    auto leave_aligned = [&]() {
+        contacts_from_aligned(a, s, om, a.CP, a.CF, s);
         a.M[s] = (int)((uint32_t)mw & ~PF_ALIGNED);
-    }
+    };
     a.XR[s] = make_float2(p.x, p.y);
-    a.V[s].w = __uint_as_float(__float_as_uint(vs.w) & VW_EXPLICIT);   // path since the build: 0
-    if (mw & PF_INACTIVE) {
-        a.NL[s] = nl_word(0, 0);
-        return;
-    }
-    if (mw & PF_LARGE) {
+    const uint32_t vw0 = __float_as_uint(vs.w) & VW_EXPLICIT;   // path since the build: 0
+    if (mw & (PF_INACTIVE | PF_LARGE)) {
+        if (al) leave_aligned();
+        a.V[s].w = __uint_as_float(vw0);
         // a large body searches the stencil of the cell list every step
-        a.NL[s] = nl_word(-1, 0);
-        atomicAdd(&a.STAT[5], 1ull);
+        a.NL[s] = (mw & PF_INACTIVE) ? nl_word(0, 0) : nl_word(-1, 0);
+        if (!(mw & PF_INACTIVE)) atomicAdd(&a.STAT[5], 1ull);
         return;
     }
 #pragma unroll
@@ -1646,7 +1648,27 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, const SceneD
         }
     }
     int first = 0;
+    uint32_t nm = 0u;   // contacts moved beside the new list: bits of their candidates
     if (cnt <= NLMAX) {
+        // the old contacts' candidate slots and forces (the first OLD_AHEAD of them),
+        // loaded now so that they arrive during the sort
+        constexpr int OLD_AHEAD = 6;
+        int ojs[OLD_AHEAD];
+        float oft[OLD_AHEAD];
+        {
+            uint32_t mk = (al && cnt <= KMAX) ? om : 0u;
+#pragma unroll
+            for (int u = 0; u < OLD_AHEAD; u++) {
+                ojs[u] = -1;
+                oft[u] = 0.0f;
+                if (mk != 0u) {
+                    const int k = __ffs(mk) - 1;
+                    ojs[u] = k == 0 ? nl_first(w0old) : a.NL[k * cap + s];
+                    oft[u] = a.FT[k * cap + s];
+                    mk &= mk - 1u;
+                }
+            }
+        }
         // insertion sort of this thread's entries by partner id (ids are unique)
         for (int k = 1; k < cnt; k++) {
             const int2 x = lst[k * lstride];
@@ -1658,9 +1680,57 @@ __device__ __forceinline__ void build_nl_slot(const StepArgs& args, const SceneD
             }
             lst[(m + 1) * lstride] = x;
         }
+        // Old and new candidates are both in partner-id order, so one forward walk over
+        // the new list finds each old contact's slot; its tangential force goes into
+        // that entry's id word (ids are not needed once sorted).  Every old row is read
+        // before any row is written.  An old contact that is no longer a candidate, or a
+        // list longer than the mask, leaves the contacts as a partner-id list.
+        bool keep = al && cnt <= KMAX;
+        if (keep) {
+            int kp = 0, u = 0;
+            for (uint32_t mk = om; mk != 0u; mk &= mk - 1u, u++) {
+                int js;
+                float f;
+                if (u < OLD_AHEAD) {
+                    // selects, not an indexed register array (which would live in scratch)
+                    js = ojs[0];
+                    f = oft[0];
+#pragma unroll
+                    for (int v = 1; v < OLD_AHEAD; v++) {
+                        if (u == v) {
+                            js = ojs[v];
+                            f = oft[v];
+                        }
+                    }
+                } else {
+                    const int k = __ffs(mk) - 1;
+                    js = k == 0 ? nl_first(w0old) : a.NL[k * cap + s];
+                    f = a.FT[k * cap + s];
+                }
+                while (kp < cnt && lst[kp * lstride].y != js) kp++;
+                if (kp >= cnt) {
+                    keep = false;
+                    break;
+                }
+                lst[kp * lstride].x = __float_as_int(f);
+                nm |= 1u << kp;
+                kp++;
+            }
+        }
+        if (al && !keep) {
+            leave_aligned();
+            nm = 0u;
+        }
         for (int k = 1; k < cnt; k++) a.NL[k * cap + s] = lst[k * lstride].y;
+        for (uint32_t mk = nm; mk != 0u; mk &= mk - 1u) {
+            const int k = __ffs(mk) - 1;
+            a.FT[k * cap + s] = __int_as_float(lst[k * lstride].x);
+        }
         if (cnt > 0) first = lst[0].y;
+    } else if (al) {
+        leave_aligned();
     }
+    a.V[s].w = __uint_as_float(vw0 | (nm << VW_MASK_SHIFT));
     a.NL[s] = nl_word(cnt <= NLMAX ? cnt : -1, first);
     if (cnt > NLMAX) atomicAdd(&a.STAT[5], 1ull);   // particles on the stencil fallback
 }
