@@ -1756,12 +1756,13 @@ __global__ void k_rb_scatter(StepArgs args, int nlb) {
 // The lists are built in cell-list order (thread e takes the slot at list position e),
 // so that a wave's stencils overlap whatever the storage order: the re-sort moves
 // candidates to the front of their units, out of cell order.
-// The list build's LDS: NLMAX {id, slot} entries per thread of a block of at most 256.
+// The list build's LDS: NLMAX {id, slot} entries per thread (dynamic, sized by the
+// launch: 8 KB at 64 threads to 32 KB at the largest block, 256).
 constexpr int NL_LDS_THREADS = 256;
 
 __global__ void __launch_bounds__(NL_LDS_THREADS) k_build_nl(StepArgs args, int nlb) {
     if (args.a.RB[args.cur] == 0) return;
-    __shared__ int2 nl_lds[NLMAX * NL_LDS_THREADS];
+    extern __shared__ int2 nl_lds[];
     const int lbs = (int)blockDim.x;
     // the scene's list is a permutation of its slots: list position and slot share the
     // scene.  One live scene: its record is read into registers before any store (read
@@ -2194,6 +2195,13 @@ static void launch_k(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t ev0, h
     if (ev0) hipExtLaunchKernelGGL(kernel, g, b, 0, st, ev0, ev1, 0, args...);
     else hipLaunchKernelGGL(kernel, g, b, 0, st, args...);
 }
+// the same with `lds` bytes of dynamic LDS per workgroup
+template <class K, class... A>
+static void launch_k_lds(K kernel, dim3 g, dim3 b, size_t lds, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
+                         A... args) {
+    if (ev0) hipExtLaunchKernelGGL(kernel, g, b, lds, st, ev0, ev1, 0, args...);
+    else hipLaunchKernelGGL(kernel, g, b, lds, st, args...);
+}
 
 template <class K>
 static void launch_step_kernel(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
@@ -2265,7 +2273,8 @@ hipError_t launch_rb_scatter(const StepArgs& args, int nlb, int grid, int block,
 
 hipError_t launch_build_nl(const StepArgs& args, int nlb, int grid, int block, hipStream_t st, hipEvent_t ev0,
                            hipEvent_t ev1) {
-    launch_k(k_build_nl, dim3(grid), dim3(block), st, ev0, ev1, args, nlb);
+    if (block > NL_LDS_THREADS) return hipErrorInvalidValue;
+    launch_k_lds(k_build_nl, dim3(grid), dim3(block), sizeof(int2) * NLMAX * (size_t)block, st, ev0, ev1, args, nlb);
     return hipGetLastError();
 }
 
