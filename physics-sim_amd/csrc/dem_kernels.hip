@@ -552,6 +552,8 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
 // Candidates of the list path per round of loads (their slots, position and velocity
 // records in flight together).
 constexpr int STEP_ROUND = 4;
+// the rows loaded ahead (1-3 with the own state, 4-7 with the first round) follow rounds of four
+static_assert(STEP_ROUND == 4, "rounds of four candidates");
 
 template <bool F64, bool BONDS, bool LARGE>
 __device__ __forceinline__ void step_slot(const StepArgs& args) {
@@ -749,6 +751,20 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                 }
             }
             uint32_t nmask = 0u;
+            // List rows 4-7 and their forces, loaded with the first round's records where
+            // the list has more than four candidates, so that the second round waits on its
+            // records only (in the plain f32 kernel: the bonded, large-body and f64 variants
+            // have no registers to spare).
+            constexpr bool ROWS_AHEAD = !BONDS && !F64 && !LARGE;
+            int nl4[4] = {0, 0, 0, 0};
+            float ft4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (ROWS_AHEAD && nc > 4) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    nl4[t] = a.NL[(4 + t) * cap + s];
+                    ft4[t] = a.FT[min(4 + t, KMAX - 1) * cap + s];
+                }
+            }
             // four candidates at a time: slots, then their positions, velocity records,
             // ids and tangential forces in one round of loads (before the overlap tests, so
             // that a hit waits on nothing)
@@ -767,9 +783,10 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                     else if (k == 1) w = nlr[0];
                     else if (k == 2) w = nlr[1];
                     else if (k == 3) w = nlr[2];
+                    else if (ROWS_AHEAD && k0 == 4) w = nl4[t];
                     else w = a.NL[min(k, NLMAX - 1) * cap + s];
                     jj[t] = k < nc ? w : -1;
-                    fq[t] = k0 == 0 ? ftr[t] : a.FT[min(k, KMAX - 1) * cap + s];
+                    fq[t] = k0 == 0 ? ftr[t] : ((ROWS_AHEAD && k0 == 4) ? ft4[t] : a.FT[min(k, KMAX - 1) * cap + s]);
                 }
                 float4 qq[STEP_ROUND], vq[STEP_ROUND];
                 int iq[STEP_ROUND];   // partner ids: the last word of the position record
