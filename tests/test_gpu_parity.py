@@ -122,6 +122,36 @@ def test_granular_contacts_dense_bit_exact():
     prog.close()
 
 
+def test_reads_between_calls_keep_contact_history():
+    """Contacts of the list path are kept beside the neighbour list on the device (a mask
+    and one tangential force per candidate); every read from the host writes them back
+    as partner-id lists, which the next step merges into the list form again.  Ten calls
+    of five steps with the state and the contact table read after each equal one call of
+    fifty steps and the oracle bit for bit, contact table included (the compressed box of
+    the test above, a list rebuild every few steps and a re-sort every 7)."""
+    s, st = scenes.granular_box(n_side=32, seed=5, spacing=1.7)
+    n = st.p_count
+    a = ComputeProg(s, st.copy(), reorder_interval=7)
+    b = ComputeProg(s, st.copy(), reorder_interval=7)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL, slots=16)
+    for call in range(10):
+        a.step(5)
+        eng.step(5)
+        got = a.update_state()
+        assert_bits_equal(got, eng.st, n)
+        want = eng.canonical_contacts().view(np.int32).reshape(-1, 16, 6)[:n, :14]
+        assert np.array_equal(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want), call
+        c, p = a.debug_contacts(16)
+        assert int(c[:n].sum()) == int((want[:, :, 1] >= 0).sum()), call
+    b.step(50)
+    one = b.update_state()
+    assert_bits_equal(one, got, n)
+    assert np.array_equal(one.contacts.view(np.int32), got.contacts.view(np.int32))
+    assert a.counters()["contacts"] > n // 2
+    a.close()
+    b.close()
+
+
 def test_stage_views_match_closed_forms():
     """Stages 1-4: cell keys, (cell, id) order, cell starts and contact pairs equal
     the oracle's closed forms / brute force."""
