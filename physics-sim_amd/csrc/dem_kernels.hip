@@ -587,13 +587,15 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const int cap = (int)a.cap;  // KMAX * cap < 2^31 (checked at pool creation)
     // Issued with the own state as well, before the counts that say how many of them are
     // valid are known (rows past a count hold stale words of this slot's column, in
-    // bounds): neighbour-list rows 1-3 and the tangential forces of candidates 0-3.
+    // bounds): neighbour-list rows 1-3 and the tangential forces of candidates 0-3 (the
+    // latter not in the bonded variant, whose discs hold few candidates besides their
+    // bonds: there they come with the first round's records).
     int nlr[3];
     float ftr[4];
 #pragma unroll
     for (int t = 0; t < 3; t++) nlr[t] = a.NL[(t + 1) * cap + s];
 #pragma unroll
-    for (int t = 0; t < 4; t++) ftr[t] = a.FT[t * cap + s];
+    for (int t = 0; t < 4; t++) ftr[t] = BONDS ? 0.0f : a.FT[t * cap + s];
     int b_lo = 0, b_hi = 0;
     // bond records: at most four are held in registers (partner ids, alive bits) for
     // the bonded-partner test of the contact search and the merge with the contacts
@@ -614,10 +616,15 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     auto load_records = [&]() {
         if (!BONDS || rloaded) return;
         rloaded = true;
+        if (b_hi <= b_lo) return;
+        // loads without branches (indices past the range repeat its last record)
+        int w[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) w[k] = __float_as_int(a.RF[min(b_lo + k, b_hi - 1)].w);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (b_lo + k < b_hi) {
-                rx[k] = __float_as_int(a.RF[b_lo + k].w);
+                rx[k] = w[k];
                 if (rx[k] >= 0) ralive |= 1 << k;
             }
         }
@@ -786,7 +793,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                     else if (ROWS_AHEAD && k0 == 4) w = nl4[t];
                     else w = a.NL[min(k, NLMAX - 1) * cap + s];
                     jj[t] = k < nc ? w : -1;
-                    fq[t] = k0 == 0 ? ftr[t] : ((ROWS_AHEAD && k0 == 4) ? ft4[t] : a.FT[min(k, KMAX - 1) * cap + s]);
+                    fq[t] = (!BONDS && k0 == 0) ? ftr[t] : ((ROWS_AHEAD && k0 == 4) ? ft4[t] : a.FT[min(k, KMAX - 1) * cap + s]);
                 }
                 float4 qq[STEP_ROUND], vq[STEP_ROUND];
                 int iq[STEP_ROUND];   // partner ids: the last word of the position record
@@ -1004,14 +1011,17 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
             }
         }
         if (BONDS) {
-            // the records after the last contact, four results in flight at a time
+            // the records after the last contact, eight results in flight at a time (loads
+            // without branches: indices past the range repeat its last record, which the
+            // count then drops)
             while (bq < b_hi) {
-                const int m = min(b_hi - bq, 4);
-                const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                const float4 rf[4] = {a.RF[bq], m > 1 ? a.RF[bq + 1] : z, m > 2 ? a.RF[bq + 2] : z,
-                                      m > 3 ? a.RF[bq + 3] : z};
+                constexpr int RR = 8;
+                const int m = min(b_hi - bq, RR);
+                float4 rf[RR];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
+                for (int k = 0; k < RR; k++) rf[k] = a.RF[bq + min(k, m - 1)];
+#pragma unroll
+                for (int k = 0; k < RR; k++) {
                     if (k < m && __float_as_int(rf[k].w) >= 0) {
                         nfx += rf[k].x;
                         nfy += rf[k].y;
