@@ -587,15 +587,13 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
     const int cap = (int)a.cap;  // KMAX * cap < 2^31 (checked at pool creation)
     // Issued with the own state as well, before the counts that say how many of them are
     // valid are known (rows past a count hold stale words of this slot's column, in
-    // bounds): neighbour-list rows 1-3 and the tangential forces of candidates 0-3 (the
-    // latter not in the bonded variant, whose discs hold few candidates besides their
-    // bonds: there they come with the first round's records).
+    // bounds): neighbour-list rows 1-3 and the tangential forces of candidates 0-3.
     int nlr[3];
     float ftr[4];
 #pragma unroll
     for (int t = 0; t < 3; t++) nlr[t] = a.NL[(t + 1) * cap + s];
 #pragma unroll
-    for (int t = 0; t < 4; t++) ftr[t] = BONDS ? 0.0f : a.FT[t * cap + s];
+    for (int t = 0; t < 4; t++) ftr[t] = a.FT[t * cap + s];
     int b_lo = 0, b_hi = 0;
     // bond records: at most four are held in registers (partner ids, alive bits) for
     // the bonded-partner test of the contact search and the merge with the contacts
@@ -793,7 +791,7 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
                     else if (ROWS_AHEAD && k0 == 4) w = nl4[t];
                     else w = a.NL[min(k, NLMAX - 1) * cap + s];
                     jj[t] = k < nc ? w : -1;
-                    fq[t] = (!BONDS && k0 == 0) ? ftr[t] : ((ROWS_AHEAD && k0 == 4) ? ft4[t] : a.FT[min(k, KMAX - 1) * cap + s]);
+                    fq[t] = k0 == 0 ? ftr[t] : ((ROWS_AHEAD && k0 == 4) ? ft4[t] : a.FT[min(k, KMAX - 1) * cap + s]);
                 }
                 float4 qq[STEP_ROUND], vq[STEP_ROUND];
                 int iq[STEP_ROUND];   // partner ids: the last word of the position record
@@ -2142,8 +2140,9 @@ __global__ void __launch_bounds__(64) k_large_search(StepArgs args) {
                 const int row_incl = __shfl(incl, lo, 64);
                 const int row_len = __shfl(len, lo, 64);
                 const int row_e0 = __shfl(e0, lo, 64);
-                const int e = row_e0 + tc - (row_incl - row_len);
-                jj[u] = t < total ? a.SIDX[e] : -1;
+                const int e = row_e0 + tc - (row_incl - row_len);   // in range for every lane
+                const int je = a.SIDX[e];
+                jj[u] = t < total ? je : -1;
             }
             asm volatile("" ::: "memory");
             float4 q[LS_K];
