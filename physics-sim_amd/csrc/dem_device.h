@@ -32,6 +32,7 @@ __host__ __device__ inline int nl_count(int w) {
 __host__ __device__ inline int nl_first(int w) { return w & ((1 << NL_SHIFT) - 1); }
 constexpr int DATA_WORDS = 7;
 constexpr int REC_DEAD = INT32_MIN;   // bonded word of a dead bond record in BRB
+constexpr int REC_GONE = INT32_MIN + 1;   // dead, and its result already says so (RF.w = REC_RF_DEAD)
 // last word of a bond-pass result (RF.w, int bits): the partner id of a live record, or
 constexpr int REC_RF_DEAD = -1;       // a dead record (no force)
 constexpr int REC_RF_DIED = -2;       // a record that died in this launch (its partner joins the owner's list)
@@ -126,7 +127,7 @@ struct SceneDev {
     int32_t n_real;        // particles of the scene; slots [p0+n_real, p0+np) hold inactive padding
     float skin;            // neighbour-list skin: cell side minus (2 r_small + rounding margin)
     float trig2;           // squared displacement since the list build that forces a rebuild
-    float pscale;          // 2^24 / (0.45 skin): path units per unit length (1e30 for no skin)
+    float pscale;          // 2^14 / (0.45 skin): path units per unit length (1e30 for no skin)
     float rsmax;           // largest radius of the discs that are not large bodies
     int32_t lb0, nlb;      // large bodies: particle ids LB[lb0, lb0 + nlb)
     float rho1, kn1, ks1;  // density and stiffnesses of material 0 (the whole scene when nm == 1)
@@ -195,7 +196,7 @@ struct DevArrays {
     // by partner id; records never move
     Pool<int2> BOFF;        // [cap] record range [x, y) of (scene p0 + particle id)
     Pool<int4> BR;          // {partner id, bond id, bonded, alive}
-    Pool<int2> BRB;         // the bond pass's copy: {bond id, bonded, or REC_DEAD once dead}
+    Pool<int2> BRB;         // the bond pass's copy: {bond id, bonded, or REC_DEAD once dead, REC_GONE after}
     Pool<float4> BF;        // {ft, fbt, theta_b, 0}
     Pool<int2> BS;          // per bond {index word, step of break} (download and counts)
     Pool<float4> BLS;       // per bond {rest length, normal strength, shear strength, break word bits}

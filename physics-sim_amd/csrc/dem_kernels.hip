@@ -462,12 +462,24 @@ struct RecIn {
 };
 __device__ __forceinline__ void load_record(const DevArrays& a, const SceneDev& S, int cur, int q, int j, RecIn& in) {
     in.rb = a.BRB[q];
-    if (in.rb.y != REC_DEAD) {
+    if (in.rb.y != REC_DEAD && in.rb.y != REC_GONE) {
         in.bf = a.BF[q];
         in.pb = a.P[cur][j];
         in.vb = a.V[j];
         in.bl = a.BLS[S.b0 + min(max(in.rb.x, 0), max(S.nb - 1, 0))];
     }
+}
+
+// A dead record costs the bond pass its record word only: the launch after its death
+// writes its result as dead once (the death launch left REC_RF_DIED there, which its
+// owner has read) and marks it gone; a gone record is skipped.  Returns whether the
+// record at q (record word rb) is dead.
+__device__ __forceinline__ bool settle_dead_record(const DevArrays& a, int q, int2 rb) {
+    if (rb.y == REC_GONE) return true;
+    if (rb.y != REC_DEAD) return false;
+    a.RF[q] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(REC_RF_DEAD));
+    a.BRB[q].y = REC_GONE;
+    return true;
 }
 
 // One bond endpoint record of the particle `me` (position p, scene S; partner in slot j) in a launch
@@ -483,7 +495,7 @@ __device__ __forceinline__ float4 eval_bond_record(const DevArrays& a, const Sce
                                                    const RecIn& in, DD& dd) {
     const dem_uniform& u = S.u;
     const int2 rb = in.rb;
-    if (rb.y == REC_DEAD) return make_float4(0.0f, 0.0f, 0.0f, __int_as_float(REC_RF_DEAD));
+    if (rb.y == REC_DEAD || rb.y == REC_GONE) return make_float4(0.0f, 0.0f, 0.0f, __int_as_float(REC_RF_DEAD));
     float4 bf = in.bf;
     const float4 pb = in.pb;
     const int pid = __float_as_int(pb.w);   // the partner's id travels with its position
@@ -1287,7 +1299,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= nrec) return;
     const int i = a.RS[q];   // owner slot (-1: no record)
+    const int2 rb = a.BRB[q];
     if (i < 0) return;
+    if (settle_dead_record(a, q, rb)) return;
     const int j = a.RP[q];   // partner slot
     const int sid = args.blk_scene[i / lbs];
     if (sid < 0) return;
@@ -1315,13 +1329,19 @@ __global__ void __launch_bounds__(256) k_bond_forces1(StepArgs args, int nrec) {
     if (q >= nrec) return;
     const int i = a.RS[q];   // owner slot (-1: no record)
     const int j = a.RP[q];   // partner slot
+    const int2 rb = a.BRB[q];
     if (i < 0) return;
+    if (settle_dead_record(a, q, rb)) return;
     const int cur = args.cur;
     const uint32_t mi = (uint32_t)a.M[i];
     const float4 p = a.P[cur][i];
     const float4 vi = a.V[i];
     RecIn in;
-    load_record(a, S, cur, q, j, in);
+    in.rb = rb;
+    in.bf = a.BF[q];
+    in.pb = a.P[cur][j];
+    in.vb = a.V[j];
+    in.bl = a.BLS[S.b0 + min(max(rb.x, 0), max(S.nb - 1, 0))];
     asm volatile("" ::: "memory");   // the loads above go out together
     if (mi & PF_INACTIVE) return;   // an inactive owner evaluates nothing
     bond_pass_body<false>(args, S, q, i, j, lfw, p, vi, mi, in);
