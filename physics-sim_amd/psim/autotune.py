@@ -56,16 +56,18 @@ TIE = 0.02
 
 def tune(settings: Settings, state: State, blocks: Iterable[int] = BLOCKS,
          factors: Iterable[float] = CELL_FACTORS, steps: int = 60, warmup: int = 5) -> Dict:
-    """The fastest (block, cell factor); among configurations within TIE of it, the
-    default cell (factor 1), then the largest block, then the faster, so that
-    near-equal timings (a small scene's step is bound by its launches) pick the same one."""
+    """The cell factor: the default cell (factor 1) when a configuration with it is
+    within TIE of the fastest (in a batch one scene's thin skin rebuilds every scene's
+    lists), else the fastest configuration's; then the fastest block at that cell (the
+    block changes only the speed, never the results)."""
     table = {}
     for b in blocks:
         for f in factors:
             table[(b, f)] = time_config(settings, state, b, f, steps, warmup)
     best = min(table.values())
     ties = [kv for kv in table.items() if kv[1] <= best * (1.0 + TIE)]
-    (b, f), ms = max(ties, key=lambda kv: (kv[0][1] == 1.0, kv[0][0], -kv[1]))
+    f = 1.0 if any(k[1] == 1.0 for k, _ in ties) else min(ties, key=lambda kv: kv[1])[0][1]
+    (b, f), ms = min(((k, v) for k, v in table.items() if k[1] == f), key=lambda kv: kv[1])
     return {"block": b, "cell_factor": f, "cell_size": cell_size_for(settings, state, f), "ms_per_step": ms,
             "table": {"%d/%.2f" % k: v for k, v in table.items()}}
 
