@@ -13,9 +13,11 @@
 //              rebuild flag when the particle has moved more than 0.45 skin since the
 //              rebuild (a path bound in the velocity word gates the exact test)
 //              (stages 2a, 4, 5).  Per disc it streams P (x, y, r, id), TH (rotation) and V
-//              (velocity, path word) in and out and reads the flag word and the list
-//              word; the drift's displacement is re-formed as velocity x dT unless an
-//              explicit record DX holds it.  The first launch of a dem_step call runs
+//              (velocity, contact mask, path word) in and out and reads the flag word and
+//              the list word; the contacts' tangential forces sit beside the neighbour
+//              list (FT row k for candidate k), read and rewritten in place; the drift's
+//              displacement is re-formed as velocity x dT unless an explicit record DX
+//              holds it.  The first launch of a dem_step call runs
 //              only the LOM half and the last one only the Sim half, so the state
 //              between calls is the reference's post-Sim state.
 //   neighbour-list rebuild, after every drift, each kernel a no-op unless the flag
