@@ -152,6 +152,38 @@ def test_reads_between_calls_keep_contact_history():
     b.close()
 
 
+@pytest.mark.parametrize("reorder", [11, -1])
+def test_lists_around_the_slot_limit_keep_contact_history(reorder):
+    """Cells 1.9 times the default: about two-thirds of the neighbour lists fit the 14
+    slots (the list path, contacts kept beside the list), the rest take the general path
+    (15-16 candidates) or the stencil fallback (more); with discs thrown at ~150 units/s
+    the lists are rebuilt every few dozen steps, by re-sorts (every 11 steps) or by the
+    displacement check alone (no re-sorts), and discs cross between the paths at the
+    rebuilds (contacts moved beside the new list, or turned back into partner-id lists
+    and merged again).  Bit-exact with the oracle after each of three 50-step calls,
+    contact table included."""
+    s, st = scenes.granular_box(n_side=32, seed=5, spacing=1.7)
+    n = st.p_count
+    st.vel[: 2 * n] = np.random.default_rng(5).normal(0.0, 150.0, 2 * n).astype(np.float32)
+    g = ComputeProg.default_grid(s, st)
+    grid = ComputeProg.default_grid(s, st, cell_size=1.9 * g.cell_size)
+    prog = ComputeProg(s, st.copy(), grid=grid, reorder_interval=reorder)
+    eng = orc.OracleEngine(st, s, mode=orc.MODE_CANONICAL, slots=16)
+    for call in range(3):
+        prog.step(50)
+        eng.step(50)
+        got = prog.update_state()
+        assert_bits_equal(got, eng.st, n)
+        want = eng.canonical_contacts().view(np.int32).reshape(-1, 16, 6)[:n, :14]
+        assert np.array_equal(got.contacts.view(np.int32).reshape(-1, 14, 6)[:n], want), call
+    c = prog.counters()
+    assert c["list_overflow"] > 0 and c["broad_phases"] >= 3 and c["contacts"] > 0
+    assert c["contact_overflow"] == 0
+    if reorder < 0:
+        assert c["reorders"] == 0
+    prog.close()
+
+
 def test_stage_views_match_closed_forms():
     """Stages 1-4: cell keys, (cell, id) order, cell starts and contact pairs equal
     the oracle's closed forms / brute force."""
