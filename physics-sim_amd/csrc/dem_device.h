@@ -96,9 +96,10 @@ constexpr uint32_t PF_CC_MASK = 0x1fu << PF_CC_SHIFT;
 __host__ __device__ inline int cc_of(uint32_t mx) { return (int)((mx & PF_CC_MASK) >> PF_CC_SHIFT); }
 // Bit 30 of M: the contact list is held beside the neighbour list (the contact mask in
 // the velocity word, VW_MASK, and the tangential forces in FT rows of the candidates)
-// instead of in CP / CF.  Only the list path of the step kernel writes this form; the
-// list build, the re-sort, the other paths and reads from the host turn it back into
-// the partner-id list (contacts_from_aligned).
+// instead of in CP / CF.  The list path of the step kernel writes this form and the list
+// build moves it beside the new list; the re-sort, the other paths, reads from the host
+// and a list build that cannot hold it turn it back into the partner-id list
+// (contacts_from_aligned).
 constexpr uint32_t PF_ALIGNED = 1u << 30;
 
 // per-step flags (one word per scene and step)

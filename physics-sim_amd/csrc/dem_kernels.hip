@@ -753,10 +753,11 @@ __device__ __forceinline__ void step_slot(const StepArgs& args) {
 
         if (stream) {
             if (BONDS) load_records();
-            // A partner-id list (the first launch after a list build or after another
-            // path) is read through a window of four entries in registers: ow[0] is the
-            // entry under the merge pointer, the window shifts as it advances and loads
-            // the entry four ahead.  A list beside the neighbour list needs no search.
+            // A partner-id list (the first launch after a re-sort, a read from the host,
+            // another path, or a list build that could not keep the contacts beside the
+            // new list) is read through a window of four entries in registers: ow[0] is
+            // the entry under the merge pointer, the window shifts as it advances and
+            // loads the entry four ahead.  A list beside the neighbour list needs no search.
             int ow[4], pn = 4;   // pn: old-list index of the next entry to load into the window
             float fw[4];
             const uint32_t omask = aligned ? contact_mask(v) : 0u;
